@@ -1,0 +1,69 @@
+"""Case set-ups producing the initial cell types and primitive fields.
+
+Each set-up returns ghost-inclusive (nx+2, ny+2, nz+2) arrays in the
+reference's index order (i, j, k), k fastest, as LBM::LBM allocates them
+(lbm-main.cpp:16-35).  Fields that a set-up does not touch keep the MIXTURE
+defaults of lbm.hpp:56-88 (rho = 1, u = v = w = 0, p = T = 1/3, TYPE_F).
+"""
+import math
+
+import numpy as np
+
+TYPE_F, TYPE_S, TYPE_P, TYPE_A, TYPE_FS, TYPE_I, TYPE_O, TYPE_I_C, TYPE_O_C, TYPE_Q = range(10)
+
+
+def default_fields(nx, ny, nz):
+    shape = (nx + 2, ny + 2, nz + 2)
+    return {
+        "type": np.full(shape, TYPE_F, np.uint8),
+        "rho": np.ones(shape),
+        "u": np.zeros(shape),
+        "v": np.zeros(shape),
+        "w": np.zeros(shape),
+        "p": np.full(shape, 1.0 / 3.0),
+        "T": np.full(shape, 1.0 / 3.0),
+    }
+
+
+def periodic_ghosts(fields):
+    t = fields["type"]
+    t[0, :, :] = TYPE_P
+    t[-1, :, :] = TYPE_P
+    t[:, 0, :] = TYPE_P
+    t[:, -1, :] = TYPE_P
+    t[:, :, 0] = TYPE_P
+    t[:, :, -1] = TYPE_P
+
+
+def tgv_params(N, Re=1600.0, Ma=0.1, R=1.0 / 3.0, T0=1.0, p0=1.0 / 3.0, gamma=1.4, Pr=0.71):
+    """Single-component 3-D Taylor-Green parameters: U0 = Ma sqrt(gamma R T0),
+    nu = U0 L / Re with L = N / (2 pi)."""
+    U0 = Ma * math.sqrt(gamma * R * T0)
+    nu = U0 * (N / (2.0 * math.pi)) / Re
+    return dict(N=N, U0=U0, nu=nu, R=R, gamma=gamma, Pr=Pr, T0=T0, p0=p0)
+
+
+def taylor_green_3d(N, U0, p0, T0, **_unused):
+    """3-D Taylor-Green vortex of setup.cpp:94-148 on an N^3 periodic box.
+
+    The analytic fields are evaluated on the ghost-inclusive index i (the
+    reference's set-up does the same), ghosts are TYPE_P.
+    """
+    fl = default_fields(N, N, N)
+    periodic_ghosts(fl)
+    n = N + 2
+    idx = np.arange(n, dtype=np.float64)
+    a = float(N)
+    x = 2.0 * math.pi / a * idx
+    I, J, K = np.meshgrid(x, x, x, indexing="ij")
+    I2, J2, K2 = np.meshgrid(2 * x, 2 * x, 2 * x, indexing="ij")
+    m = fl["type"] == TYPE_F
+    u = U0 * np.sin(I) * np.cos(J) * np.cos(K)
+    v = -U0 * np.cos(I) * np.sin(J) * np.cos(K)
+    p = p0 + 1.0 * U0 * U0 / 16.0 * (np.cos(I2) + np.cos(J2)) * (np.cos(K2) + 2.0)
+    fl["u"][m] = u[m]
+    fl["v"][m] = v[m]
+    fl["w"][m] = 0.0
+    fl["p"][m] = p[m]
+    fl["T"][m] = T0
+    return fl

@@ -1,0 +1,316 @@
+/*
+ * CPU oracle, single-component path.  TEST INFRASTRUCTURE ONLY: the checker
+ * of the HIP kernels, never part of the product (see lbm_oracle.h).
+ */
+#include "lbm_oracle.h"
+#include <math.h>
+#include <stddef.h>
+
+/* D3Q27 velocity set, order and weights of lbm.hpp:49-53. */
+static const int CX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+static const int CY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+static const int CZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+static const int OPP[27] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 20, 19, 22, 21, 24, 23, 26, 25};
+
+static int cdir(int l, int d) { return d == 0 ? CX[l] : (d == 1 ? CY[l] : CZ[l]); }
+
+/* One-axis factor of the product-form equilibrium, lbm.hpp:282-301:
+ * Psi(0) = 1 - P, Psi(+-1) = (+-u + P)/2 with P = theta + u^2 + corr. */
+static double psi1(int c, double u, double P)
+{
+    if (c == 0) return 1.0 - P;
+    if (c == 1) return 0.5 * (u + P);
+    return 0.5 * (P - u);
+}
+
+/* calculate_feq, lbm.hpp:276-305. */
+double orc_feq(int l, double rho, const double vel[3], double theta, const double corr[3])
+{
+    double r = rho;
+    for (int d = 0; d < 3; ++d) {
+        double P = theta + vel[d] * vel[d] + corr[d];
+        r *= psi1(cdir(l, d), vel[d], P);
+    }
+    return r;
+}
+
+/* calculate_geq(l, rho, U, theta, v), lbm-edf.cpp:38-98.
+ *
+ * The reference spells out 27 expanded polynomials.  They are exactly
+ *   g/rho = (U - 3 theta/2) Px Py Pz + 1/2 sum_d (O_d^2 P_d) prod_{e!=d} P_e
+ * with P_d = psi1(c_d, v_d, theta + v_d^2) and the operator
+ * O_d = v_d + theta d/dv_d applied to P_d as a function of v_d.  Carried out:
+ *   O^2 Psi(0)   = theta - 3 theta^2 + v^2 - 6 theta v^2 - v^4
+ *   O^2 Psi(+-1) = (+-(v^3 + 3 theta v) + 3 theta^2 + 6 theta v^2 + v^4)/2
+ * (the oracle tests check this against the reference's compiled polynomials
+ * and against sum_l g = rho (U + |v|^2/2)). */
+double orc_geq(int l, double rho, double U, double theta, const double vel[3])
+{
+    double Pf[3], Kf[3];
+    for (int d = 0; d < 3; ++d) {
+        int c = cdir(l, d);
+        double v = vel[d], v2 = v * v;
+        Pf[d] = psi1(c, v, theta + v2);
+        if (c == 0)
+            Kf[d] = theta - 3.0 * theta * theta + v2 - 6.0 * theta * v2 - v2 * v2;
+        else
+            Kf[d] = 0.5 * (c * (v2 * v + 3.0 * theta * v) + 3.0 * theta * theta + 6.0 * theta * v2 + v2 * v2);
+    }
+    double prod = Pf[0] * Pf[1] * Pf[2];
+    double kin = Kf[0] * Pf[1] * Pf[2] + Pf[0] * Kf[1] * Pf[2] + Pf[0] * Pf[1] * Kf[2];
+    return rho * ((U - 1.5 * theta) * prod + 0.5 * kin);
+}
+
+/* calculate_geq(l, rhoe, q, R, theta), lbm-edf.cpp:100-138 (Init only). */
+double orc_geq_moment(int l, double rhoe, const double q[3], const double Rt[9], double theta)
+{
+    double c[3] = {CX[l], CY[l], CZ[l]};
+    double g = rhoe + (q[0] * c[0] + q[1] * c[1] + q[2] * c[2]) / theta;
+    double tr = 0.0;
+    for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n) {
+            double A = Rt[3 * m + n] - (m == n ? rhoe * theta : 0.0);
+            double B = c[m] * c[n] - (m == n ? theta : 0.0);
+            tr += A * B;
+        }
+    g += tr / (2.0 * theta * theta);
+    double wgt = 1.0;
+    for (int m = 0; m < 3; ++m) wgt *= (c[m] == 0.0) ? (1.0 - theta) : 0.5 * theta;
+    g *= wgt;
+    double c2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    for (int m = 0; m < 3; ++m) {
+        double B = (c2 == 0.0) ? 1.0 : (c2 == 1.0 ? -0.5 * fabs(c[m]) : 0.0);
+        double Z = (1.0 - 3.0 * theta) / (2.0 * theta) * (Rt[4 * m] - theta * rhoe);
+        g += B * Z;
+    }
+    return g;
+}
+
+/* calculate_gstr, lbm-edf.cpp:140-150: only the six axis directions shift. */
+double orc_gstr(int l, double geq, const double dq[3])
+{
+    int c2 = CX[l] * CX[l] + CY[l] * CY[l] + CZ[l] * CZ[l];
+    if (c2 != 1) return geq;
+    return geq + 0.5 * (CX[l] * dq[0] + CY[l] * dq[1] + CZ[l] * dq[2]);
+}
+
+/* fd_central, FD.hpp:14-33: periodic ghosts count as walls (one-sided). */
+static int is_wallish(int t) { return t == ORC_S || t == ORC_A || t == ORC_FS || t == ORC_P || t == ORC_O || t == ORC_OC; }
+static double fd_central(double l, double c, double r, int tl, int tr)
+{
+    if (is_wallish(tl)) return tr == ORC_P ? 0.0 : (r - c);
+    if (is_wallish(tr)) return tl == ORC_P ? 0.0 : (c - l);
+    return 0.5 * (r - l);
+}
+
+static size_t cell(const orc_single* s, int i, int j, int k) { return ((size_t)i * s->ny + j) * s->nz + k; }
+static size_t ncell(const orc_single* s) { return (size_t)s->nx * s->ny * s->nz; }
+
+/* LBM::Init single-component, lbm-init.cpp:8-82.  f and g are built at ZERO
+ * velocity while the heat flux uses the set-up velocity (reference quirk). */
+void orc_init_single(orc_single* s)
+{
+    size_t N = ncell(s);
+    double cv = s->R / (s->gamma - 1.0), cp = cv + s->R;
+    for (size_t c = 0; c < N; ++c) {
+        int t = s->type[c];
+        if (!(t == ORC_F || t == ORC_S || t == ORC_I || t == ORC_O || t == ORC_Q)) continue;
+        double vel0[3] = {0.0, 0.0, 0.0}, corr[3] = {0.0, 0.0, 0.0};
+        double rho = s->p[c] / (s->R * s->T[c]);
+        s->rho[c] = rho;
+        double theta = s->R * s->T[c];
+        for (int a = 0; a < 9; ++a) s->ptens[a * N + c] = (a % 4 == 0) ? s->p[c] : 0.0;
+        double rhoe = rho * (cv * s->T[c]);
+        s->rhoe[c] = rhoe;
+        double H = cp * s->T[c];
+        double Rt[9];
+        for (int a = 0; a < 9; ++a) Rt[a] = (a % 4 == 0) ? H * s->p[c] : 0.0;
+        double q[3] = {H * rho * s->u[c], H * rho * s->v[c], H * rho * s->w[c]};
+        for (int l = 0; l < 27; ++l) {
+            s->f[l * N + c] = orc_feq(l, rho, vel0, theta, corr);
+            s->g[l * N + c] = orc_geq_moment(l, rhoe, q, Rt, theta);
+        }
+    }
+}
+
+/* Q_d = rho u_d (1 - 3 R T) - rho u_d^3, the flux whose divergence the
+ * single-component Collide corrects for (lbm-collision.cpp:65-67). */
+static double qdev(const orc_single* s, size_t c, int d)
+{
+    double ud = d == 0 ? s->u[c] : (d == 1 ? s->v[c] : s->w[c]);
+    return s->rho[c] * ud * (1.0 - 3.0 * s->R * s->T[c]) - s->rho[c] * ud * ud * ud;
+}
+
+/* LBM::Collide single-component, lbm-collision.cpp:9-99. */
+void orc_collide_single(orc_single* s)
+{
+    size_t N = ncell(s);
+    double cv = s->R / (s->gamma - 1.0), cp = cv + s->R;
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                size_t c = cell(s, i, j, k);
+                if (s->type[c] != ORC_F) continue;
+                double rho = s->rho[c], p = s->p[c];
+                double theta = s->R * s->T[c];
+                double mu = s->nu * rho;
+                double kappa = mu * cp / s->prtl;
+                double om1 = 2.0 * p * cp / (p * cp + 2.0 * kappa);
+                double om = 2.0 * p / (p + 2.0 * mu);
+                if (1.0 / om <= 0.5 || 1.0 / om1 <= 0.5) s->tau_warnings++;
+                double vel[3] = {s->u[c], s->v[c], s->w[c]};
+                double Uint = s->rhoe[c] / rho - 0.5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+                size_t nbm[3] = {cell(s, i - 1, j, k), cell(s, i, j - 1, k), cell(s, i, j, k - 1)};
+                size_t nbp[3] = {cell(s, i + 1, j, k), cell(s, i, j + 1, k), cell(s, i, j, k + 1)};
+                double dQ[3];
+                for (int d = 0; d < 3; ++d)
+                    dQ[d] = fd_central(qdev(s, nbm[d], d), qdev(s, c, d), qdev(s, nbp[d], d), s->type[nbm[d]],
+                                       s->type[nbp[d]]);
+                double dq[3], corr[3];
+                for (int b = 0; b < 3; ++b) {
+                    double acc = 0.0;
+                    for (int a = 0; a < 3; ++a) {
+                        double eqP = (a == b ? p : 0.0) + rho * vel[a] * vel[b];
+                        acc += vel[a] * (s->ptens[(3 * a + b) * N + c] - eqP);
+                    }
+                    dq[b] = acc + 0.5 * vel[b] * dQ[b];
+                    corr[b] = (2.0 - om) / (2.0 * rho * om) * dQ[b];
+                }
+                for (int l = 0; l < 27; ++l) {
+                    double feq = orc_feq(l, rho, vel, theta, corr);
+                    s->fpc[l * N + c] = (1.0 - om) * s->f[l * N + c] + om * feq;
+                    double geq = orc_geq(l, rho, Uint, theta, vel);
+                    double gs = orc_gstr(l, geq, dq);
+                    double g = s->g[l * N + c];
+                    s->gpc[l * N + c] = g + om1 * (geq - g) + (om - om1) * (geq - gs);
+                }
+            }
+}
+
+static int dir_index(int cx, int cy, int cz)
+{
+    for (int a = 0; a < 27; ++a)
+        if (CX[a] == cx && CY[a] == cy && CZ[a] == cz) return a;
+    return -1;
+}
+
+/* LBM::dirSlip, lbm-bc.cpp:1352-1399. */
+static void dir_slip(const orc_single* s, int l, int i, int j, int k, int* lp, int* ip, int* jp, int* kp)
+{
+    int in = i - CX[l], jn = j - CY[l], kn = k - CZ[l];
+    int cx = CX[l], cy = CY[l], cz = CZ[l];
+    *ip = in; *jp = jn; *kp = kn;
+    if (s->type[cell(s, in, j, k)] == ORC_FS) { cx = -cx; *ip = i; }
+    else if (s->type[cell(s, i, jn, k)] == ORC_FS) { cy = -cy; *jp = j; }
+    else if (s->type[cell(s, i, j, kn)] == ORC_FS) { cz = -cz; *kp = k; }
+    *ip = ((*ip - 1 + (s->nx - 2)) % (s->nx - 2)) + 1;
+    *jp = ((*jp - 1 + (s->ny - 2)) % (s->ny - 2)) + 1;
+    *kp = ((*kp - 1 + (s->nz - 2)) % (s->nz - 2)) + 1;
+    *lp = dir_index(cx, cy, cz);
+}
+
+/* LBM::Streaming, lbm-streaming.cpp:3-119 (pull; walls, free slip, skipped
+ * inflow/outflow sources, periodic wrap through the ghost index). */
+void orc_stream_single(orc_single* s)
+{
+    size_t N = ncell(s);
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                size_t c = cell(s, i, j, k);
+                if (s->type[c] != ORC_F) continue;
+                for (int l = 0; l < 27; ++l) {
+                    int in = i - CX[l], jn = j - CY[l], kn = k - CZ[l];
+                    int tn = s->type[cell(s, in, jn, kn)];
+                    if (tn == ORC_S) {
+                        s->f[l * N + c] = s->fpc[OPP[l] * N + c];
+                    } else if (tn == ORC_FS) {
+                        int lp, ip, jp, kp;
+                        dir_slip(s, l, i, j, k, &lp, &ip, &jp, &kp);
+                        size_t src = cell(s, ip, jp, kp);
+                        s->f[l * N + c] = s->fpc[lp * N + src];
+                        s->g[l * N + c] = s->gpc[lp * N + src];
+                    } else if (tn == ORC_A) {
+                        s->f[l * N + c] = s->fpc[OPP[l] * N + c];
+                        s->g[l * N + c] = s->gpc[OPP[l] * N + c];
+                    } else if (tn == ORC_O || tn == ORC_OC || tn == ORC_I || tn == ORC_IC) {
+                        /* source is an inflow/outflow ghost: left for TMS_BC */
+                    } else {
+                        if (in == 0) in = s->nx - 2; else if (in == s->nx - 1) in = 1;
+                        if (jn == 0) jn = s->ny - 2; else if (jn == s->ny - 1) jn = 1;
+                        if (kn == 0) kn = s->nz - 2; else if (kn == s->nz - 1) kn = 1;
+                        size_t src = cell(s, in, jn, kn);
+                        s->f[l * N + c] = s->fpc[l * N + src];
+                        s->g[l * N + c] = s->gpc[l * N + src];
+                    }
+                }
+            }
+}
+
+/* LBM::calculate_moment single-component, lbm-moment.cpp:11-85. */
+void orc_moments_single(orc_single* s)
+{
+    size_t N = ncell(s);
+    double cv = s->R / (s->gamma - 1.0);
+    for (size_t c = 0; c < N; ++c) {
+        if (s->type[c] != ORC_F) continue;
+        double rho = 0, mx = 0, my = 0, mz = 0, re = 0, qx = 0, qy = 0, qz = 0;
+        double P[9] = {0};
+        for (int l = 0; l < 27; ++l) {
+            double fl = s->f[l * N + c], gl = s->g[l * N + c];
+            double e[3] = {CX[l], CY[l], CZ[l]};
+            rho += fl;
+            mx += fl * e[0];
+            my += fl * e[1];
+            mz += fl * e[2];
+            for (int a = 0; a < 3; ++a)
+                for (int b = a; b < 3; ++b) P[3 * a + b] += fl * e[a] * e[b];
+            re += gl;
+            qx += gl * e[0];
+            qy += gl * e[1];
+            qz += gl * e[2];
+        }
+        P[3] = P[1]; P[6] = P[2]; P[7] = P[5];
+        for (int a = 0; a < 9; ++a) s->ptens[a * N + c] = P[a];
+        s->rho[c] = rho;
+        s->u[c] = mx / rho;
+        s->v[c] = my / rho;
+        s->w[c] = mz / rho;
+        s->rhoe[c] = re;
+        s->eflux[c] = qx;
+        s->eflux[N + c] = qy;
+        s->eflux[2 * N + c] = qz;
+        double e_int = re / rho - 0.5 * (s->u[c] * s->u[c] + s->v[c] * s->v[c] + s->w[c] * s->w[c]);
+        s->T[c] = e_int / cv;
+        s->p[c] = rho * s->R * s->T[c];
+    }
+}
+
+/* Kinetic energy and enstrophy as output.cpp:212-276 defines them, plus the
+ * conserved sums. */
+void orc_diag_single(const orc_single* s, double out[4])
+{
+    double ke = 0, ens = 0, mass = 0, en = 0;
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                size_t c = cell(s, i, j, k);
+                if (s->type[c] != ORC_F) continue;
+                ke += s->rho[c] * (s->u[c] * s->u[c] + s->v[c] * s->v[c] + s->w[c] * s->w[c]);
+                int ia = i + 1, ib = i - 1, ja = j + 1, jb = j - 1, ka = k + 1, kb = k - 1;
+                if (ib < 1) ib = nx - 2; else if (ia > nx - 2) ia = 1;
+                if (jb < 1) jb = ny - 2; else if (ja > ny - 2) ja = 1;
+                if (kb < 1) kb = nz - 2; else if (ka > nz - 2) ka = 1;
+                double uy = 0.5 * (s->u[cell(s, i, ja, k)] - s->u[cell(s, i, jb, k)]);
+                double uz = 0.5 * (s->u[cell(s, i, j, ka)] - s->u[cell(s, i, j, kb)]);
+                double vx = 0.5 * (s->v[cell(s, ia, j, k)] - s->v[cell(s, ib, j, k)]);
+                double vz = 0.5 * (s->v[cell(s, i, j, ka)] - s->v[cell(s, i, j, kb)]);
+                double wx = 0.5 * (s->w[cell(s, ia, j, k)] - s->w[cell(s, ib, j, k)]);
+                double wy = 0.5 * (s->w[cell(s, i, ja, k)] - s->w[cell(s, i, jb, k)]);
+                ens += s->rho[c] * ((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+                mass += s->rho[c];
+                en += s->rhoe[c];
+            }
+    out[0] = ke; out[1] = ens; out[2] = mass; out[3] = en;
+}
