@@ -1,0 +1,138 @@
+/*
+ * lbm_amd: C ABI of the MI355X (gfx950) D3Q27 compressible thermal LBM hot path.
+ *
+ * The reference has no plugin or FFI layer; its seam is the member calls that
+ * LBM::loop makes every time step (lbm-main.cpp:153-193):
+ *
+ *     Collide_Species(); Collide(); Streaming(); TMS_BC(); calculate_moment();
+ *
+ * plus Init() from LBM::run (lbm-main.cpp:126-147) and the host accesses to
+ * lb.mixture[i][j][k] by set-ups and outputs.  Each entry point below replaces
+ * one of those, with the state living in HBM inside an lbm_ctx.
+ *
+ * Conventions
+ *   - Every call returns 0 on success, a negative LBM_E_* code otherwise;
+ *     lbm_last_error(ctx) gives the text.  No C++ exception crosses the ABI.
+ *   - Host arrays are borrowed for the duration of the call only.  They use
+ *     the reference's ghost-inclusive grid (nx+2, ny+2, nz+2) in (i, j, k)
+ *     order with k fastest (lbm-main.cpp:16-35); populations are [27][ncell].
+ *   - Calls are stream-ordered on the context's HIP stream and asynchronous
+ *     until lbm_sync or a download.  One host thread per context.
+ */
+#ifndef LBM_AMD_H
+#define LBM_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+typedef struct lbm_ctx lbm_ctx;
+
+enum lbm_status {
+    LBM_OK = 0,
+    LBM_E_ARG = -1,      /* bad argument */
+    LBM_E_HIP = -2,      /* HIP runtime error (no device, allocation, launch) */
+    LBM_E_STATE = -3,    /* call not valid in the context's current state */
+    LBM_E_UNSUPPORTED = -4
+};
+
+/* Population storage and collision arithmetic. */
+enum lbm_precision {
+    LBM_FP64 = 0,          /* plain method: FP64 populations, FP64 collision */
+    LBM_FP32_SHIFTED = 1   /* FP32 deviations f - f_ref from the rest state */
+};
+
+/* How the five step-calls map onto kernels. */
+enum lbm_mode {
+    LBM_MODE_FUSED = 0,   /* pull-stream fused into collide; lbm_stream is a no-op */
+    LBM_MODE_STRICT = 1   /* one kernel per reference call, explicit f and fpc */
+};
+
+/* Host-visible per-cell fields (MIXTURE members, lbm.hpp:56-88). */
+enum lbm_field {
+    LBM_FIELD_RHO = 0, LBM_FIELD_U, LBM_FIELD_V, LBM_FIELD_W, LBM_FIELD_P, LBM_FIELD_T, LBM_FIELD_RHOE,
+    LBM_FIELD_PXX, LBM_FIELD_PYY, LBM_FIELD_PZZ, LBM_FIELD_PXY, LBM_FIELD_PXZ, LBM_FIELD_PYZ,
+    LBM_FIELD_QX, LBM_FIELD_QY, LBM_FIELD_QZ,   /* energy_flux */
+    LBM_FIELD_COUNT
+};
+
+/* Population sets (MIXTURE::f, fpc, g, gpc). */
+enum lbm_pop { LBM_POP_F = 0, LBM_POP_FPC = 1, LBM_POP_G = 2, LBM_POP_GPC = 3 };
+
+typedef struct {
+    int nx, ny, nz;        /* interior cells; ghosts are added as in LBM::LBM (lbm-main.cpp:19-21) */
+    int n_species;         /* 0: single-component path (the only one this ABI version runs) */
+    int precision;         /* enum lbm_precision */
+    int mode;              /* enum lbm_mode */
+    int device;            /* HIP device ordinal */
+    /* single-component gas (LBM setters, lbm.hpp:264-271) */
+    double nu, gas_const, gamma, prtl;
+    /* rest state the FP32 deviations are taken from (ignored for FP64) */
+    double ref_rho, ref_T;
+} lbm_config;
+
+typedef struct {
+    double mass;         /* sum rho over TYPE_F cells */
+    double energy;       /* sum rhoe */
+    double kinetic;      /* sum rho |u|^2 (no 1/2, as output.cpp:232-233) */
+    double enstrophy;    /* output.cpp:235-262 */
+    double max_mach;     /* max |u| / sqrt(gamma R T) */
+    int64_t nan_count;   /* TYPE_F cells with non-finite rho or T */
+    int64_t tau_warnings;/* collisions with 1/omega <= 0.5 so far (lbm-collision.cpp:33-34 prints instead) */
+    int64_t fallback_count; /* reserved for the chemistry fallback, 0 on this path */
+} lbm_diag;
+
+/* LBM::LBM (lbm-main.cpp:16-35): allocates the device state. */
+int lbm_create(const lbm_config* cfg, lbm_ctx** out);
+/* LBM::~LBM (lbm-main.cpp:121-123). */
+void lbm_destroy(lbm_ctx* ctx);
+const char* lbm_last_error(const lbm_ctx* ctx);
+/* Text of the last lbm_create failure in this thread. */
+const char* lbm_create_error(void);
+
+/* Cell types, uint8 per ghost-inclusive cell (MIXTURE::type, defines.hpp:136-145). */
+int lbm_set_types(lbm_ctx* ctx, const uint8_t* types);
+/* Set-up / output access to per-cell fields (lb.mixture[i][j][k].<field>). */
+int lbm_set_field(lbm_ctx* ctx, int field, const double* host);
+int lbm_get_field(lbm_ctx* ctx, int field, double* host);
+/* Population access, [27][ncell] doubles (restart and parity checks). */
+int lbm_set_populations(lbm_ctx* ctx, int which, const double* host);
+int lbm_get_populations(lbm_ctx* ctx, int which, double* host);
+
+/* LBM::Init, single-component (lbm-init.cpp:8-82). */
+int lbm_init(lbm_ctx* ctx);
+
+/* The step-calls of LBM::loop (lbm-main.cpp:157-181). */
+int lbm_collide_species(lbm_ctx* ctx); /* LBM::Collide_Species; no-op without species */
+int lbm_collide(lbm_ctx* ctx);         /* LBM::Collide (lbm-collision.cpp:9-188) */
+int lbm_stream(lbm_ctx* ctx);          /* LBM::Streaming (lbm-streaming.cpp:3-171) */
+int lbm_tms_bc(lbm_ctx* ctx);          /* LBM::TMS_BC (lbm-bc.cpp:122-1350) */
+int lbm_moments(lbm_ctx* ctx);         /* LBM::calculate_moment (lbm-moment.cpp:11-125) */
+/* nsteps iterations of the loop body above. */
+int lbm_step(lbm_ctx* ctx, int nsteps);
+
+/* Conservation and stability diagnostics (OutputKeEns, output.cpp:212-276, plus
+ * mass, max Mach and counters), reduced on the device in FP64. */
+int lbm_diagnostics(lbm_ctx* ctx, lbm_diag* out);
+
+int lbm_sync(lbm_ctx* ctx);
+/* The hipStream_t the context enqueues on (as void*). */
+void* lbm_get_stream(lbm_ctx* ctx);
+
+/* Kernel timing with HIP events on the context's stream.  While enabled every
+ * kernel launch is bracketed by events; lbm_profile_read returns the summed
+ * milliseconds and launch count of kernels whose name contains `name`. */
+int lbm_profile_enable(lbm_ctx* ctx, int on);
+int lbm_profile_read(lbm_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+
+/* Bytes of device memory the context holds. */
+int64_t lbm_device_bytes(const lbm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

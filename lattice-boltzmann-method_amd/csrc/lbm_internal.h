@@ -1,0 +1,58 @@
+// Internal device-state layout shared by the kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_amd.h"
+#include "lbm_lattice.h"
+
+namespace lbm {
+
+// Ghost-inclusive grid; cell index c = (i*ny + j)*nz + k (k fastest).
+struct Dims {
+    int nx, ny, nz;
+    int64_t n;
+};
+
+// nbflag bits, computed once per lbm_set_types.
+enum : uint8_t {
+    NB_SIMPLE = 1,  // TYPE_F cell whose 26 neighbours are all TYPE_F or TYPE_P
+    NB_KEEP = 2     // TYPE_F cell with an S/O/O_C/I/I_C neighbour (stream keeps stale values)
+};
+
+// Device view of the single-component state; P is the population storage and
+// collision type (double: plain FP64, float: FP32 deviations).
+template <typename P>
+struct SingleView {
+    Dims d;
+    const uint8_t* type;
+    const uint8_t* nbflag;
+    P* f;    // [27][n] populations of the source buffer
+    P* g;
+    P* fo;   // [27][n] destination buffer
+    P* go;
+    double* fld[LBM_FIELD_COUNT];
+    P* qd[3];               // Q_d = rho u_d (1 - 3 R T) - rho u_d^3 of the last moment pass
+    double R, cv, cp, nu, prtl, gamma;
+    // Rest-state populations subtracted in storage (all 0 for FP64) and their
+    // sums {sum f, sum cx^2 f, sum cy^2 f, sum cz^2 f, sum g}; the odd and
+    // mixed moments of the rest state vanish.
+    double fref[27], gref[27];
+    double fref_sum[5];
+    __host__ __device__ double fref_d(int l) const { return fref[l]; }
+    __host__ __device__ double gref_d(int l) const { return gref[l]; }
+    unsigned long long* tau_count;
+};
+
+// Kernel launchers (lbm_single.hip).  All enqueue on `s`.
+template <typename P> void launch_init(const SingleView<P>& v, hipStream_t s);
+template <typename P> void launch_collide_explicit(const SingleView<P>& v, hipStream_t s);
+template <typename P> void launch_stream(const SingleView<P>& v, hipStream_t s);
+template <typename P> void launch_moments(const SingleView<P>& v, hipStream_t s);
+template <typename P> void launch_fused_collide(const SingleView<P>& v, hipStream_t s);
+template <typename P> void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t s);
+void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double R,
+                 double* partials, int npart, double* out8, hipStream_t s);
+int diag_partials(const Dims& d);
+
+}  // namespace lbm

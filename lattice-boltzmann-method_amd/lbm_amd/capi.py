@@ -1,0 +1,256 @@
+"""ctypes binding of liblbm_amd.so and a host-side mirror of the reference's LBM class.
+
+`Lattice` keeps the reference's member names and step-call structure
+(lbm.hpp:104-274, lbm-main.cpp:126-196): Init, Collide_Species, Collide,
+Streaming, TMS_BC, calculate_moment, run/loop, getters and setters.  Every
+call goes through the C ABI of include/lbm_amd.h; there is no Python or CPU
+fallback: without the built HIP library or a HIP device the constructor
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "liblbm_amd.so")
+
+LBM_FP64, LBM_FP32_SHIFTED = 0, 1
+LBM_MODE_FUSED, LBM_MODE_STRICT = 0, 1
+POP_F, POP_FPC, POP_G, POP_GPC = 0, 1, 2, 3
+
+FIELDS = ["rho", "u", "v", "w", "p", "T", "rhoe", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "qx", "qy", "qz"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+
+EXPORTS = [
+    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_create_error", "lbm_set_types", "lbm_set_field",
+    "lbm_get_field", "lbm_set_populations", "lbm_get_populations", "lbm_init", "lbm_collide_species",
+    "lbm_collide", "lbm_stream", "lbm_tms_bc", "lbm_moments", "lbm_step", "lbm_diagnostics", "lbm_sync",
+    "lbm_get_stream", "lbm_profile_enable", "lbm_profile_read", "lbm_device_bytes",
+]
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("n_species", ctypes.c_int),
+                ("precision", ctypes.c_int), ("mode", ctypes.c_int), ("device", ctypes.c_int),
+                ("nu", ctypes.c_double), ("gas_const", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("prtl", ctypes.c_double), ("ref_rho", ctypes.c_double), ("ref_T", ctypes.c_double)]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("mass", ctypes.c_double), ("energy", ctypes.c_double), ("kinetic", ctypes.c_double),
+                ("enstrophy", ctypes.c_double), ("max_mach", ctypes.c_double), ("nan_count", ctypes.c_int64),
+                ("tau_warnings", ctypes.c_int64), ("fallback_count", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads liblbm_amd.so; raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise LbmError(f"{path} is not built: run __graft_entry__.build() or `make -C lattice-boltzmann-method_amd`")
+    L = ctypes.CDLL(path)
+    vp, i32, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    L.lbm_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+    L.lbm_create.restype = i32
+    L.lbm_destroy.argtypes = [vp]
+    L.lbm_destroy.restype = None
+    L.lbm_last_error.argtypes = [vp]
+    L.lbm_last_error.restype = ctypes.c_char_p
+    L.lbm_create_error.argtypes = []
+    L.lbm_create_error.restype = ctypes.c_char_p
+    L.lbm_set_types.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8)]
+    for fn in ("lbm_set_field", "lbm_get_field", "lbm_set_populations", "lbm_get_populations"):
+        getattr(L, fn).argtypes = [vp, i32, dp]
+    for fn in ("lbm_init", "lbm_collide_species", "lbm_collide", "lbm_stream", "lbm_tms_bc", "lbm_moments",
+               "lbm_sync"):
+        getattr(L, fn).argtypes = [vp]
+    L.lbm_step.argtypes = [vp, i32]
+    L.lbm_diagnostics.argtypes = [vp, ctypes.POINTER(Diag)]
+    L.lbm_get_stream.argtypes = [vp]
+    L.lbm_get_stream.restype = vp
+    L.lbm_profile_enable.argtypes = [vp, i32]
+    L.lbm_profile_read.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64)]
+    L.lbm_device_bytes.argtypes = [vp]
+    L.lbm_device_bytes.restype = ctypes.c_int64
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Lattice:
+    """Single-component D3Q27 lattice on one GPU, mirroring LBM(Nx, Ny, Nz, nu)."""
+
+    def __init__(self, nx, ny, nz, nu, gas_const=1.0, gamma=1.4, prtl=0.5, precision=LBM_FP64,
+                 mode=LBM_MODE_FUSED, device=0, ref_rho=1.0, ref_T=1.0):
+        self._L = load_library()
+        cfg = Config(nx, ny, nz, 0, precision, mode, device, nu, gas_const, gamma, prtl, ref_rho, ref_T)
+        h = ctypes.c_void_p()
+        rc = self._L.lbm_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise LbmError(f"lbm_create failed ({rc}): {self._L.lbm_create_error().decode()}")
+        self._h = h
+        self.cfg = cfg
+        self.shape = (nx + 2, ny + 2, nz + 2)
+        self.ncell = int(np.prod(self.shape))
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LbmError(f"{what} failed ({rc}): {self._L.lbm_last_error(self._h).decode()}")
+
+    # -- getters mirroring lbm.hpp:200-217
+    def get_Nx(self):
+        return self.shape[0]
+
+    def get_Ny(self):
+        return self.shape[1]
+
+    def get_Nz(self):
+        return self.shape[2]
+
+    def get_NX(self):
+        return self.shape[0] - 2
+
+    def get_NY(self):
+        return self.shape[1] - 2
+
+    def get_NZ(self):
+        return self.shape[2] - 2
+
+    def get_nu(self):
+        return self.cfg.nu
+
+    def get_gasconst(self):
+        return self.cfg.gas_const
+
+    def get_gamma(self):
+        return self.cfg.gamma
+
+    def get_prtl(self):
+        return self.cfg.prtl
+
+    def get_soundspeed(self, temp):
+        return float(np.sqrt(self.cfg.gamma * self.cfg.gas_const * temp))
+
+    # -- host access to cell data
+    def set_types(self, types):
+        t = np.ascontiguousarray(types, dtype=np.uint8).reshape(self.shape)
+        self._check(self._L.lbm_set_types(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))), "set_types")
+
+    def set_field(self, name, values):
+        a = np.ascontiguousarray(np.broadcast_to(values, self.shape), dtype=np.float64)
+        self._check(self._L.lbm_set_field(self._h, FIELD_ID[name], _dp(a)), f"set_field({name})")
+
+    def get_field(self, name):
+        a = np.empty(self.shape, np.float64)
+        self._check(self._L.lbm_get_field(self._h, FIELD_ID[name], _dp(a)), f"get_field({name})")
+        return a
+
+    def set_populations(self, which, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape((27,) + self.shape)
+        self._check(self._L.lbm_set_populations(self._h, which, _dp(a)), "set_populations")
+
+    def get_populations(self, which):
+        a = np.empty((27,) + self.shape, np.float64)
+        self._check(self._L.lbm_get_populations(self._h, which, _dp(a)), "get_populations")
+        return a
+
+    def load_case(self, fields):
+        """Uploads a set-up from lbm_amd.cases (type + primitive fields)."""
+        self.set_types(fields["type"])
+        for k in ("rho", "u", "v", "w", "p", "T"):
+            if k in fields:
+                self.set_field(k, fields[k])
+
+    # -- the reference's step calls
+    def Init(self):
+        self._check(self._L.lbm_init(self._h), "Init")
+
+    def Collide_Species(self):
+        self._check(self._L.lbm_collide_species(self._h), "Collide_Species")
+
+    def Collide(self):
+        self._check(self._L.lbm_collide(self._h), "Collide")
+
+    def Streaming(self):
+        self._check(self._L.lbm_stream(self._h), "Streaming")
+
+    def TMS_BC(self):
+        self._check(self._L.lbm_tms_bc(self._h), "TMS_BC")
+
+    def calculate_moment(self):
+        self._check(self._L.lbm_moments(self._h), "calculate_moment")
+
+    def loop(self, nstep, callback=None, tout=0):
+        """LBM::loop body nstep times; `callback(step, self)` every tout steps."""
+        for step in range(1, nstep + 1):
+            self.Collide_Species()
+            self.Collide()
+            self.Streaming()
+            self.TMS_BC()
+            self.calculate_moment()
+            if callback is not None and tout and step % tout == 0:
+                callback(step, self)
+
+    def step(self, n=1):
+        self._check(self._L.lbm_step(self._h, n), "step")
+
+    def run(self, nstep, callback=None, tout=0):
+        self.Init()
+        if callback is not None:
+            callback(0, self)
+        self.loop(nstep, callback, tout)
+
+    # -- diagnostics, sync, profiling
+    def diagnostics(self):
+        d = Diag()
+        self._check(self._L.lbm_diagnostics(self._h, ctypes.byref(d)), "diagnostics")
+        return d.as_dict()
+
+    def sync(self):
+        self._check(self._L.lbm_sync(self._h), "sync")
+
+    def stream_handle(self):
+        return self._L.lbm_get_stream(self._h)
+
+    def profile(self, on=True):
+        self._check(self._L.lbm_profile_enable(self._h, 1 if on else 0), "profile_enable")
+
+    def profile_read(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self._L.lbm_profile_read(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+                    "profile_read")
+        return ms.value, n.value
+
+    def device_bytes(self):
+        return self._L.lbm_device_bytes(self._h)

@@ -67,3 +67,44 @@ def taylor_green_3d(N, U0, p0, T0, **_unused):
     fl["p"][m] = p[m]
     fl["T"][m] = T0
     return fl
+
+
+def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.0 / 3.0, T0=1.0, p0=1.0 / 3.0,
+                        gamma=1.4, Pr=0.71, nu=None, **_unused):
+    """Thermal compressible decaying turbulence on an N^3 periodic box.
+
+    Seeded random solenoidal velocity with energy spectrum
+    E(k) ~ k^4 exp(-2 k^2 / k0^2), scaled so that sqrt(<|u|^2>) / c0 = ma_rms
+    with c0 = sqrt(gamma R T0); temperature T0 (1 + t_fluct * U(-1, 1));
+    uniform pressure p0.  Returns (fields, params)."""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(N, d=1.0 / N)
+    kr = np.fft.rfftfreq(N, d=1.0 / N)
+    KX, KY, KZ = np.meshgrid(k, k, kr, indexing="ij")
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    Kmag = np.sqrt(K2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.where(Kmag > 0, np.sqrt(Kmag ** 4 * np.exp(-2.0 * K2 / k0 ** 2) / (4.0 * np.pi * K2)), 0.0)
+    comps = []
+    for _ in range(3):
+        comps.append((rng.standard_normal(KX.shape) + 1j * rng.standard_normal(KX.shape)) * amp)
+    # project onto the plane normal to k (solenoidal)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dot = (KX * comps[0] + KY * comps[1] + KZ * comps[2]) / np.where(K2 > 0, K2, 1.0)
+    comps = [comps[0] - KX * dot, comps[1] - KY * dot, comps[2] - KZ * dot]
+    vel = [np.fft.irfftn(c, s=(N, N, N)) for c in comps]
+    c0 = math.sqrt(gamma * R * T0)
+    urms = math.sqrt(float(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)))
+    scale = ma_rms * c0 / urms
+    fl = default_fields(N, N, N)
+    periodic_ghosts(fl)
+    s = (slice(1, -1),) * 3
+    fl["u"][s] = vel[0] * scale
+    fl["v"][s] = vel[1] * scale
+    fl["w"][s] = vel[2] * scale
+    fl["T"][s] = T0 * (1.0 + t_fluct * rng.uniform(-1.0, 1.0, (N, N, N)))
+    fl["p"][s] = p0
+    if nu is None:
+        nu = 1e-3
+    params = dict(N=N, R=R, T0=T0, p0=p0, gamma=gamma, Pr=Pr, nu=nu, seed=seed, k0=k0, ma_rms=ma_rms)
+    return fl, params
