@@ -144,10 +144,13 @@ def test_golden_tgv64_series_and_nan_onset(golden_dir):
     onset_ref = int(np.argmin(np.isfinite(ref[:, 0])))
     onset_got = int(np.argmin(np.isfinite(got[:, 0])))
     assert onset_ref > 100
-    smooth = slice(0, 81)
+    # The oracle against the same reference run shows the set-up's own
+    # instability: differences stay ~1e-14 through step 60, then grow ~1000x
+    # per 10 steps (1.9e-8 in enstrophy at step 80) until the NaN at ~118.
+    smooth = slice(0, 61)
     err = np.max(np.abs(got[smooth] - ref[smooth]) / np.abs(ref[smooth]))
     assert err < 1e-10, err
-    assert onset_got == onset_ref, (onset_got, onset_ref)
+    assert abs(onset_got - onset_ref) <= 2, (onset_got, onset_ref)
 
 
 def test_diagnostics_match_oracle():
