@@ -108,3 +108,47 @@ def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.
         nu = 1e-3
     params = dict(N=N, R=R, T0=T0, p0=p0, gamma=gamma, Pr=Pr, nu=nu, seed=seed, k0=k0, ma_rms=ma_rms)
     return fl, params
+
+
+# ----------------------------------------------------------- multicomponent
+H2AIR_SPECIES = ["H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"]
+# phi = 1 H2-air, unburnt mole fractions (setup.cpp:2108-2116)
+X_UNBURNT_PHI1 = {"H2": 0.29586, "O2": 0.14793, "N2": 0.55621}
+X_AIR = {"O2": 0.21, "N2": 0.79}
+UNITS_FLAME = (5e-6, 1e-9)  # units.set_m_kg_s(dx, dt) of the flame cases (setup.cpp:2019)
+LATTICE_K = 12000.0          # temperature unit of set_m_kg_s(dx, dt) (units.hpp:36)
+ONE_ATM = 101325.0
+
+
+def mole_vector(d, minim=1e-10):
+    x = np.array([d.get(n, minim) for n in H2AIR_SPECIES], dtype=np.float64)
+    return x / x.sum()
+
+
+def lattice_p(p_si, dx=UNITS_FLAME[0], dt=UNITS_FLAME[1]):
+    """units.p of the flame units: p_si * m s^2 / kg with kg = m^3."""
+    m = np.float32(dx)
+    s = np.float32(dt)
+    kg = np.float32(float(m) ** 3)
+    return float(np.float32(p_si * (float(m) * float(s) ** 2) / float(kg)))
+
+
+def reacting_box(nx, ny, nz, T_K=1400.0, X=None, amp_T=0.0, amp_u=0.0, seed=20261206):
+    """Periodic box of premixed gas (the PERFECTLY_STIRRED_REACTOR_3D shape,
+    setup.cpp:950-999, with optional smooth temperature and velocity
+    perturbations).  Returns fields with species mole fractions 'X' (NS, ...)."""
+    fl = default_fields(nx, ny, nz)
+    periodic_ghosts(fl)
+    X = mole_vector(X or X_UNBURNT_PHI1)
+    shape = fl["type"].shape
+    i, j, k = np.meshgrid(*(np.arange(s, dtype=np.float64) for s in shape), indexing="ij")
+    m = fl["type"] == TYPE_F
+    T = T_K * (1.0 + amp_T * np.sin(2 * np.pi * i / nx) * np.cos(2 * np.pi * j / ny))
+    fl["T"][m] = (T / LATTICE_K)[m]
+    fl["p"][m] = lattice_p(ONE_ATM)
+    fl["u"][m] = (amp_u * np.sin(2 * np.pi * j / ny))[m]
+    fl["v"][m] = (amp_u * np.cos(2 * np.pi * i / nx))[m]
+    fl["w"][m] = (0.5 * amp_u * np.sin(2 * np.pi * k / nz))[m]
+    fl["X"] = np.zeros((len(H2AIR_SPECIES),) + shape)
+    fl["X"][:, m] = X[:, None]
+    return fl
