@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 typedef struct lbm_ctx lbm_ctx;
 
@@ -57,15 +57,20 @@ enum lbm_field {
     LBM_FIELD_RHO = 0, LBM_FIELD_U, LBM_FIELD_V, LBM_FIELD_W, LBM_FIELD_P, LBM_FIELD_T, LBM_FIELD_RHOE,
     LBM_FIELD_PXX, LBM_FIELD_PYY, LBM_FIELD_PZZ, LBM_FIELD_PXY, LBM_FIELD_PXZ, LBM_FIELD_PYZ,
     LBM_FIELD_QX, LBM_FIELD_QY, LBM_FIELD_QZ,   /* energy_flux */
+    LBM_FIELD_HRR,                              /* heat release rate (multicomponent) */
     LBM_FIELD_COUNT
 };
 
-/* Population sets (MIXTURE::f, fpc, g, gpc). */
+/* Population sets (MIXTURE::f, fpc, g, gpc; with species, f and fpc are the
+ * per-species SPECIES::f, fpc). */
 enum lbm_pop { LBM_POP_F = 0, LBM_POP_FPC = 1, LBM_POP_G = 2, LBM_POP_GPC = 3 };
+
+/* Per-species fields (SPECIES members, lbm.hpp:90-102). */
+enum lbm_species_field { LBM_SP_X = 0, LBM_SP_RHO = 1, LBM_SP_U = 2, LBM_SP_V = 3, LBM_SP_W = 4 };
 
 typedef struct {
     int nx, ny, nz;        /* interior cells; ghosts are added as in LBM::LBM (lbm-main.cpp:19-21) */
-    int n_species;         /* 0: single-component path (the only one this ABI version runs) */
+    int n_species;         /* 0: single-component path; 9: the H2-air mixture (H2 H O O2 OH H2O HO2 H2O2 N2) */
     int precision;         /* enum lbm_precision */
     int mode;              /* enum lbm_mode */
     int device;            /* HIP device ordinal */
@@ -73,6 +78,10 @@ typedef struct {
     double nu, gas_const, gamma, prtl;
     /* rest state the FP32 deviations are taken from (ignored for FP64) */
     double ref_rho, ref_T;
+    /* multicomponent only: units.set_m_kg_s(dx, dt) (units.hpp:32-44), and
+     * whether Collide_Species integrates the chemistry source (REACTION) */
+    double dx_si, dt_si;
+    int reaction;
 } lbm_config;
 
 typedef struct {
@@ -80,7 +89,7 @@ typedef struct {
     double energy;       /* sum rhoe */
     double kinetic;      /* sum rho |u|^2 (no 1/2, as output.cpp:232-233) */
     double enstrophy;    /* output.cpp:235-262 */
-    double max_mach;     /* max |u| / sqrt(gamma R T) */
+    double max_mach;     /* max |u| / sqrt(gamma p / rho) (gamma of the config, 1.4 if unset) */
     int64_t nan_count;   /* TYPE_F cells with non-finite rho or T */
     int64_t tau_warnings;/* collisions with 1/omega <= 0.5 so far (lbm-collision.cpp:33-34 prints instead) */
     int64_t fallback_count; /* reserved for the chemistry fallback, 0 on this path */
@@ -99,15 +108,19 @@ int lbm_set_types(lbm_ctx* ctx, const uint8_t* types);
 /* Set-up / output access to per-cell fields (lb.mixture[i][j][k].<field>). */
 int lbm_set_field(lbm_ctx* ctx, int field, const double* host);
 int lbm_get_field(lbm_ctx* ctx, int field, double* host);
-/* Population access, [27][ncell] doubles (restart and parity checks). */
+/* Population access, [27][ncell] doubles (restart and parity checks).  With
+ * species, LBM_POP_F / LBM_POP_FPC move all species at once, [n_species][27][ncell]. */
 int lbm_set_populations(lbm_ctx* ctx, int which, const double* host);
 int lbm_get_populations(lbm_ctx* ctx, int which, double* host);
+/* Per-species fields, [n_species][ncell] doubles (lb.species[a][i][j][k].<field>). */
+int lbm_set_species_field(lbm_ctx* ctx, int field, const double* host);
+int lbm_get_species_field(lbm_ctx* ctx, int field, double* host);
 
-/* LBM::Init, single-component (lbm-init.cpp:8-82). */
+/* LBM::Init (single component lbm-init.cpp:8-82, multicomponent :85-146). */
 int lbm_init(lbm_ctx* ctx);
 
 /* The step-calls of LBM::loop (lbm-main.cpp:157-181). */
-int lbm_collide_species(lbm_ctx* ctx); /* LBM::Collide_Species; no-op without species */
+int lbm_collide_species(lbm_ctx* ctx); /* LBM::Collide_Species (lbm-collision.cpp:334-526); no-op without species */
 int lbm_collide(lbm_ctx* ctx);         /* LBM::Collide (lbm-collision.cpp:9-188) */
 int lbm_stream(lbm_ctx* ctx);          /* LBM::Streaming (lbm-streaming.cpp:3-171) */
 int lbm_tms_bc(lbm_ctx* ctx);          /* LBM::TMS_BC (lbm-bc.cpp:122-1350) */

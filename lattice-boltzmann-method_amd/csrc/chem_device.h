@@ -1,0 +1,291 @@
+// Thermo, kinetics, transport and unit conversions of the multicomponent
+// path, as __host__ __device__ functions over the generated H2-air tables.
+//
+// These replace the Cantera calls the reference makes per cell (ideal-gas
+// NASA7 thermo, setState_UV / TD / TP, getNetProductionRates with
+// Arrhenius, third-body and Troe falloff reactions and reverse rates from
+// equilibrium constants, mixture-averaged transport) and the binary32
+// Units_Conv crossings between lattice and SI units (units.hpp:75-131).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "mech_h2air.h"
+
+namespace lbm {
+namespace chem {
+
+using namespace mech;
+constexpr double SPECIES_MIN = 1e-8;  // defines.hpp:96
+
+// ---------------------------------------------------------------- units
+// Units_Conv after set_m_kg_s(dx, dt): m, kg, s, K stored as float; every
+// conversion rounds its argument to float and computes in float, or in
+// double where the reference's double cb()/sq() enters, then returns float.
+struct Units {
+    float m, kg, s, K;
+
+    __host__ __device__ float temp(double x) const { return (float)x / K; }
+    __host__ __device__ float si_temp(double x) const { return (float)x * K; }
+    __host__ __device__ float energy_mass(double x) const { return (float)x * (s * s) / (m * m); }
+    __host__ __device__ float si_energy_mass(double x) const { return (float)x * (m * m) / (s * s); }
+    __host__ __device__ float rho(double x) const
+    {
+        const double md = m;
+        return (float)((double)(float)x * (md * md * md) / (double)kg);
+    }
+    __host__ __device__ float si_rho(double x) const
+    {
+        const double md = m;
+        return (float)((double)((float)x * kg) / (md * md * md));
+    }
+    __host__ __device__ float rho_dot(double x) const
+    {
+        const double md = m;
+        return (float)((double)(float)x * (md * md * md) * (double)s / (double)kg);
+    }
+    __host__ __device__ float nu(double x) const
+    {
+        const double md = m;
+        return (float)((double)((float)x * s) / (md * md));
+    }
+    __host__ __device__ float mu(double x) const { return (float)x * s * m / kg; }
+    __host__ __device__ float thermal_conductivity(double x) const { return (float)x * s * s * s * K / (kg * m); }
+    __host__ __device__ float cp(double x) const { return (float)x * s * s * K / (m * m); }
+    __host__ __device__ float p(double x) const
+    {
+        const double sd = s;
+        return (float)((double)(float)x * ((double)m * (sd * sd)) / (double)kg);
+    }
+    __host__ __device__ float si_p(double x) const
+    {
+        const double sd = s;
+        return (float)((double)((float)x * kg) / ((double)m * (sd * sd)));
+    }
+};
+
+// set_m_kg_s(dx, dt) of units.hpp:32-44.
+inline Units make_units(double dx, double dt)
+{
+    Units u;
+    u.m = (float)((double)(float)dx / 1.0);
+    const double md = u.m;
+    u.kg = (float)(1.0 / 1.0 * (md * md * md));
+    u.s = (float)((double)(float)dt / 1.0);
+    u.K = 12000.0f;
+    return u;
+}
+
+// --------------------------------------------------------------- thermo
+__host__ __device__ inline const double* nasa(int k, double T) { return T < T_MID[k] ? NASA_LO[k] : NASA_HI[k]; }
+
+__host__ __device__ inline double cp_R(int k, double T)
+{
+    const double* a = nasa(k, T);
+    return a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+}
+
+__host__ __device__ inline double h_RT(int k, double T)
+{
+    const double* a = nasa(k, T);
+    return a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] / T;
+}
+
+__host__ __device__ inline double s_R(int k, double T, double lnT)
+{
+    const double* a = nasa(k, T);
+    return a[0] * lnT + T * (a[1] + T * (a[2] / 2 + T * (a[3] / 3 + T * a[4] / 4))) + a[6];
+}
+
+// Cantera's setMassFractions: negatives clipped, then normalised.
+__host__ __device__ inline void normalize(double Y[NS])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        Y[k] = Y[k] > 0.0 ? Y[k] : 0.0;
+        s += Y[k];
+    }
+    const double inv = 1.0 / s;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) Y[k] *= inv;
+}
+
+__host__ __device__ inline double mean_W(const double Y[NS])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s += Y[k] / W[k];
+    return 1.0 / s;
+}
+
+__host__ __device__ inline void X_from_Y(const double Y[NS], double X[NS])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        X[k] = Y[k] / W[k];
+        s += X[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) X[k] /= s;
+}
+
+__host__ __device__ inline void Y_from_X(const double X[NS], double Y[NS])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        Y[k] = (X[k] > 0.0 ? X[k] : 0.0) * W[k];
+        s += Y[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) Y[k] /= s;
+}
+
+__host__ __device__ inline double cp_mass(double T, const double Y[NS])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s += Y[k] * cp_R(k, T) / W[k];
+    return s * R_UNIV;
+}
+
+__host__ __device__ inline double u_mass(double T, const double Y[NS])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s += Y[k] * (h_RT(k, T) - 1.0) / W[k];
+    return s * R_UNIV * T;
+}
+
+// setState_UV at fixed composition (tolerance 1e-15 as the reference asks):
+// Newton on u(T) = u_target from the current temperature T0.
+__host__ __device__ inline double T_from_UV(double u_target, const double Y[NS], double T0)
+{
+    double T = T0;
+    for (int it = 0; it < 60; ++it) {
+        double u = 0, cv = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const double yw = Y[k] / W[k];
+            u += yw * (h_RT(k, T) - 1.0);
+            cv += yw * (cp_R(k, T) - 1.0);
+        }
+        const double dT = -(u * R_UNIV * T - u_target) / (cv * R_UNIV);
+        T += dT;
+        if (fabs(dT) <= 1e-15 * fabs(T)) break;
+    }
+    return T;
+}
+
+__host__ __device__ inline double pressure(double T, double rho, const double Y[NS])
+{
+    return rho * R_UNIV * T / mean_W(Y);
+}
+
+// ------------------------------------------------------------ kinetics
+// Net molar production rates [kmol/m^3/s] at (T, rho, Y), SI units.
+__host__ __device__ inline void net_production(double T, double rho, const double Y[NS], double wdot[NS])
+{
+    double C[NS], gRT[NS];
+    const double lnT = log(T), invT = 1.0 / T;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        C[k] = rho * Y[k] / W[k];
+        gRT[k] = h_RT(k, T) - s_R(k, T, lnT);
+        wdot[k] = 0.0;
+    }
+    const double ln_pref_RT = log(P_REF / (R_UNIV * T));
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double kf = ARR_A[r] * exp(ARR_B[r] * lnT - ARR_EA_R[r] * invT);
+        if (KIND[r] != 0) {
+            double M = 0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) M += EFF[r][k] * C[k];
+            if (KIND[r] == 1) {
+                kf *= M;
+            } else {
+                const double k0 = LOW_A[r] * exp(LOW_B[r] * lnT - LOW_EA_R[r] * invT);
+                const double Pr = k0 * M / kf;
+                double Fc = (1.0 - TROE[r][0]) * exp(-T / TROE[r][1]) + TROE[r][0] * exp(-T / TROE[r][2]);
+                if (TROE_HAS_T2[r]) Fc += exp(-TROE[r][3] * invT);
+                const double lFc = log10(Fc);
+                const double lPr = log10(Pr > 1e-300 ? Pr : 1e-300);
+                const double cc = -0.4 - 0.67 * lFc;
+                const double nn = 0.75 - 1.27 * lFc;
+                const double f1 = (lPr + cc) / (nn - 0.14 * (lPr + cc));
+                kf = kf * Pr / (1.0 + Pr) * exp10(lFc / (1.0 + f1 * f1));
+            }
+        }
+        double dG = 0, cf = 1, cr = 1;
+        int dn = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int nf = NU_F[r][k], nr = NU_R[r][k];
+            dG += (nr - nf) * gRT[k];
+            dn += nr - nf;
+            if (nf == 1) cf *= C[k];
+            if (nf == 2) cf *= C[k] * C[k];
+            if (nr == 1) cr *= C[k];
+            if (nr == 2) cr *= C[k] * C[k];
+        }
+        // k_r = k_f / K_c, K_c = exp(-dG) (p_ref / RT)^dn
+        const double kr = kf * exp(dG - dn * ln_pref_RT);
+        const double q = kf * cf - kr * cr;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (NU_R[r][k] - NU_F[r][k]) wdot[k] += (NU_R[r][k] - NU_F[r][k]) * q;
+    }
+}
+
+// ------------------------------------------------------------ transport
+__host__ __device__ inline double poly4(const double c[5], double x)
+{
+    return c[0] + x * (c[1] + x * (c[2] + x * (c[3] + x * c[4])));
+}
+
+__host__ __device__ inline double species_viscosity(int k, double lnT) { return exp(poly4(VISC_FIT[k], lnT)); }
+__host__ __device__ inline double species_conductivity(int k, double lnT) { return exp(poly4(COND_FIT[k], lnT)); }
+// D_ab [m^2/s] at (T, p).
+__host__ __device__ inline double binary_diff(int a, int b, double lnT, double p)
+{
+    return exp(poly4(DIFF_FIT[a][b], lnT)) / p;
+}
+
+// Wilke mixture viscosity with the direct Phi_ab formula.
+__host__ __device__ inline double mix_viscosity(double lnT, const double X[NS])
+{
+    double mu[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) mu[k] = species_viscosity(k, lnT);
+    double tot = 0;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        double den = 0;
+#pragma unroll
+        for (int b = 0; b < NS; ++b) {
+            const double r = 1.0 + sqrt(mu[a] / mu[b]) * sqrt(sqrt(W[b] / W[a]));
+            den += X[b] * (r * r / sqrt(8.0 * (1.0 + W[a] / W[b])));
+        }
+        tot += X[a] * mu[a] / den;
+    }
+    return tot;
+}
+
+// Combination-averaged mixture conductivity 1/2 (sum X l + 1 / sum X / l).
+__host__ __device__ inline double mix_conductivity(double lnT, const double X[NS])
+{
+    double s1 = 0, s2 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const double l = species_conductivity(k, lnT);
+        s1 += X[k] * l;
+        s2 += X[k] / l;
+    }
+    return 0.5 * (s1 + 1.0 / s2);
+}
+
+}  // namespace chem
+}  // namespace lbm

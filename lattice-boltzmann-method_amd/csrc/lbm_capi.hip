@@ -10,6 +10,7 @@
 
 #include "lbm_amd.h"
 #include "lbm_internal.h"
+#include "lbm_multi.h"
 
 using namespace lbm;
 
@@ -58,6 +59,12 @@ struct lbm_ctx {
     bool needs_tms = false;
 
     double fref[27] = {}, gref[27] = {}, fref_sum[5] = {};
+
+    // multicomponent state (ns > 0): species populations A/B, species fields
+    int ns = 0;
+    void* sf[2] = {nullptr, nullptr};
+    double* sp_block = nullptr;  // X, rho_a [NS][n] and u_a [3][NS][n]
+    chem::Units units{};
 
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -126,6 +133,34 @@ SingleView<P> make_view(lbm_ctx* c, int src, int dst)
     v.tau_count = c->tau_count;
     return v;
 }
+
+MultiView<double> make_mview(lbm_ctx* c, int src, int dst)
+{
+    MultiView<double> v{};
+    v.d = c->d;
+    v.type = c->type;
+    v.f = static_cast<double*>(c->sf[src]);
+    v.fo = static_cast<double*>(c->sf[dst]);
+    v.g = static_cast<double*>(c->pg[src]);
+    v.go = static_cast<double*>(c->pg[dst]);
+    for (int i = 0; i < LBM_FIELD_COUNT; ++i) v.fld[i] = c->fld[i];
+    v.hrr = c->fld[LBM_FIELD_HRR];
+    const int64_t n = c->d.n;
+    v.X = c->sp_block;
+    v.rhoa = c->sp_block + (size_t)c->ns * n;
+    v.ua = c->sp_block + (size_t)2 * c->ns * n;
+    v.u = c->units;
+    v.reaction = c->cfg.reaction;
+    v.tau_count = c->tau_count;
+    return v;
+}
+
+int run_mc(lbm_ctx* c, McKernel k, const char* name, int src, int dst)
+{
+    return launch(c, name, [&] { launch_mc<double>(k, make_mview(c, src, dst), c->stream); });
+}
+
+bool multi(const lbm_ctx* c) { return c->ns > 0; }
 
 bool fp32(const lbm_ctx* c) { return c->cfg.precision == LBM_FP32_SHIFTED; }
 bool fused(const lbm_ctx* c) { return c->cfg.mode == LBM_MODE_FUSED; }
@@ -205,7 +240,7 @@ void release(lbm_ctx* c)
     }
     c->pending.clear();
     void* ptrs[] = {c->type, c->nbflag, c->pf[0], c->pf[1], c->pg[0], c->pg[1], c->fld_block, c->qd_block,
-                    c->tau_count, c->diag_part, c->diag_out};
+                    c->tau_count, c->diag_part, c->diag_out, c->sf[0], c->sf[1], c->sp_block};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -236,9 +271,19 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         g_create_error = "grid extents must be >= 1";
         return LBM_E_ARG;
     }
-    if (cfg->n_species != 0) {
-        g_create_error = "multicomponent state (n_species > 0) is not provided by this library build";
+    if (cfg->n_species != 0 && cfg->n_species != mech::NS) {
+        g_create_error = "n_species must be 0 (single component) or 9 (the H2-air mechanism)";
         return LBM_E_UNSUPPORTED;
+    }
+    if (cfg->n_species == mech::NS) {
+        if (cfg->mode != LBM_MODE_STRICT || cfg->precision != LBM_FP64) {
+            g_create_error = "the multicomponent path runs in LBM_MODE_STRICT with LBM_FP64";
+            return LBM_E_UNSUPPORTED;
+        }
+        if (!(cfg->dx_si > 0.0) || !(cfg->dt_si > 0.0)) {
+            g_create_error = "multicomponent state needs dx_si > 0 and dt_si > 0 (units.set_m_kg_s)";
+            return LBM_E_ARG;
+        }
     }
     if (cfg->precision != LBM_FP64 && cfg->precision != LBM_FP32_SHIFTED) {
         g_create_error = "unknown precision";
@@ -248,7 +293,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         g_create_error = "unknown mode";
         return LBM_E_ARG;
     }
-    if (!(cfg->gamma > 1.0) || !(cfg->gas_const > 0.0) || !(cfg->prtl > 0.0) || !(cfg->nu >= 0.0)) {
+    if (cfg->n_species == 0 &&
+        (!(cfg->gamma > 1.0) || !(cfg->gas_const > 0.0) || !(cfg->prtl > 0.0) || !(cfg->nu >= 0.0))) {
         g_create_error = "gas constants out of range (gamma > 1, R > 0, Pr > 0, nu >= 0)";
         return LBM_E_ARG;
     }
@@ -281,9 +327,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("hipStreamCreate", e);
     const size_t pop_bytes = (size_t)n * Q * c->esize;
     for (int b = 0; b < 2; ++b) {
-        if ((e = hipMalloc(&c->pf[b], pop_bytes)) != hipSuccess) return bail("hipMalloc populations", e);
+        if (cfg->n_species == 0) {  // the mixture f exists only without species (lbm.hpp:62-64)
+            if ((e = hipMalloc(&c->pf[b], pop_bytes)) != hipSuccess) return bail("hipMalloc populations", e);
+            (void)hipMemsetAsync(c->pf[b], 0, pop_bytes, c->stream);
+            c->bytes += pop_bytes;
+        }
         if ((e = hipMalloc(&c->pg[b], pop_bytes)) != hipSuccess) return bail("hipMalloc populations", e);
-        (void)hipMemsetAsync(c->pf[b], 0, pop_bytes, c->stream);
         (void)hipMemsetAsync(c->pg[b], 0, pop_bytes, c->stream);
     }
     if ((e = hipMalloc(&c->type, n)) != hipSuccess) return bail("hipMalloc types", e);
@@ -299,10 +348,23 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     (void)hipMemsetAsync(c->qd_block, 0, c->esize * n * 3, c->stream);
     if ((e = hipMalloc(&c->tau_count, sizeof(unsigned long long))) != hipSuccess) return bail("hipMalloc", e);
     (void)hipMemsetAsync(c->tau_count, 0, sizeof(unsigned long long), c->stream);
+    if (cfg->n_species > 0) {
+        c->ns = cfg->n_species;
+        c->units = chem::make_units(cfg->dx_si, cfg->dt_si);
+        const size_t sbytes = (size_t)n * Q * c->ns * sizeof(double);
+        for (int b = 0; b < 2; ++b) {
+            if ((e = hipMalloc(&c->sf[b], sbytes)) != hipSuccess) return bail("hipMalloc species populations", e);
+            (void)hipMemsetAsync(c->sf[b], 0, sbytes, c->stream);
+        }
+        const size_t fb = sizeof(double) * (size_t)n * c->ns * 5;
+        if ((e = hipMalloc((void**)&c->sp_block, fb)) != hipSuccess) return bail("hipMalloc species fields", e);
+        (void)hipMemsetAsync(c->sp_block, 0, fb, c->stream);
+        c->bytes += 2 * sbytes + fb;
+    }
     c->diag_npart = diag_partials(c->d);
     if ((e = hipMalloc(&c->diag_part, sizeof(double) * 8 * c->diag_npart)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = hipMalloc(&c->diag_out, sizeof(double) * 8)) != hipSuccess) return bail("hipMalloc", e);
-    c->bytes = 4 * pop_bytes + 2 * n + sizeof(double) * n * LBM_FIELD_COUNT + c->esize * n * 3 +
+    c->bytes += 2 * pop_bytes + 2 * n + sizeof(double) * n * LBM_FIELD_COUNT + c->esize * n * 3 +
                sizeof(double) * (8 * c->diag_npart + 9);
     // MIXTURE defaults (lbm.hpp:70-83): rho = 1, p = T = 1/3.
     std::vector<double> tmp(n, 1.0);
@@ -350,6 +412,11 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                 if (keep) needs_tms = true;
                 flag[cidx] = (simple ? NB_SIMPLE : 0) | (keep ? NB_KEEP : 0);
             }
+    if (multi(c)) {
+        for (int64_t i = 0; i < d.n; ++i)
+            if (types[i] == TYPE_FS)
+                return fail(c, LBM_E_UNSUPPORTED, "free-slip walls are not provided for the multicomponent state");
+    }
     if (fused(c) && (needs_tms || !fused_ok))
         return fail(c, LBM_E_UNSUPPORTED,
                     "fused mode handles periodic, adiabatic (TYPE_A) and free-slip (TYPE_FS) neighbours; "
@@ -419,6 +486,14 @@ int lbm_set_populations(lbm_ctx* c, int which, const double* host)
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
     if (which < 0 || which > 3) return fail(c, LBM_E_ARG, "population set out of range");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (multi(c)) {
+        const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
+        const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
+        const size_t bytes = (size_t)c->d.n * Q * (sp ? c->ns : 1) * sizeof(double);
+        HIP_TRY(c, hipMemcpyAsync(sp ? c->sf[slot] : c->pg[slot], host, bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return LBM_OK;
+    }
     if (fused(c)) {
         // keep the other member of the f/g pair in the same buffer
         if (c->kind == lbm_ctx::POST_COLLISION && (which == LBM_POP_F || which == LBM_POP_G)) {
@@ -459,6 +534,14 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
     if (which < 0 || which > 3) return fail(c, LBM_E_ARG, "population set out of range");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (multi(c)) {
+        const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
+        const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
+        const size_t bytes = (size_t)c->d.n * Q * (sp ? c->ns : 1) * sizeof(double);
+        HIP_TRY(c, hipMemcpyAsync(host, sp ? c->sf[slot] : c->pg[slot], bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return LBM_OK;
+    }
     void* buf;
     const double* ref;
     const int slot = pop_slot(c, which, false, &buf, &ref);
@@ -477,11 +560,59 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
     return LBM_OK;
 }
 
+namespace {
+int species_field_ptr(lbm_ctx* c, int field, double** dev, size_t* count)
+{
+    if (!multi(c)) return fail(c, LBM_E_STATE, "no species in this context");
+    const int64_t n = c->d.n;
+    switch (field) {
+    case LBM_SP_X: *dev = c->sp_block; *count = (size_t)c->ns * n; return LBM_OK;
+    case LBM_SP_RHO: *dev = c->sp_block + (size_t)c->ns * n; *count = (size_t)c->ns * n; return LBM_OK;
+    case LBM_SP_U:
+    case LBM_SP_V:
+    case LBM_SP_W:
+        *dev = c->sp_block + (size_t)(2 + field - LBM_SP_U) * c->ns * n;
+        *count = (size_t)c->ns * n;
+        return LBM_OK;
+    default: return fail(c, LBM_E_ARG, "species field id out of range");
+    }
+}
+}  // namespace
+
+int lbm_set_species_field(lbm_ctx* c, int field, const double* host)
+{
+    if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    double* dev;
+    size_t cnt;
+    if (int rc = species_field_ptr(c, field, &dev, &cnt)) return rc;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipMemcpyAsync(dev, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_get_species_field(lbm_ctx* c, int field, double* host)
+{
+    if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    double* dev;
+    size_t cnt;
+    if (int rc = species_field_ptr(c, field, &dev, &cnt)) return rc;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipMemcpyAsync(host, dev, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
 int lbm_init(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede lbm_init");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (multi(c)) {
+        if (int rc = run_mc(c, McKernel::Init, "mc_init", 0, 0)) return rc;
+        c->fields_valid = true;
+        return LBM_OK;
+    }
     if (int rc = materialize(c)) return rc;
     if (int rc = run_init(c)) return rc;
     c->cur = 0;
@@ -492,12 +623,19 @@ int lbm_init(lbm_ctx* c)
     return LBM_OK;
 }
 
-int lbm_collide_species(lbm_ctx* c) { return c ? LBM_OK : LBM_E_ARG; }
+int lbm_collide_species(lbm_ctx* c)
+{
+    if (!c) return LBM_E_ARG;
+    if (!multi(c)) return LBM_OK;  // Collide_Species exists only with species (lbm-main.cpp:156-158)
+    if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
+    return run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+}
 
 int lbm_collide(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
+    if (multi(c)) return run_mc(c, McKernel::Collide, "mc_collide", 0, 1);
     if (!fused(c)) return run_collide_explicit(c, 0, 1);
     int rc;
     if (c->kind == lbm_ctx::EXPLICIT_F) {
@@ -519,6 +657,7 @@ int lbm_stream(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
+    if (multi(c)) return run_mc(c, McKernel::Stream, "mc_stream", 1, 0);
     if (fused(c)) return LBM_OK;  // streaming happens in the next pull
     return run_stream(c, 1, 0);
 }
@@ -536,6 +675,7 @@ int lbm_moments(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
+    if (multi(c)) return run_mc(c, McKernel::Moments, "mc_moments", 0, 0);
     if (!fused(c)) {
         if (int rc = run_moments(c, 0)) return rc;
         c->fields_valid = true;
@@ -570,8 +710,8 @@ int lbm_diagnostics(lbm_ctx* c, lbm_diag* out)
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (int rc = materialize(c)) return rc;
     if (int rc = launch(c, "diagnostics", [&] {
-            launch_diag(c->d, c->type, c->fld, c->cfg.gamma, c->cfg.gas_const, c->diag_part, c->diag_npart,
-                        c->diag_out, c->stream);
+            launch_diag(c->d, c->type, c->fld, c->cfg.gamma > 1.0 ? c->cfg.gamma : 1.4, c->diag_part,
+                        c->diag_npart, c->diag_out, c->stream);
         }))
         return rc;
     double h[8] = {};

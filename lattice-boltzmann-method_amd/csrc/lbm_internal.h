@@ -51,8 +51,9 @@ template <typename P> void launch_stream(const SingleView<P>& v, hipStream_t s);
 template <typename P> void launch_moments(const SingleView<P>& v, hipStream_t s);
 template <typename P> void launch_fused_collide(const SingleView<P>& v, hipStream_t s);
 template <typename P> void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t s);
-void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double R,
-                 double* partials, int npart, double* out8, hipStream_t s);
+// gamma: ratio of specific heats used for the Mach number |u| / sqrt(gamma p / rho)
+void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
+                 double* out8, hipStream_t s);
 int diag_partials(const Dims& d);
 
 }  // namespace lbm

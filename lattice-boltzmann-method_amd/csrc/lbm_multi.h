@@ -1,0 +1,31 @@
+// Device view of the multicomponent state (internal).
+#pragma once
+#include "chem_device.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+
+template <typename P>
+struct MultiView {
+    Dims d;
+    const uint8_t* type;
+    P* f;   // species populations of the source buffer, [NS][27][n]
+    P* fo;  // destination buffer
+    P* g;   // mixture energy populations, [27][n]
+    P* go;
+    double* fld[LBM_FIELD_COUNT];
+    double* hrr;   // [n] heat release rate
+    double* X;     // [NS][n] mole fractions
+    double* rhoa;  // [NS][n] species densities
+    double* ua;    // [3][NS][n] species velocities
+    chem::Units u;
+    int reaction;
+    unsigned long long* tau_count;
+};
+
+enum class McKernel { Init, CollideSpecies, Collide, Stream, Moments };
+
+template <typename P>
+void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);
+
+}  // namespace lbm

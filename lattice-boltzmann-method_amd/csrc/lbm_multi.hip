@@ -1,0 +1,549 @@
+// Multicomponent reacting D3Q27 kernels for gfx950 (the reference's
+// MULTICOMP path), one kernel per reference call.
+//
+// Storage: species populations [NS][27][ncell], mixture energy populations
+// [27][ncell], per-species fields [NS][ncell] (X, rho_a) and [3][NS][ncell]
+// (species velocity), mixture fields as in the single-component path.  One
+// thread per cell; species are processed one after another so that only 27
+// populations are live in registers at a time.
+//
+// Reference routines restated here:
+//   Init             lbm-init.cpp:85-146
+//   Collide_Species  lbm-collision.cpp:334-526
+//   Collide          lbm-collision.cpp:191-332
+//   Streaming        lbm-streaming.cpp:3-119
+//   calculate_moment lbm-moment.cpp:193-377
+#include "chem_device.h"
+#include "lbm_multi.h"
+
+namespace lbm {
+
+using namespace chem;
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+struct Ijk {
+    int i, j, k;
+};
+__device__ inline Ijk ijk(const Dims& d, int c)
+{
+    const int k = c % d.nz;
+    const int t = c / d.nz;
+    return {t / d.ny, t % d.ny, k};
+}
+__device__ inline int cell(const Dims& d, int i, int j, int k) { return (i * d.ny + j) * d.nz + k; }
+
+__device__ inline bool fluidlike(uint8_t t)
+{
+    return t == TYPE_F || t == TYPE_I || t == TYPE_O || t == TYPE_I_C || t == TYPE_O_C;
+}
+
+template <typename P>
+__device__ inline P& spop(P* base, int a, int l, int64_t n, int c)
+{
+    return base[((int64_t)a * Q + l) * n + c];
+}
+
+// product-form equilibrium of one direction from the per-axis factors
+template <typename T>
+__device__ inline T feq_dir(const Axis3<T> F[3], int l, T rho)
+{
+    return rho * pick(F[0], cx(l)) * pick(F[1], cy(l)) * pick(F[2], cz(l));
+}
+
+template <typename T>
+__device__ inline void feq_axes(Axis3<T> F[3], const T u[3], T theta, const T corr[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = psi_axis<T>(u[d], theta + u[d] * u[d] + corr[d]);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ Init
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_init(MultiView<P> v)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n || !fluidlike(v.type[c])) return;
+    const int64_t n = v.d.n;
+    double X[NS], Y[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) X[a] = v.X[a * n + c];
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) s += (X[a] > 0 ? X[a] : 0.0);
+#pragma unroll
+    for (int a = 0; a < NS; ++a) X[a] = (X[a] > 0 ? X[a] : 0.0) / s;
+    Y_from_X(X, Y);
+    const double Tsi = v.u.si_temp(v.fld[LBM_FIELD_T][c]);
+    const double psi = v.u.si_p(v.fld[LBM_FIELD_P][c]);
+    const double Wm = mean_W(Y);
+    const double rho = v.u.rho(psi * Wm / (R_UNIV * Tsi));
+    const double vel[3] = {v.fld[LBM_FIELD_U][c], v.fld[LBM_FIELD_V][c], v.fld[LBM_FIELD_W][c]};
+    const double U = v.u.energy_mass(u_mass(Tsi, Y));
+    v.fld[LBM_FIELD_RHO][c] = rho;
+    v.fld[LBM_FIELD_RHOE][c] = rho * (U + 0.5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]));
+    const double theta = v.u.energy_mass(R_UNIV * Tsi / Wm);
+    const GeqAxes<double> G = geq_axes<double>(theta, vel);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) v.go[(int64_t)l * n + c] = (P)geq_dir<double>(G, l, rho, U - 1.5 * theta);
+    const double zero[3] = {0, 0, 0};
+    for (int a = 0; a < NS; ++a) {
+        const double yr = Y[a] * rho;
+        const double ra = yr > SPECIES_MIN ? yr : SPECIES_MIN;
+        v.rhoa[a * n + c] = ra;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = vel[d];
+        const double th = v.u.energy_mass(R_UNIV * Tsi / W[a]);
+        Axis3<double> F[3];
+        feq_axes<double>(F, vel, th, zero);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) spop(v.fo, a, l, n, c) = (P)feq_dir<double>(F, l, ra);
+    }
+}
+
+// ------------------------------------------------------- Collide_Species
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    double* const* Fd = v.fld;
+    const double rho = Fd[LBM_FIELD_RHO][c], T = Fd[LBM_FIELD_T][c], p = Fd[LBM_FIELD_P][c];
+    const double umix[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
+    double ra[NS], Xa[NS], Y0[NS], src[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        ra[a] = v.rhoa[a * n + c];
+        Xa[a] = v.X[a * n + c];
+        Y0[a] = ra[a] / rho;
+        src[a] = 0.0;
+    }
+    normalize(Y0);
+    const double Tsi = v.u.si_temp(T);
+    const double rho_si = v.u.si_rho(rho);
+    if (v.reaction) {
+        double Uint = Fd[LBM_FIELD_RHOE][c] / rho;
+        for (int a = 0; a < NS; ++a) {
+            const double ux = v.ua[(0 * NS + a) * n + c], uy = v.ua[(1 * NS + a) * n + c],
+                         uz = v.ua[(2 * NS + a) * n + c];
+            Uint = Uint - 0.5 * ra[a] / rho * (ux * ux + uy * uy + uz * uz);
+        }
+        const double usi = v.u.si_energy_mass(Uint);
+        double Tcur = Tsi, Ycur[NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) Ycur[a] = Y0[a];
+        for (int sub = 0; sub < 2; ++sub) {  // two explicit-Euler sub-steps of dt/2
+            double wdot[NS];
+            net_production(Tcur, rho_si, Ycur, wdot);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) {
+                src[a] += 1.0 / 2.0 * (double)v.u.rho_dot(wdot[a] * W[a]);
+                Ycur[a] = (ra[a] + src[a]) / rho;
+            }
+            normalize(Ycur);
+            Tcur = T_from_UV(usi, Ycur, Tcur);
+        }
+        double hrr = 0;
+#pragma unroll
+        for (int a = 0; a < NS; ++a)
+            hrr += -1.0 * (double)v.u.energy_mass(h_RT(a, Tcur) * R_UNIV * Tcur / W[a]) * src[a];
+        v.hrr[c] = hrr;
+    }
+    // species viscosities and Wilke weights at (T, rho)
+    const double lnT = log(Tsi);
+    double mu[NS], den[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        mu[a] = species_viscosity(a, lnT);
+        den[a] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int b = a; b < NS; ++b) {
+            const double f1 = 1.0 + sqrt(mu[a] / mu[b]) * sqrt(sqrt(W[b] / W[a]));
+            const double pab = f1 * f1 / sqrt(8.0 * (1.0 + W[a] / W[b]));
+            den[a] += Xa[b] * pab;
+            if (b != a) den[b] += Xa[a] * (mu[b] / mu[a] * W[a] / W[b] * pab);
+        }
+    double om[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        const double visc = Xa[a] * (double)v.u.mu(mu[a]) / den[a];
+        om[a] = 2 * Xa[a] * p / (Xa[a] * p + 2 * visc);
+    }
+    // Maxwell-Stefan friction with the binary diffusivities of the state
+    const double p_si = pressure(Tsi, rho_si, Y0);
+    double Fa[3][NS];
+    for (int a = 0; a < NS; ++a) {
+        double acc[3] = {0, 0, 0};
+        const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
+                     uaz = v.ua[(2 * NS + a) * n + c];
+        for (int b = 0; b < NS; ++b) {
+            const double D = v.u.nu(binary_diff(a, b, lnT, p_si));
+            const double k = -1.0 * p * Xa[a] * Xa[b] / D;
+            acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
+            acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
+            acc[2] += k * (uaz - v.ua[(2 * NS + b) * n + c]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Fa[d][a] = acc[d] * 1.0 / ra[a];
+    }
+    const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
+    const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
+    const double* Tf = Fd[LBM_FIELD_T];
+    for (int a = 0; a < NS; ++a) {
+        P fl[Q];
+        double mom[3] = {0, 0, 0};
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            fl[l] = spop(v.f, a, l, n, c);
+            if (cx(l)) mom[0] += cx(l) * (double)fl[l];
+            if (cy(l)) mom[1] += cy(l) * (double)fl[l];
+            if (cz(l)) mom[2] += cz(l) * (double)fl[l];
+        }
+        const double uraw[3] = {mom[0] / ra[a], mom[1] / ra[a], mom[2] / ra[a]};
+        const double udu[3] = {uraw[0] + Fa[0][a], uraw[1] + Fa[1][a], uraw[2] + Fa[2][a]};
+        const double Ra = v.u.cp(R_UNIV / W[a]);
+        const double theta = Ra * T;
+        double dQ[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            auto Qf = [&](int m) {
+                const double r = v.rhoa[a * n + m], ud = v.ua[(d * NS + a) * n + m];
+                return r * ud * (1.0 - 3.0 * Ra * Tf[m]) - r * (ud * ud * ud);
+            };
+            dQ[d] = fd_central<double>(Qf(nm[d]), Qf(c), Qf(np[d]), v.type[nm[d]], v.type[np[d]]);
+        }
+        double corr[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) corr[d] = 1 * (2.0 - om[a]) / (2.0 * ra[a] * om[a]) * dQ[d];
+        const double zero[3] = {0, 0, 0};
+        Axis3<double> Fe[3], Fu[3], Fr[3];
+        feq_axes<double>(Fe, uraw, theta, corr);
+        feq_axes<double>(Fu, udu, theta, corr);
+        feq_axes<double>(Fr, umix, theta, zero);
+        const bool low = ra[a] <= SPECIES_MIN;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const double fe = feq_dir<double>(Fe, l, ra[a]);
+            const double fdu = feq_dir<double>(Fu, l, ra[a]);
+            const double fr = feq_dir<double>(Fr, l, src[a]);
+            if (low && fr <= 0.0) continue;  // post-collision value left as it was
+            const double f = fl[l];
+            spop(v.fo, a, l, n, c) = (P)(f + om[a] * (fe - f) + (fdu - fe) + fr);
+        }
+    }
+}
+
+// --------------------------------------------------------------- Collide
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    bool warn = false;
+    if (c < v.d.n && v.type[c] == TYPE_F) {
+        const int64_t n = v.d.n;
+        const Ijk x = ijk(v.d, c);
+        double* const* Fd = v.fld;
+        const double rho = Fd[LBM_FIELD_RHO][c], T = Fd[LBM_FIELD_T][c], p = Fd[LBM_FIELD_P][c];
+        const double vel[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
+        double ra[NS], Y0[NS], X0[NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            ra[a] = v.rhoa[a * n + c];
+            Y0[a] = ra[a] / rho;
+        }
+        normalize(Y0);
+        X_from_Y(Y0, X0);
+        const double Tsi = v.u.si_temp(T);
+        const double lnT = log(Tsi);
+        const double cp = v.u.cp(cp_mass(Tsi, Y0));
+        const double Wm = mean_W(Y0);
+        const double Rmix = v.u.cp(R_UNIV / Wm);
+        const double mu = v.u.mu(mix_viscosity(lnT, X0));
+        const double kap = v.u.thermal_conductivity(mix_conductivity(lnT, X0));
+        const double theta = v.u.energy_mass(R_UNIV * Tsi / Wm);
+        const double om1 = 2 * p * cp * 1 / (p * cp * 1 + 2 * kap);
+        const double om = 2 * p * 1 / (p * 1 + 2 * mu);
+        warn = (1 / om <= 0.5) || (1 / om1 <= 0.5);
+        double Uint = Fd[LBM_FIELD_RHOE][c] / rho;
+        for (int a = 0; a < NS; ++a) {
+            const double ux = v.ua[(0 * NS + a) * n + c], uy = v.ua[(1 * NS + a) * n + c],
+                         uz = v.ua[(2 * NS + a) * n + c];
+            Uint = Uint - 0.5 * ra[a] / rho * (ux * ux + uy * uy + uz * uz);
+        }
+        const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
+        const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
+        double qdiff[3] = {0, 0, 0}, qcorr[3] = {0, 0, 0};
+        for (int a = 0; a < NS; ++a) {
+            const double ha = v.u.energy_mass(h_RT(a, Tsi) * R_UNIV * Tsi / W[a]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double uad = v.ua[(d * NS + a) * n + c];
+                qdiff[d] += om1 / (-om + om1) * ha * ra[a] * (uad - vel[d]);
+                const double Yn = v.rhoa[a * n + nm[d]] / Fd[LBM_FIELD_RHO][nm[d]];
+                const double Yp = v.rhoa[a * n + np[d]] / Fd[LBM_FIELD_RHO][np[d]];
+                const double dY = fd_fuw<double>(Yn, ra[a] / rho, Yp, uad - vel[d], v.type[nm[d]], v.type[np[d]]);
+                qcorr[d] += 0.5 * (2.0 - om1) / (-om + om1) * 1 * p * ha * dY;
+            }
+        }
+        auto Rn = [&](int m) -> double {
+            if (!fluidlike(v.type[m])) return 0.0;
+            double sYW = 0, sY = 0;
+            for (int a = 0; a < NS; ++a) {
+                const double Ya = v.rhoa[a * n + m] / Fd[LBM_FIELD_RHO][m];
+                sYW += Ya / W[a];
+                sY += Ya;
+            }
+            return v.u.cp(R_UNIV * sYW / sY);
+        };
+        double dQ[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double* ud = Fd[LBM_FIELD_U + d];
+            auto Qf = [&](int m, double R) {
+                const double r = Fd[LBM_FIELD_RHO][m], u = ud[m];
+                return r * u * (1.0 - 3.0 * R * Fd[LBM_FIELD_T][m]) - r * (u * u * u);
+            };
+            dQ[d] = fd_central<double>(Qf(nm[d], Rn(nm[d])), Qf(c, Rmix), Qf(np[d], Rn(np[d])), v.type[nm[d]],
+                                       v.type[np[d]]);
+        }
+        double dq[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            double acc = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int idx = (a == b) ? a : (3 + a + b - 1);
+                const double eqP = (a == b ? p : 0.0) + rho * vel[a] * vel[b];
+                acc += vel[a] * (Fd[LBM_FIELD_PXX + idx][c] - eqP);
+            }
+            dq[b] = acc + qdiff[b] + qcorr[b] + 0.5 * 1 * vel[b] * dQ[b];
+        }
+        const GeqAxes<double> G = geq_axes<double>(theta, vel);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const double ge = geq_dir<double>(G, l, rho, Uint - 1.5 * theta);
+            const double cdq = 0.5 * (cx(l) * dq[0] + cy(l) * dq[1] + cz(l) * dq[2]);
+            const double gs = (c2(l) == 1) ? ge + cdq : ge;
+            const double g = v.g[(int64_t)l * n + c];
+            v.go[(int64_t)l * n + c] = (P)(g + om1 * (ge - g) + (om - om1) * (ge - gs));
+        }
+    }
+    const unsigned long long m = __ballot(warn);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1)
+        atomicAdd(v.tau_count, (unsigned long long)__popcll(m));
+}
+
+// ------------------------------------------------------------- Streaming
+// v.f/v.g: post-collision (source); v.fo/v.go: streamed (destination).
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    const int nx = v.d.nx, ny = v.d.ny, nz = v.d.nz;
+    for (int l = 0; l < Q; ++l) {
+        int in = x.i - cx(l), jn = x.j - cy(l), kn = x.k - cz(l);
+        const uint8_t tn = v.type[cell(v.d, in, jn, kn)];
+        if (tn == TYPE_S || tn == TYPE_A) {
+            for (int a = 0; a < NS; ++a) spop(v.fo, a, l, n, c) = spop(v.f, a, opp(l), n, c);
+            if (tn == TYPE_A) v.go[(int64_t)l * n + c] = v.g[(int64_t)opp(l) * n + c];
+        } else if (tn == TYPE_O || tn == TYPE_O_C || tn == TYPE_I || tn == TYPE_I_C || tn == TYPE_FS) {
+            // inflow/outflow sources are left for TMS_BC; free-slip walls are
+            // rejected for the multicomponent state at set-up
+        } else {
+            if (in == 0) in = nx - 2; else if (in == nx - 1) in = 1;
+            if (jn == 0) jn = ny - 2; else if (jn == ny - 1) jn = 1;
+            if (kn == 0) kn = nz - 2; else if (kn == nz - 1) kn = 1;
+            const int src = cell(v.d, in, jn, kn);
+            for (int a = 0; a < NS; ++a) spop(v.fo, a, l, n, c) = spop(v.f, a, l, n, src);
+            v.go[(int64_t)l * n + c] = v.g[(int64_t)l * n + src];
+        }
+    }
+}
+
+// ------------------------------------------------------- calculate_moment
+// Dense symmetric positive definite solve A x = b (Cholesky, lower factor).
+__device__ inline void cholesky_solve3(double A[NS][NS], double B[NS][3])
+{
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double s = A[j][j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s -= A[j][k] * A[j][k];
+        const double d = sqrt(s);
+        A[j][j] = d;
+#pragma unroll
+        for (int i = j + 1; i < NS; ++i) {
+            double t = A[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t -= A[i][k] * A[j][k];
+            A[i][j] = t / d;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            double t = B[i][r];
+#pragma unroll
+            for (int k = 0; k < i; ++k) t -= A[i][k] * B[k][r];
+            B[i][r] = t / A[i][i];
+        }
+#pragma unroll
+        for (int i = NS - 1; i >= 0; --i) {
+            double t = B[i][r];
+#pragma unroll
+            for (int k = i + 1; k < NS; ++k) t -= A[k][i] * B[k][r];
+            B[i][r] = t / A[i][i];
+        }
+    }
+}
+
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int64_t n = v.d.n;
+    double* const* Fd = v.fld;
+    // Cantera state of the previous moments (old T, rho, composition)
+    const double rho_old = Fd[LBM_FIELD_RHO][c];
+    const double Tsi_old = v.u.si_temp(Fd[LBM_FIELD_T][c]);
+    const double rho_old_si = v.u.si_rho(rho_old);
+    const double p_old = Fd[LBM_FIELD_P][c];
+    double Yold[NS], Xold[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        Yold[a] = v.rhoa[a * n + c] / rho_old;
+        Xold[a] = v.X[a * n + c];
+    }
+    normalize(Yold);
+    const double p_old_si = pressure(Tsi_old, rho_old_si, Yold);
+    const double lnT_old = log(Tsi_old);
+    // energy moments
+    double re = 0, qx = 0, qy = 0, qz = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const double g = v.g[(int64_t)l * n + c];
+        re += g;
+        if (cx(l)) qx += cx(l) * g;
+        if (cy(l)) qy += cy(l) * g;
+        if (cz(l)) qz += cz(l) * g;
+    }
+    // species moments
+    double Pt[6] = {0, 0, 0, 0, 0, 0};
+    double rr[NS], B[NS][3];
+    double rho = 0, m0 = 0, m1 = 0, m2 = 0;
+    for (int a = 0; a < NS; ++a) {
+        double r = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const double f = spop(v.f, a, l, n, c);
+            r += f;
+            if (cx(l)) mx += cx(l) * f;
+            if (cy(l)) my += cy(l) * f;
+            if (cz(l)) mz += cz(l) * f;
+            if (cx(l)) Pt[0] += f;
+            if (cy(l)) Pt[1] += f;
+            if (cz(l)) Pt[2] += f;
+            if (cx(l) * cy(l)) Pt[3] += (cx(l) * cy(l)) * f;
+            if (cx(l) * cz(l)) Pt[4] += (cx(l) * cz(l)) * f;
+            if (cy(l) * cz(l)) Pt[5] += (cy(l) * cz(l)) * f;
+        }
+        rr[a] = r > SPECIES_MIN ? (r > SPECIES_MIN ? r : SPECIES_MIN) : SPECIES_MIN;
+        B[a][0] = mx;
+        B[a][1] = my;
+        B[a][2] = mz;
+        rho += r;
+        m0 += mx;
+        m1 += my;
+        m2 += mz;
+    }
+    const double vel[3] = {m0 / rho, m1 / rho, m2 / rho};
+    // Maxwell-Stefan velocities: A u = (rho u)_a with the old state's D, p, X
+    double A[NS][NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int b = 0; b < NS; ++b) {
+            if (b < a) continue;
+            if (b == a) {
+                A[a][a] = rr[a] == 0.0 ? 1.0 : rr[a];
+            } else {
+                const double D = v.u.nu(binary_diff(a, b, lnT_old, p_old_si));
+                A[a][b] = -1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D;
+                A[b][a] = A[a][b];
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int b = 0; b < NS; ++b)
+            if (b != a) A[a][a] -= A[a][b];
+    cholesky_solve3(A, B);
+    // temperature from energy at the new composition
+    double Y[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Y[a] = rr[a] / rho;
+    normalize(Y);
+    double Uint = re / rho;
+    for (int a = 0; a < NS; ++a)
+        Uint = Uint - 0.5 * rr[a] / rho * (B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2]);
+    const double usi = v.u.si_energy_mass(Uint);
+    const double rho_si = v.u.si_rho(rho);
+    const double Tsi = T_from_UV(usi, Y, Tsi_old);
+    double X[NS];
+    X_from_Y(Y, X);
+    // write back
+    Fd[LBM_FIELD_RHO][c] = rho;
+    Fd[LBM_FIELD_U][c] = vel[0];
+    Fd[LBM_FIELD_V][c] = vel[1];
+    Fd[LBM_FIELD_W][c] = vel[2];
+    Fd[LBM_FIELD_RHOE][c] = re;
+    Fd[LBM_FIELD_QX][c] = qx;
+    Fd[LBM_FIELD_QY][c] = qy;
+    Fd[LBM_FIELD_QZ][c] = qz;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) Fd[LBM_FIELD_PXX + a][c] = Pt[a];
+    Fd[LBM_FIELD_T][c] = v.u.temp(Tsi);
+    Fd[LBM_FIELD_P][c] = v.u.p(pressure(Tsi, rho_si, Y));
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        v.rhoa[a * n + c] = rr[a];
+        v.X[a * n + c] = X[a];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = B[a][d];
+    }
+}
+
+// ------------------------------------------------------------- launchers
+static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+
+template <typename P>
+void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
+{
+    switch (k) {
+    case McKernel::Init: hipLaunchKernelGGL(k_mc_init<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::CollideSpecies:
+        hipLaunchKernelGGL(k_mc_collide_species<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        break;
+    case McKernel::Collide: hipLaunchKernelGGL(k_mc_collide<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::Stream: hipLaunchKernelGGL(k_mc_stream<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::Moments: hipLaunchKernelGGL(k_mc_moments<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    }
+}
+
+template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
+
+}  // namespace lbm

@@ -541,7 +541,7 @@ __device__ inline double wave_max(double x)
 
 __global__ __launch_bounds__(BLOCK) void k_diag(Dims d, const uint8_t* type, const double* rho, const double* u,
                                                 const double* vv, const double* w, const double* T,
-                                                const double* rhoe, double gamma, double R, double* part)
+                                                const double* rhoe, const double* p, double gamma, double* part)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     double acc[DIAG_Q] = {0, 0, 0, 0, 0, 0};
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag(Dims d, const uint8_t* type, con
         acc[3] = r * vort;
         const bool bad = !isfinite(r) || !isfinite(T[c]);
         acc[4] = bad ? 1.0 : 0.0;
-        const double mach = sqrt(uu) / sqrt(gamma * R * T[c]);
+        const double mach = sqrt(uu) / sqrt(gamma * p[c] / r);  // p / rho = R T of the mixture
         acc[5] = isfinite(mach) ? mach : 0.0;
     }
     __shared__ double sm[BLOCK / 64][DIAG_Q];
@@ -646,11 +646,12 @@ void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t 
 
 int diag_partials(const Dims& d) { return (int)((d.n + BLOCK - 1) / BLOCK); }
 
-void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double R, double* partials,
-                 int npart, double* out8, hipStream_t s)
+void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
+                 double* out8, hipStream_t s)
 {
     hipLaunchKernelGGL(k_diag, grid_of(d.n), dim3(BLOCK), 0, s, d, type, fld[LBM_FIELD_RHO], fld[LBM_FIELD_U],
-                       fld[LBM_FIELD_V], fld[LBM_FIELD_W], fld[LBM_FIELD_T], fld[LBM_FIELD_RHOE], gamma, R, partials);
+                       fld[LBM_FIELD_V], fld[LBM_FIELD_W], fld[LBM_FIELD_T], fld[LBM_FIELD_RHOE], fld[LBM_FIELD_P],
+                       gamma, partials);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(BLOCK), 0, s, partials, npart, out8);
 }
 
