@@ -29,6 +29,7 @@ struct Units {
     __host__ __device__ float si_temp(double x) const { return (float)x * K; }
     __host__ __device__ float energy_mass(double x) const { return (float)x * (s * s) / (m * m); }
     __host__ __device__ float si_energy_mass(double x) const { return (float)x * (m * m) / (s * s); }
+    __host__ __device__ float u(double x) const { return (float)x * s / m; }  // velocity
     __host__ __device__ float rho(double x) const
     {
         const double md = m;

@@ -65,6 +65,11 @@ struct lbm_ctx {
     void* sf[2] = {nullptr, nullptr};
     double* sp_block = nullptr;  // X, rho_a [NS][n] and u_a [3][NS][n]
     chem::Units units{};
+    // TMS_BC outflow interface cells (cell, direction) and unsupported boundary kinds
+    int* tms_cell = nullptr;
+    int* tms_dir = nullptr;
+    int tms_count = 0;
+    std::string tms_unsupported;
 
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -240,7 +245,8 @@ void release(lbm_ctx* c)
     }
     c->pending.clear();
     void* ptrs[] = {c->type, c->nbflag, c->pf[0], c->pf[1], c->pg[0], c->pg[1], c->fld_block, c->qd_block,
-                    c->tau_count, c->diag_part, c->diag_out, c->sf[0], c->sf[1], c->sp_block};
+                    c->tau_count, c->diag_part, c->diag_out, c->sf[0], c->sf[1], c->sp_block, c->tms_cell,
+                    c->tms_dir};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -412,10 +418,40 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                 if (keep) needs_tms = true;
                 flag[cidx] = (simple ? NB_SIMPLE : 0) | (keep ? NB_KEEP : 0);
             }
+    std::vector<int> tcell, tdir;
+    std::string tms_unsupported;
     if (multi(c)) {
         for (int64_t i = 0; i < d.n; ++i)
             if (types[i] == TYPE_FS)
                 return fail(c, LBM_E_UNSUPPORTED, "free-slip walls are not provided for the multicomponent state");
+        // the outflow sweep's interface detection (lbm-bc.cpp:141-155): the first
+        // direction whose upstream cell is an outflow ghost and whose downstream
+        // cell is fluid; only axis-aligned interfaces are reconstructed
+        auto at = [&](int i, int j, int k) { return types[((int64_t)i * d.ny + j) * d.nz + k]; };
+        for (int i = 1; i < d.nx - 1; ++i)
+            for (int j = 1; j < d.ny - 1; ++j)
+                for (int k = 1; k < d.nz - 1; ++k) {
+                    if (at(i, j, k) != TYPE_F) continue;
+                    for (int l = 0; l < Q; ++l) {
+                        const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
+                        if (tn == TYPE_I || tn == TYPE_I_C || tn == TYPE_S)
+                            tms_unsupported = "inflow (TYPE_I, TYPE_I_C) and isothermal-wall (TYPE_S) TMS "
+                                              "reconstruction is not provided for the multicomponent state";
+                    }
+                    for (int l = 0; l < Q; ++l) {
+                        const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
+                        const uint8_t tp = at(i + cx(l), j + cy(l), k + cz(l));
+                        if ((tn == TYPE_O || tn == TYPE_O_C) && tp == TYPE_F) {
+                            if (c2(l) == 1) {
+                                if (tn == TYPE_O)
+                                    tms_unsupported = "TYPE_O (zero-gradient) outflow is not provided; use TYPE_O_C";
+                                tcell.push_back((int)(((int64_t)i * d.ny + j) * d.nz + k));
+                                tdir.push_back(l);
+                            }
+                            break;
+                        }
+                    }
+                }
     }
     if (fused(c) && (needs_tms || !fused_ok))
         return fail(c, LBM_E_UNSUPPORTED,
@@ -425,6 +461,20 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
     HIP_TRY(c, hipMemcpyAsync(c->type, types, d.n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->nbflag, flag.data(), d.n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (multi(c)) {
+        if (c->tms_cell) (void)hipFree(c->tms_cell);
+        if (c->tms_dir) (void)hipFree(c->tms_dir);
+        c->tms_cell = c->tms_dir = nullptr;
+        c->tms_count = (int)tcell.size();
+        if (c->tms_count) {
+            HIP_TRY(c, hipMalloc((void**)&c->tms_cell, sizeof(int) * tcell.size()));
+            HIP_TRY(c, hipMalloc((void**)&c->tms_dir, sizeof(int) * tdir.size()));
+            HIP_TRY(c, hipMemcpy(c->tms_cell, tcell.data(), sizeof(int) * tcell.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->tms_dir, tdir.data(), sizeof(int) * tdir.size(), hipMemcpyHostToDevice));
+        }
+        c->tms_unsupported = tms_unsupported;
+        needs_tms = false;
+    }
     c->needs_tms = needs_tms;
     c->types_set = true;
     return LBM_OK;
@@ -665,6 +715,13 @@ int lbm_stream(lbm_ctx* c)
 int lbm_tms_bc(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
+    if (multi(c)) {
+        if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
+        if (!c->tms_unsupported.empty()) return fail(c, LBM_E_UNSUPPORTED, c->tms_unsupported);
+        return launch(c, "mc_tms_outflow", [&] {
+            launch_mc_tms<double>(make_mview(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
+        });
+    }
     if (c->needs_tms)
         return fail(c, LBM_E_UNSUPPORTED,
                     "single-component TMS reconstruction next to TYPE_S/I/O cells is not provided by this build");

@@ -28,4 +28,12 @@ enum class McKernel { Init, CollideSpecies, Collide, Stream, Moments };
 template <typename P>
 void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);
 
+// NSCBC sigma of the partially non-reflecting outflow (NSCBC_SIGMA_OUT, defines.hpp:100-102)
+constexpr double NSCBC_SIGMA_OUT = 0.25;
+
+// TMS_BC outflow reconstruction over `count` interface cells (cell index and
+// the axis-aligned direction whose upstream neighbour is the TYPE_O_C ghost).
+template <typename P>
+void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
+
 }  // namespace lbm

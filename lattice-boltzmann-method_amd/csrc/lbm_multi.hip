@@ -527,6 +527,244 @@ __global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
     }
 }
 
+
+// ---------------------------------------------------------------- TMS_BC
+// Outflow sweep of TMS_BC (lbm-bc.cpp:126-340) for TYPE_O_C ghosts with the
+// NSCBC target state of impose_nscbc.cpp:45-546.  One thread per interface
+// cell of the compact list built at lbm_set_types; each thread rewrites only
+// its own cell's populations, reading neighbours' moments of the last
+// calculate_moment.
+namespace {
+
+constexpr int NV = 5 + NS;  // p, u, v, w, rho, Y[NS]
+
+template <typename P>
+__device__ inline void cell_vals(const MultiView<P>& v, int c, double out[NV])
+{
+    const int64_t n = v.d.n;
+    out[0] = v.fld[LBM_FIELD_P][c];
+    out[1] = v.fld[LBM_FIELD_U][c];
+    out[2] = v.fld[LBM_FIELD_V][c];
+    out[3] = v.fld[LBM_FIELD_W][c];
+    const double r = v.fld[LBM_FIELD_RHO][c];
+    out[4] = r;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) out[5 + a] = v.rhoa[a * n + c] / r;
+}
+
+// one-sided second order into the domain (normal_derivative) or the
+// tangential rule of tangential_derivative (z repeats its k-1 test)
+template <typename P>
+__device__ inline void deriv(const MultiView<P>& v, int c, int stride, int mode, double D[NV])
+{
+    double a[NV], b[NV], e[NV];
+    if (mode == 0) {  // central
+        cell_vals(v, c + stride, a);
+        cell_vals(v, c - stride, b);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) D[q] = (a[q] - b[q]) / (2.0 * 1);
+        return;
+    }
+    cell_vals(v, c, a);
+    cell_vals(v, c + mode * stride, b);
+    cell_vals(v, c + 2 * mode * stride, e);
+    const double s = (double)mode;  // +1 forward, -1 backward
+#pragma unroll
+    for (int q = 0; q < NV; ++q) D[q] = s * (-1.5 * a[q] + 2.0 * b[q] - 0.5 * e[q]) / 1;
+}
+
+// T from (rho, p, Y) in SI (setState_DP) with the lattice crossings.
+__device__ inline double T_from_DP(const Units& u, double rho, double p, double Y[NS], double* Tsi_out)
+{
+    normalize(Y);
+    const double rho_si = u.si_rho(rho), p_si = u.si_p(p);
+    const double Tsi = p_si * mean_W(Y) / (R_UNIV * rho_si);
+    *Tsi_out = Tsi;
+    return u.temp(Tsi);
+}
+
+// frozen sound speed (lattice) and gamma at the DP state of a cell
+__device__ inline void sound_DP(const Units& u, double rho, double p, double Y[NS], double* a_lat, double* gam)
+{
+    double Tsi;
+    T_from_DP(u, rho, p, Y, &Tsi);
+    const double cp = cp_mass(Tsi, Y);
+    const double Wm = mean_W(Y);
+    const double cv = cp - R_UNIV / Wm;
+    *gam = cp / cv;
+    *a_lat = u.u(sqrt(*gam * R_UNIV * Tsi / Wm));
+}
+
+}  // namespace
+
+template <typename P>
+__global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells, const int* dirs, int count)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t], l = dirs[t];
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    const int strides[3] = {v.d.ny * v.d.nz, v.d.nz, 1};
+    const int Ns[3] = {v.d.nx - 2, v.d.ny - 2, v.d.nz - 2};
+    const int dims[3] = {v.d.nx, v.d.ny, v.d.nz};
+    const int cl[3] = {cx(l), cy(l), cz(l)};
+    const int idir = cl[0] ? 0 : (cl[1] ? 1 : 2);
+    const int isign = cl[idir];
+    double* const* Fd = v.fld;
+    // ---- impose_NSCBC: LODI waves at the boundary cell
+    double V0[NV];
+    cell_vals(v, c, V0);
+    const double p = V0[0], rho = V0[4];
+    const double vel[3] = {V0[1], V0[2], V0[3]};
+    double Yc[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yc[a] = V0[5 + a];
+    double a_s, gam;
+    sound_DP(v.u, rho, p, Yc, &a_s, &gam);
+    double Dn[NV];
+    deriv(v, c, strides[idir], isign, Dn);
+    double T1 = 0, T5 = 0;
+    {
+        int t1 = (idir + 1) % 3, t2 = (idir + 2) % 3;
+        if (t1 > t2) { const int tmp = t1; t1 = t2; t2 = tmp; }
+        double Dt[2][NV];
+        const int tt[2] = {t1, t2};
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int d = tt[q];
+#pragma unroll
+            for (int r = 0; r < NV; ++r) Dt[q][r] = 0.0;
+            if (Ns[d] > 3) {
+                any = true;
+                const int s = strides[d];
+                int mode;
+                if (v.type[c - s] != TYPE_F) mode = 1;
+                else if (d != 2 && v.type[c + s] != TYPE_F) mode = -1;
+                else mode = 0;
+                deriv(v, c, s, mode, Dt[q]);
+            }
+        }
+        if (any) {
+            // Yoo & Im transverse terms for the normal direction idir
+            T1 = vel[t1] * (Dt[0][0] - rho * a_s * Dt[0][1 + idir]) + vel[t2] * (Dt[1][0] - rho * a_s * Dt[1][1 + idir]) +
+                 gam * p * (Dt[0][1 + t1] + Dt[1][1 + t2]);
+            T5 = vel[t1] * (Dt[0][0] + rho * a_s * Dt[0][1 + idir]) + vel[t2] * (Dt[1][0] + rho * a_s * Dt[1][1 + idir]) +
+                 gam * p * (Dt[0][1 + t1] + Dt[1][1 + t2]);
+        }
+    }
+    const double un = vel[idir], dun = Dn[1 + idir], dpn = Dn[0], drhon = Dn[4];
+    const double mach = sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]) / a_s;
+    double L1 = (un - a_s) * (dpn - rho * a_s * dun);
+    double L5 = (un + a_s) * (dpn + rho * a_s * dun);
+    const double Ls = un * (a_s * a_s * drhon - dpn);
+    const int cb = c - isign * strides[idir];
+    if (mach < 1.0) {
+        const double Kout = NSCBC_SIGMA_OUT * (1.0 - mach * mach) * (a_s / dims[idir]);
+        if (isign == 1) L5 = Kout * (p - Fd[LBM_FIELD_P][cb]) - (1.0 - mach) * T5;
+        else L1 = Kout * (p - Fd[LBM_FIELD_P][cb]) - (1.0 - mach) * T1;
+    }
+    // update_bc_cells, dt = 1
+    const double drho = (Ls + 0.5 * (L1 + L5)) / (a_s * a_s);
+    double vout[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vout[d] = vel[d] - 1 * (d == idir ? (L5 - L1) / (2.0 * a_s * rho) : un * Dn[1 + d]);
+    const double rho_out = rho - 1 * drho;
+    const double p_out = p - 1 * (0.5 * (L1 + L5));
+    double ra_out[NS], Yo[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        ra_out[a] = (Yc[a] - 1 * (un * Dn[5 + a])) * rho_out;
+        Yo[a] = ra_out[a] / rho_out;
+    }
+    double Tsi_out;
+    const double T_out = T_from_DP(v.u, rho_out, p_out, Yo, &Tsi_out);
+    // ---- equilibria of the target state (calculate_feq_geq, lbm-edf.cpp:198-209)
+    const double Tso = v.u.si_temp(T_out);
+    const double U_out = v.u.energy_mass(u_mass(Tso, Yo));
+    const double th_out = v.u.energy_mass(R_UNIV * Tso / mean_W(Yo));
+    uint32_t unknown = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint8_t tq = v.type[cell(v.d, x.i - cx(q), x.j - cy(q), x.k - cz(q))];
+        if (tq == TYPE_O || tq == TYPE_O_C) unknown |= 1u << q;
+    }
+    const double zero[3] = {0, 0, 0};
+    // ---- local moments with the target populations in the unknown directions
+    double ra_loc[NS], ma[NS][3], re_loc = 0;
+    double rho_loc = 0, mloc[3] = {0, 0, 0};
+    for (int a = 0; a < NS; ++a) {
+        const double th_a = v.u.energy_mass(R_UNIV * Tso / W[a]);
+        Axis3<double> Fo[3];
+        feq_axes<double>(Fo, vout, th_a, zero);
+        double r = 0, m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out[a]) : (double)spop(v.f, a, q, n, c);
+            r += f;
+            if (cx(q)) m0 += cx(q) * f;
+            if (cy(q)) m1 += cy(q) * f;
+            if (cz(q)) m2 += cz(q) * f;
+        }
+        ra_loc[a] = r;
+        ma[a][0] = m0;
+        ma[a][1] = m1;
+        ma[a][2] = m2;
+    }
+    const GeqAxes<double> Go = geq_axes<double>(th_out, vout);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        re_loc += (unknown >> q & 1u) ? geq_dir<double>(Go, q, rho_out, U_out - 1.5 * th_out)
+                                      : (double)v.g[(int64_t)q * n + c];
+    for (int a = 0; a < NS; ++a) {
+        rho_loc += ra_loc[a];
+        mloc[0] += ma[a][0];
+        mloc[1] += ma[a][1];
+        mloc[2] += ma[a][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ma[a][d] = ra_loc[a] > 0.0 ? ma[a][d] / ra_loc[a] : 0.0;
+    }
+    double vloc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vloc[d] = rho_loc > 0 ? mloc[d] / rho_loc : 0.0;
+    double Uloc = re_loc / rho_loc;
+    for (int a = 0; a < NS; ++a)
+        Uloc = Uloc - 0.5 * ra_loc[a] / rho_loc * (ma[a][0] * ma[a][0] + ma[a][1] * ma[a][1] + ma[a][2] * ma[a][2]);
+    // calculate_temp (lbm-moment.cpp:540-553)
+    double Yl[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yl[a] = ra_loc[a] / rho_loc;
+    normalize(Yl);
+    const double T_loc = v.u.temp(T_from_UV(v.u.si_energy_mass(Uloc), Yl, v.u.si_temp(Fd[LBM_FIELD_T][c])));
+    const double Tsl = v.u.si_temp(T_loc);
+    const double U_loc = v.u.energy_mass(u_mass(Tsl, Yl));
+    const double th_loc = v.u.energy_mass(R_UNIV * Tsl / mean_W(Yl));
+    // ---- reconstruction: unknown 2 f_out - f_loc, known f_out + f - f_loc
+    for (int a = 0; a < NS; ++a) {
+        const double tho = v.u.energy_mass(R_UNIV * Tso / W[a]);
+        const double thl = v.u.energy_mass(R_UNIV * Tsl / W[a]);
+        Axis3<double> Fo[3], Fl[3];
+        feq_axes<double>(Fo, vout, tho, zero);
+        feq_axes<double>(Fl, ma[a], thl, zero);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double fo = feq_dir<double>(Fo, q, ra_out[a]);
+            const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
+            P& dst = spop(v.f, a, q, n, c);
+            dst = (unknown >> q & 1u) ? (P)(2 * fo - flc) : (P)(fo + (double)dst - flc);
+        }
+    }
+    const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double go = geq_dir<double>(Go, q, rho_out, U_out - 1.5 * th_out);
+        const double gl = geq_dir<double>(Gl, q, rho_loc, U_loc - 1.5 * th_loc);
+        P& dst = v.g[(int64_t)q * n + c];
+        dst = (unknown >> q & 1u) ? (P)(2 * go - gl) : (P)(go + (double)dst - gl);
+    }
+}
+
 // ------------------------------------------------------------- launchers
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
@@ -544,6 +782,13 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
     }
 }
 
+template <typename P>
+void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s)
+{
+    if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+}
+
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
+template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
 
 }  // namespace lbm

@@ -152,3 +152,48 @@ def reacting_box(nx, ny, nz, T_K=1400.0, X=None, amp_T=0.0, amp_u=0.0, seed=2026
     fl["X"] = np.zeros((len(H2AIR_SPECIES),) + shape)
     fl["X"][:, m] = X[:, None]
     return fl
+
+
+X_FUEL_ML = {"H2": 0.3, "N2": 0.7}  # N2-diluted hydrogen stream of the mixing layer
+
+
+def _all_species(d, minim=1e-6):
+    out = {n: minim for n in H2AIR_SPECIES}
+    out.update(d)
+    return out
+
+
+def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, noise=0.01, seed=20261206,
+                 X_fuel=None, X_ox=None, minim=1e-6):
+    """Temporal H2-air / hot-air mixing layer: periodic x and z, NSCBC outflow
+    (TYPE_O_C) on both y faces (the y ghost layers are typed last, so edges and
+    corners are TYPE_O_C).  Lower stream: N2-diluted H2 at T_fuel; upper
+    stream: air at T_ox; u = U_s tanh((y - y0)/delta) with seeded broadband
+    noise of `noise` U_s on u, v, w; p = 1 atm everywhere (also the outflow
+    target pressure held by the ghosts)."""
+    rng = np.random.default_rng(seed)
+    fl = default_fields(nx, ny, nz)
+    t = fl["type"]
+    t[0, :, :] = TYPE_P
+    t[-1, :, :] = TYPE_P
+    t[:, :, 0] = TYPE_P
+    t[:, :, -1] = TYPE_P
+    t[:, 0, :] = TYPE_O_C
+    t[:, -1, :] = TYPE_O_C
+    shape = t.shape
+    delta = delta or max(ny / 16.0, 1.0)
+    j = np.arange(shape[1], dtype=np.float64)
+    s = 0.5 * (1.0 + np.tanh((j - 0.5 * (shape[1] - 1)) / delta))  # 0 fuel side .. 1 oxidiser side
+    s3 = np.broadcast_to(s[None, :, None], shape)
+    Xf = mole_vector(_all_species(X_fuel or X_FUEL_ML, minim))
+    Xo = mole_vector(_all_species(X_ox or X_AIR, minim))
+    fl["X"] = (1.0 - s3)[None] * Xf[:, None, None, None] + s3[None] * Xo[:, None, None, None]
+    fl["T"][...] = (T_fuel + (T_ox - T_fuel) * s3) / LATTICE_K
+    fl["p"][...] = lattice_p(ONE_ATM)
+    fl["u"][...] = U_s * (2.0 * s3 - 1.0)
+    fl["v"][...] = 0.0
+    fl["w"][...] = 0.0
+    m = t == TYPE_F
+    for k in ("u", "v", "w"):
+        fl[k][m] += noise * U_s * rng.uniform(-1.0, 1.0, int(m.sum()))
+    return fl

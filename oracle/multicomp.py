@@ -485,3 +485,200 @@ class MultiState:
         return {"mass": float(self.rho[c].sum()), "energy": float(self.rhoe[c].sum()),
                 "species_mass": self.rho_a[:, c].sum(axis=1), "T_max": float(self.T[c].max()),
                 "HRR": float(self.HRR[c].sum())}
+
+
+# --------------------------------------------------------------------- TMS
+def _onlyone(a, b, c):
+    """onlyOne_1 (math-util.cpp:89-100): exactly one component is +-1."""
+    return (abs(a) + abs(b) + abs(c)) == 1
+
+
+class MultiStateBC(MultiState):
+    """MultiState plus the multicomponent TMS_BC outflow branch with the NSCBC
+    target state (lbm-bc.cpp:126-340, impose_nscbc.cpp:45-546) for TYPE_O_C
+    ghosts."""
+
+    SIGMA_OUT = 0.25  # NSCBC_SIGMA_OUT, defines.hpp:100-102
+
+    def _Y_at(self, c):
+        return self.rho_a[:, c] / self.rho[c]
+
+    def _state_DP(self, rho, p, Y):
+        """setState_DP at lattice (rho, p) and Y: returns (T_si, Yn)."""
+        U_, m = self.units, self.mech
+        Yn = m.normalize_Y(Y)
+        rho_si = np.float64(U_.si_rho(rho))
+        p_si = np.float64(U_.si_p(p))
+        T_si = p_si * m.W_mean(Yn) / (m.R * rho_si)
+        return T_si, Yn
+
+    def _sound(self, c):
+        """Frozen sound speed (lattice) and gamma at the DP state of cell c."""
+        U_, m = self.units, self.mech
+        T_si, Yn = self._state_DP(self.rho[c], self.p[c], self._Y_at(c))
+        cp = m.cp_mass(np.array(T_si), Yn)
+        cv = cp - m.R / m.W_mean(Yn)
+        gam = cp / cv
+        a_si = np.sqrt(gam * m.R * T_si / m.W_mean(Yn))
+        return np.float64(U_.u(a_si)), gam, Yn, T_si
+
+    def _vals(self, c):
+        """(p, u, v, w, rho, Y[ns]) of cell c."""
+        return np.concatenate([[self.p[c], self.u[c], self.v[c], self.w[c], self.rho[c]], self._Y_at(c)])
+
+    def _normal_derivative(self, c, idir, isign):
+        s = self._strides()[idir]
+        if isign == 1:
+            return (-1.5 * self._vals(c) + 2.0 * self._vals(c + s) - 0.5 * self._vals(c + 2 * s)) / 1
+        return (1.5 * self._vals(c) - 2.0 * self._vals(c - s) + 0.5 * self._vals(c - 2 * s)) / 1
+
+    def _tangential_derivative(self, c, idir):
+        s = self._strides()[idir]
+        tm, tp = self.type[c - s], self.type[c + s]
+        if tm != TYPE_F:
+            return (-1.5 * self._vals(c) + 2.0 * self._vals(c + s) - 0.5 * self._vals(c + 2 * s)) / 1
+        # the z branch tests k-1 twice (impose_nscbc.cpp:233,242): its
+        # backward stencil is unreachable and the central one is used instead
+        if idir != 2 and tp != TYPE_F:
+            return (1.5 * self._vals(c) - 2.0 * self._vals(c - s) + 0.5 * self._vals(c - 2 * s)) / 1
+        return (self._vals(c + s) - self._vals(c - s)) / (2.0 * 1)
+
+    def _strides(self):
+        nx, ny, nz = self.dims
+        return (ny * nz, nz, 1)
+
+    def _nscbc(self, c, l):
+        """impose_NSCBC for the interface cell c whose TYPE_O_C neighbour is
+        c - e_l: returns rho_out, rhoa_out, vel_out, T_out (lattice)."""
+        U_, m = self.units, self.mech
+        cvec = (CX[l], CY[l], CZ[l])
+        idir = int(np.nonzero(cvec)[0][0])
+        isign = int(cvec[idir])
+        Ns = [d - 2 for d in self.dims]
+        D = [np.zeros(5 + self.ns) for _ in range(3)]
+        D[idir] = self._normal_derivative(c, idir, isign)
+        others = [d for d in range(3) if d != idir]
+        for d in others:
+            if Ns[d] > 3:
+                D[d] = self._tangential_derivative(c, d)
+        Tt = np.zeros(5)
+        a, gam, Yn, T_si = self._sound(c)
+        rho, p = self.rho[c], self.p[c]
+        vel = np.array([self.u[c], self.v[c], self.w[c]])
+        if any(Ns[d] > 3 for d in others):
+            inv_rho = 1.0 / rho
+            dp = [D[d][0] for d in range(3)]
+            du = [D[d][1:4] for d in range(3)]  # du[d][comp]
+            drho = [D[d][4] for d in range(3)]
+            n, t1, t2 = idir, others[0], others[1]
+            # Yoo & Im transverse terms (impose_nscbc.cpp:264-335), normal n
+            T1 = vel[t1] * (dp[t1] - rho * a * du[t1][n]) + vel[t2] * (dp[t2] - rho * a * du[t2][n]) + \
+                gam * p * (du[t1][t1] + du[t2][t2])
+            T5 = vel[t1] * (dp[t1] + rho * a * du[t1][n]) + vel[t2] * (dp[t2] + rho * a * du[t2][n]) + \
+                gam * p * (du[t1][t1] + du[t2][t2])
+            Tt = np.array([T1, 0.0, 0.0, 0.0, T5])
+        Dn = D[idir]
+        dpn, dvel, drhon, dY = Dn[0], Dn[1:4], Dn[4], Dn[5:]
+        un = vel[idir]
+        dun = dvel[idir]
+        mach = np.sqrt(vel @ vel) / a
+        strides = self._strides()
+        cb = c - isign * strides[idir]
+        L1 = (un - a) * (dpn - rho * a * dun)
+        L5 = (un + a) * (dpn + rho * a * dun)
+        Ls = un * (a * a * drhon - dpn)
+        Lt = un * dvel
+        L6 = un * dY
+        if self.type[cb] != TYPE_O_C:
+            raise ValueError("only TYPE_O_C boundaries are handled")
+        if mach < 1.0:
+            Kout = self.SIGMA_OUT * (1.0 - mach * mach) * (a / self.dims[idir])
+            if isign == 1:
+                L5 = Kout * (p - self.p[cb]) - (1.0 - mach) * Tt[4]
+            else:
+                L1 = Kout * (p - self.p[cb]) - (1.0 - mach) * Tt[0]
+        # update_bc_cells (impose_nscbc.cpp:479-545), dt = 1
+        drho = (Ls + 0.5 * (L1 + L5)) / (a * a)
+        dvel_t = Lt.copy()
+        dvel_t[idir] = (L5 - L1) / (2.0 * a * rho)
+        dp_t = 0.5 * (L1 + L5)
+        rho_out = rho - 1 * drho
+        vel_out = vel - 1 * dvel_t
+        p_out = p - 1 * dp_t
+        rhoa_out = (self.rho_a[:, c] / rho - 1 * L6) * rho_out
+        T_si_out, _ = self._state_DP(rho_out, p_out, rhoa_out / rho_out)
+        return rho_out, rhoa_out, vel_out, np.float64(U_.temp(T_si_out))
+
+    def _feq_geq(self, rho, rhoa, vel, vela, T):
+        """calculate_feq_geq (lbm-edf.cpp:168-196): equilibria at (T, rho, Y)
+        with species velocities vela."""
+        U_, m = self.units, self.mech
+        Y = m.normalize_Y(rhoa / rho)
+        T_si = np.float64(U_.si_temp(T))
+        Uint = np.float64(U_.energy_mass(m.u_mass(np.array(T_si), Y)))
+        theta = np.float64(U_.energy_mass(m.R * T_si / m.W_mean(Y)))
+        fa = np.zeros((self.ns, 27))
+        for a in range(self.ns):
+            th_a = np.float64(U_.energy_mass(m.R * T_si / m.W[a]))
+            fa[a] = feq(np.array([rhoa[a]]), vela[a][:, None], np.array([th_a]), np.zeros((3, 1)))[:, 0]
+        g = geq(np.array([rho]), np.array([Uint]), np.array([theta]), vel[:, None])[:, 0]
+        return fa, g
+
+    def _calculate_temp(self, U, rho, rhoa, T0_si):
+        """calculate_temp (lbm-moment.cpp:540-553): T from (U, rho, Y) by UV."""
+        U_, m = self.units, self.mech
+        Y = m.normalize_Y(rhoa / rho)
+        T_si = m.T_from_UV(np.float64(U_.si_energy_mass(U)), Y[None], np.array([T0_si]))[0]
+        return np.float64(U_.temp(T_si))
+
+    def interface_cells(self):
+        """(cell, l_interface) of every fluid cell whose first outflow-ghost
+        direction is axis-aligned (lbm-bc.cpp:141-155)."""
+        out = []
+        for c in np.nonzero(self.type == TYPE_F)[0]:
+            i, j, k = self._ijk(c)
+            for l in range(27):
+                tn = self.type[self._idx(i - CX[l], j - CY[l], k - CZ[l])]
+                tp = self.type[self._idx(i + CX[l], j + CY[l], k + CZ[l])]
+                if tn in (TYPE_O, TYPE_O_C) and tp == TYPE_F:
+                    if _onlyone(CX[l], CY[l], CZ[l]):
+                        out.append((int(c), l))
+                    break
+        return out
+
+    def tms_bc(self):
+        """TMS_BC outflow sweep (lbm-bc.cpp:126-340) for TYPE_O_C ghosts."""
+        m, U_ = self.mech, self.units
+        updates = []
+        for c, l in self.interface_cells():
+            i, j, k = self._ijk(c)
+            rho_out, rhoa_out, vel_out, T_out = self._nscbc(c, l)
+            fa_out, g_out = self._feq_geq(rho_out, rhoa_out, vel_out, np.repeat(vel_out[None], self.ns, 0), T_out)
+            unknown = np.array([self.type[self._idx(i - CX[q], j - CY[q], k - CZ[q])] in (TYPE_O, TYPE_O_C)
+                                for q in range(27)])
+            fsrc = np.where(unknown[None, :], fa_out, self.f[:, :, c])
+            gsrc = np.where(unknown, g_out, self.g[:, c])
+            rhoa_loc = fsrc.sum(axis=1)
+            mom = fsrc @ C3.T.astype(np.float64)  # (ns, 3)
+            rhoe_loc = gsrc.sum()
+            rho_loc = rhoa_loc.sum()
+            vel_loc = mom.sum(axis=0) / rho_loc if rho_loc > 0 else np.zeros(3)
+            vela_loc = np.where(rhoa_loc[:, None] > 0, mom / np.where(rhoa_loc > 0, rhoa_loc, 1.0)[:, None], 0.0)
+            Uint = rhoe_loc / rho_loc
+            for a in range(self.ns):
+                Uint = Uint - 0.5 * rhoa_loc[a] / rho_loc * (vela_loc[a] @ vela_loc[a])
+            T_loc = self._calculate_temp(Uint, rho_loc, rhoa_loc, np.float64(U_.si_temp(self.T[c])))
+            fa_loc, g_loc = self._feq_geq(rho_loc, rhoa_loc, vel_loc, vela_loc, T_loc)
+            fnew = np.where(unknown[None, :], 2 * fa_out - fa_loc, fa_out + self.f[:, :, c] - fa_loc)
+            gnew = np.where(unknown, 2 * g_out - g_loc, g_out + self.g[:, c] - g_loc)
+            updates.append((c, fnew, gnew))
+        for c, fnew, gnew in updates:
+            self.f[:, :, c] = fnew
+            self.g[:, c] = gnew
+
+    def step(self, reaction=True):
+        self.collide_species(reaction)
+        self.collide()
+        self.stream()
+        self.tms_bc()
+        self.moments()

@@ -13,7 +13,7 @@ import pytest
 
 from lbm_amd import cases
 from lbm_amd.capi import LBM_MODE_STRICT, POP_F, POP_FPC, POP_G, POP_GPC, MixtureLattice
-from oracle.multicomp import MultiState, Units
+from oracle.multicomp import MultiStateBC, Units
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,7 @@ def pair(fl, reaction=True):
     nx, ny, nz = (s - 2 for s in fl["type"].shape)
     lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=reaction, mode=LBM_MODE_STRICT)
     lat.load_case(fl)
-    s = MultiState(nx + 2, ny + 2, nz + 2, Units(*cases.UNITS_FLAME))
+    s = MultiStateBC(nx + 2, ny + 2, nz + 2, Units(*cases.UNITS_FLAME))
     s.type[:] = fl["type"].ravel()
     for k in ("rho", "u", "v", "w", "p", "T"):
         getattr(s, k)[:] = fl[k].ravel()
@@ -134,3 +134,43 @@ def test_mc_deterministic():
         lat.step(4)
         out.append((lat.get_populations(POP_F), lat.get_field("T")))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_mc_mixing_layer_outflow_matches_oracle(reaction):
+    """Periodic x/z, NSCBC outflow on both y faces: TMS reconstruction with
+    the LODI target state, call by call."""
+    fl = cases.mixing_layer(4, 8, 4, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
+    lat, s = pair(fl, reaction=reaction)
+    lat.Init()
+    s.init()
+    F = fluid(s)
+    for step in range(1, 4):
+        lat.Collide_Species()
+        lat.Collide()
+        lat.Streaming()
+        s.collide_species(reaction)
+        s.collide()
+        s.stream()
+        lat.TMS_BC()
+        s.tms_bc()
+        e = relerr(lat.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], s.f[:, :, F])
+        assert e < TOL, f"step {step} species f after TMS {e:.3e}"
+        e = relerr(lat.get_populations(POP_G).reshape(27, -1)[:, F], s.g[:, F])
+        assert e < TOL, f"step {step} g after TMS {e:.3e}"
+        lat.calculate_moment()
+        s.moments()
+        check_state(lat, s, TOL, f"step {step}")
+
+
+def test_mc_unsupported_boundaries_fail_loudly():
+    fl = cases.mixing_layer(4, 6, 4)
+    fl["type"][:, 0, :] = cases.TYPE_I_C
+    lat, _ = pair(fl)
+    lat.Init()
+    lat.Collide_Species()
+    lat.Collide()
+    lat.Streaming()
+    from lbm_amd.capi import LbmError
+    with pytest.raises(LbmError, match="inflow"):
+        lat.TMS_BC()
