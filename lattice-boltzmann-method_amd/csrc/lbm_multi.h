@@ -21,7 +21,22 @@ struct MultiView {
     chem::Units u;
     int reaction;
     unsigned long long* tau_count;
+    // chemistry surrogate (all null when unused)
+    unsigned step;                 // step counter for sampling hashes
+    float* cap_buf;                // capture rows [cap_max][32]: features 16 | targets 16
+    unsigned long long* cap_count;
+    long long cap_max;
+    int cap_stride;
+    const float* sur_pred;         // [n][16] normalised network outputs
+    const uint8_t* sur_flag;       // [n] 1: outside the training envelope
+    const float* sur_stats;        // feature mean/std [16]x2, target mean/std [16]x2, envelope lo/hi [16]x2
+    unsigned long long* sur_count; // fallback, audited, audit failures, max audit error (float bits)
+    int audit_stride;
+    float audit_tol;
 };
+
+// layout of sur_stats
+enum { SS_FMEAN = 0, SS_FSTD = 16, SS_TMEAN = 32, SS_TSTD = 48, SS_LO = 64, SS_HI = 80, SS_SIZE = 96 };
 
 enum class McKernel { Init, CollideSpecies, Collide, Stream, Moments };
 
@@ -35,5 +50,9 @@ constexpr double NSCBC_SIGMA_OUT = 0.25;
 // the axis-aligned direction whose upstream neighbour is the TYPE_O_C ghost).
 template <typename P>
 void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
+
+// Normalised surrogate features [n][16] and envelope flags of every fluid cell.
+template <typename P>
+void launch_mc_sur_features(const MultiView<P>& v, float* X, uint8_t* flag, hipStream_t s);
 
 }  // namespace lbm

@@ -106,6 +106,43 @@ __global__ __launch_bounds__(BLOCK) void k_mc_init(MultiView<P> v)
 }
 
 // ------------------------------------------------------- Collide_Species
+__device__ inline unsigned mix_hash(unsigned c, unsigned step)
+{
+    unsigned h = c * 0x9E3779B1u ^ (step * 0x85EBCA77u + 0x165667B1u);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h;
+}
+
+// The reference's chemistry source (lbm-collision.cpp:370-393): two explicit
+// Euler sub-steps of dt/2 of the net production rates, each followed by a
+// UV temperature re-solve at the new composition.  src[a] is the species
+// mass added over the step, *Tout the final temperature [K].
+template <typename P>
+__device__ inline void plain_chemistry(const MultiView<P>& v, double rho, double rho_si, const double ra[NS],
+                                       const double Y0[NS], double Tsi, double usi, double src[NS], double* Tout)
+{
+    double Tcur = Tsi, Ycur[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        Ycur[a] = Y0[a];
+        src[a] = 0.0;
+    }
+    for (int sub = 0; sub < 2; ++sub) {
+        double wdot[NS];
+        net_production(Tcur, rho_si, Ycur, wdot);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            src[a] += 1.0 / 2.0 * (double)v.u.rho_dot(wdot[a] * W[a]);
+            Ycur[a] = (ra[a] + src[a]) / rho;
+        }
+        normalize(Ycur);
+        Tcur = T_from_UV(usi, Ycur, Tcur);
+    }
+    *Tout = Tcur;
+}
+
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
 {
@@ -135,19 +172,61 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
             Uint = Uint - 0.5 * ra[a] / rho * (ux * ux + uy * uy + uz * uz);
         }
         const double usi = v.u.si_energy_mass(Uint);
-        double Tcur = Tsi, Ycur[NS];
-#pragma unroll
-        for (int a = 0; a < NS; ++a) Ycur[a] = Y0[a];
-        for (int sub = 0; sub < 2; ++sub) {  // two explicit-Euler sub-steps of dt/2
-            double wdot[NS];
-            net_production(Tcur, rho_si, Ycur, wdot);
+        double Tcur = Tsi;
+        bool plain = true;
+        if (v.sur_pred) {
+            // learned source: d_a = src_a / rho and dT predicted from (T, p, Y, dt)
+            const float* pr = v.sur_pred + (int64_t)c * 16;
+            const float* st = v.sur_stats;
+            double sum = 0;
 #pragma unroll
             for (int a = 0; a < NS; ++a) {
-                src[a] += 1.0 / 2.0 * (double)v.u.rho_dot(wdot[a] * W[a]);
-                Ycur[a] = (ra[a] + src[a]) / rho;
+                src[a] = rho * ((double)pr[1 + a] * st[SS_TSTD + 1 + a] + st[SS_TMEAN + 1 + a]);
+                sum += src[a];
             }
-            normalize(Ycur);
-            Tcur = T_from_UV(usi, Ycur, Tcur);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) src[a] -= Y0[a] * sum;  // keep sum_a src_a = 0
+            Tcur = Tsi + ((double)pr[0] * st[SS_TSTD] + st[SS_TMEAN]);
+            plain = v.sur_flag[c] != 0;
+            const bool audit = v.audit_stride > 0 && mix_hash(c, v.step) % (unsigned)v.audit_stride == 0;
+            if (plain) atomicAdd(&v.sur_count[0], 1ull);
+            if (audit && !plain) {
+                double sp[NS], Tp;
+                plain_chemistry(v, rho, rho_si, ra, Y0, Tsi, usi, sp, &Tp);
+                float err = (float)(fabs(Tp - Tcur) / st[SS_TSTD]);
+#pragma unroll
+                for (int a = 0; a < NS; ++a) {
+                    const float e = (float)(fabs(sp[a] - src[a]) / (rho * st[SS_TSTD + 1 + a]));
+                    err = e > err ? e : err;
+                }
+                atomicAdd(&v.sur_count[1], 1ull);
+                atomicMax(&v.sur_count[3], (unsigned long long)__float_as_uint(err));
+                if (err > v.audit_tol) {  // the audit overrules the network for this cell
+                    atomicAdd(&v.sur_count[2], 1ull);
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) src[a] = sp[a];
+                    Tcur = Tp;
+                }
+            }
+        }
+        if (plain) plain_chemistry(v, rho, rho_si, ra, Y0, Tsi, usi, src, &Tcur);
+        if (v.cap_buf && !v.sur_pred && mix_hash(c, v.step) % (unsigned)v.cap_stride == 0) {
+            const unsigned long long slot = atomicAdd(v.cap_count, 1ull);
+            if ((long long)slot < v.cap_max) {
+                float* row = v.cap_buf + slot * 32;
+                row[0] = (float)Tsi;
+                row[1] = (float)pressure(Tsi, rho_si, Y0);
+#pragma unroll
+                for (int a = 0; a < NS; ++a) row[2 + a] = (float)Y0[a];
+                row[11] = v.u.s;
+#pragma unroll
+                for (int q = 12; q < 16; ++q) row[q] = 0.f;
+                row[16] = (float)(Tcur - Tsi);
+#pragma unroll
+                for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / rho);
+#pragma unroll
+                for (int q = 26; q < 32; ++q) row[q] = 0.f;
+            }
         }
         double hrr = 0;
 #pragma unroll
@@ -765,6 +844,47 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     }
 }
 
+// ------------------------------------------------ surrogate features
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_sur_features(MultiView<P> v, float* X, uint8_t* flag)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n) return;
+    float* row = X + (int64_t)c * 16;
+    if (v.type[c] != TYPE_F) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) row[q] = 0.f;
+        flag[c] = 0;
+        return;
+    }
+    const int64_t n = v.d.n;
+    const double rho = v.fld[LBM_FIELD_RHO][c];
+    double Y0[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Y0[a] = v.rhoa[a * n + c] / rho;
+    normalize(Y0);
+    const double Tsi = v.u.si_temp(v.fld[LBM_FIELD_T][c]);
+    const double rho_si = v.u.si_rho(rho);
+    float raw[16];
+    raw[0] = (float)Tsi;
+    raw[1] = (float)pressure(Tsi, rho_si, Y0);
+#pragma unroll
+    for (int a = 0; a < NS; ++a) raw[2 + a] = (float)Y0[a];
+    raw[11] = v.u.s;
+    const float* st = v.sur_stats;
+    bool out = false;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        if (q < 12) {
+            out |= raw[q] < st[SS_LO + q] || raw[q] > st[SS_HI + q];
+            row[q] = (raw[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q];
+        } else {
+            row[q] = 0.f;
+        }
+    }
+    flag[c] = out ? 1 : 0;
+}
+
 // ------------------------------------------------------------- launchers
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
@@ -788,7 +908,14 @@ void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int
     if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
 }
 
+template <typename P>
+void launch_mc_sur_features(const MultiView<P>& v, float* X, uint8_t* flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mc_sur_features<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, X, flag);
+}
+
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
+template void launch_mc_sur_features<double>(const MultiView<double>&, float*, uint8_t*, hipStream_t);
 template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
 
 }  // namespace lbm

@@ -1,0 +1,50 @@
+// Chemistry surrogate MLP: device structures and host entry points (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+namespace sur {
+
+constexpr int NL = 4;                              // weight layers
+constexpr int DIMS[NL + 1] = {16, 128, 128, 128, 16};
+constexpr int NIN = 12;                            // T, p, Y_k (9), dt
+constexpr int NOUT = 10;                           // dT, dY_k (9)
+
+// FP32 master weights W[L] [DIMS[L+1]][DIMS[L]], biases, Adam moments and the
+// BF16 copies (row-major and transposed) the MFMA kernels read.
+struct Mlp {
+    float *W[NL] = {}, *b[NL] = {}, *mW[NL] = {}, *vW[NL] = {}, *mb[NL] = {}, *vb[NL] = {};
+    uint16_t *Wb[NL] = {}, *WbT[NL] = {};
+};
+
+// Activations and gradient scratch for batches of up to `cap` rows.
+struct Work {
+    int cap = 0, chunk = 256;
+    float* Z[NL] = {};
+    uint16_t* H[NL] = {};
+    uint16_t* dZ[NL] = {};
+    uint16_t *dZT = nullptr, *XT = nullptr;
+    float* part = nullptr;
+    float *gW[NL] = {}, *gb[NL] = {};
+    float* loss_rows = nullptr;
+};
+
+hipError_t mlp_alloc(Mlp& net, uint64_t seed, hipStream_t s);
+void mlp_free(Mlp& net);
+hipError_t work_alloc(Work& w, int cap);
+void work_free(Work& w);
+void refresh_bf16(Mlp& net, hipStream_t s);
+void to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s);
+// Forward pass of M rows (M multiple of 16); the outputs are w.Z[NL-1] [M][16].
+void mlp_forward(const Mlp& net, const uint16_t* X, int M, Work& w, hipStream_t s);
+// Gradients of the mean-squared loss over the NOUT outputs for one batch (M
+// multiple of 16) into w.gW / w.gb; per-row losses land in w.loss_rows.
+void mlp_grad(const Mlp& net, const uint16_t* X, const float* T, int M, Work& w, hipStream_t s);
+// One Adam step (step >= 1) with the gradients in w.
+void mlp_adam(Mlp& net, Work& w, float lr, int step, hipStream_t s);
+// Fused inference: Y [M][16] FP32 from X [M][16] FP32 (any M), activations in LDS.
+void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s);
+
+}  // namespace sur
+}  // namespace lbm

@@ -146,11 +146,12 @@ def test_golden_tgv64_series_and_nan_onset(golden_dir):
     assert onset_ref > 100
     # The oracle against the same reference run shows the set-up's own
     # instability: differences stay ~1e-14 through step 60, then grow ~1000x
-    # per 10 steps (1.9e-8 in enstrophy at step 80) until the NaN at ~118.
+    # per 10 steps (1.9e-8 in enstrophy at step 80, ~1e-3 at step 100) until
+    # the NaN near step 118, whose exact step then depends on rounding.
     smooth = slice(0, 61)
     err = np.max(np.abs(got[smooth] - ref[smooth]) / np.abs(ref[smooth]))
     assert err < 1e-10, err
-    assert abs(onset_got - onset_ref) <= 2, (onset_got, onset_ref)
+    assert abs(onset_got - onset_ref) <= 5, (onset_got, onset_ref)
 
 
 def test_diagnostics_match_oracle():
@@ -262,20 +263,57 @@ def test_nan_is_detected():
 
 
 # ------------------------------------------------------------------ FP32
-def test_fp32_shifted_tracks_fp64_tgv():
+def _shear_wave(N, U0=0.05, nu=0.02):
+    """Decaying shear wave u_x = U0 sin(2 pi j / N) in a periodic box (the
+    reference's VISCOSITY_TEST shape): its finite-difference correction
+    vanishes, so the flow is stable and decays as exp(-nu k^2 t)."""
+    pr = dict(R=1.0 / 3.0, gamma=1.4, Pr=0.71, nu=nu, T0=1.0, p0=1.0 / 3.0)
+    fl = cases.default_fields(N, N, N)
+    cases.periodic_ghosts(fl)
+    m = fl["type"] == cases.TYPE_F
+    j = np.arange(N + 2)[None, :, None] * np.ones((N + 2, N + 2, N + 2))
+    fl["u"][m] = (U0 * np.sin(2 * np.pi * (j - 1) / N))[m]
+    fl["p"][m] = pr["p0"]
+    fl["T"][m] = pr["T0"]
+    return fl, pr
+
+
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_shear_wave_decay_rate(precision):
+    N, steps = 32, 400
+    fl, pr = _shear_wave(N)
+    lat = Lattice(N, N, N, pr["nu"], gas_const=pr["R"], gamma=pr["gamma"], prtl=pr["Pr"], precision=precision)
+    lat.load_case(fl)
+    lat.Init()
+    lat.step(20)
+    a0 = np.abs(interior(lat.get_field("u"))).max()
+    lat.step(steps)
+    a1 = np.abs(interior(lat.get_field("u"))).max()
+    k = 2 * np.pi / N
+    rate = -np.log(a1 / a0) / steps
+    assert abs(rate / (pr["nu"] * k * k) - 1.0) < 0.02, rate / (pr["nu"] * k * k)
+
+
+def test_fp32_shifted_tracks_fp64_shear_wave():
     """FP32 deviation storage and FP32 collision against the FP64 plain path
-    on the Taylor-Green box in its smooth window."""
-    a, _, _ = gpu_tgv(32, precision=LBM_FP64)
-    b, _, _ = gpu_tgv(32, precision=LBM_FP32_SHIFTED)
-    for lat in (a, b):
+    on a stable flow (the Taylor-Green set-up of the reference is unstable and
+    amplifies any rounding difference, so it cannot measure precision)."""
+    N = 32
+    fl, pr = _shear_wave(N, U0=0.05, nu=0.01)
+    lats = []
+    for prec in (LBM_FP64, LBM_FP32_SHIFTED):
+        lat = Lattice(N, N, N, pr["nu"], gas_const=pr["R"], gamma=pr["gamma"], prtl=pr["Pr"], precision=prec)
+        lat.load_case(fl)
         lat.Init()
-        lat.step(40)
-    for k, tol in (("rho", 1e-5), ("u", 1e-4), ("v", 1e-4), ("T", 1e-5)):
+        lat.step(200)
+        lats.append(lat)
+    a, b = lats
+    for k, tol in (("rho", 1e-6), ("u", 1e-4), ("T", 1e-6)):
         e = relerr(interior(b.get_field(k)), interior(a.get_field(k)))
         assert e < tol, (k, e)
     da, db = a.diagnostics(), b.diagnostics()
     assert abs(db["kinetic"] - da["kinetic"]) / da["kinetic"] < 1e-4
-    assert abs(db["mass"] - da["mass"]) / da["mass"] < 1e-6
+    assert abs(db["mass"] - da["mass"]) / da["mass"] < 1e-7
 
 
 def test_fp32_populations_round_trip():
