@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 typedef struct lbm_ctx lbm_ctx;
 
@@ -92,7 +92,8 @@ typedef struct {
     double max_mach;     /* max |u| / sqrt(gamma p / rho) (gamma of the config, 1.4 if unset) */
     int64_t nan_count;   /* TYPE_F cells with non-finite rho or T */
     int64_t tau_warnings;/* collisions with 1/omega <= 0.5 so far (lbm-collision.cpp:33-34 prints instead) */
-    int64_t fallback_count; /* reserved for the chemistry fallback, 0 on this path */
+    int64_t fallback_count; /* cell updates whose chemistry the plain integrator redid
+                               (surrogate envelope fallbacks + failed audits) */
 } lbm_diag;
 
 /* LBM::LBM (lbm-main.cpp:16-35): allocates the device state. */
@@ -144,6 +145,74 @@ int lbm_profile_read(lbm_ctx* ctx, const char* name, double* total_ms, int64_t* 
 
 /* Bytes of device memory the context holds. */
 int64_t lbm_device_bytes(const lbm_ctx* ctx);
+
+/*
+ * Learned chemistry surrogate (multicomponent contexts with reaction on).
+ *
+ * The reference has no counterpart: it integrates the per-cell chemistry
+ * source of Collide_Species (lbm-collision.cpp:370-399) every step.  These
+ * calls let the per-cell MLP (T, p, Y_k, dt) -> (dT, dY_k) stand in for that
+ * integration: 12 inputs (zero-padded to 16) -> 128 -> 128 -> 128 -> 10
+ * outputs (padded to 16), GELU hidden layers, BF16 weights with FP32
+ * accumulation on the matrix cores, Adam on FP32 master weights.
+ *
+ * Workflow: lbm_surrogate_capture while stepping with the plain integrator
+ * samples (state, result) rows; lbm_surrogate_train fits the network on them
+ * on the GPU; lbm_surrogate_enable switches Collide_Species to the network.
+ * Cells whose features leave the captured min/max envelope, or whose network
+ * output is not finite, fall back to the plain integrator in the same step;
+ * one 32-cell chunk in `audit_stride` is recomputed by the plain integrator
+ * and the plain result replaces the network's where they differ by more than
+ * `audit_tol` target standard deviations.
+ *
+ * Parameter layout (lbm_surrogate_get/set_params, lbm_surrogate_gradient):
+ * for each layer L = 0..3, W_L [N_L][K_L] row-major then b_L [N_L], with
+ * (K, N) = (16, 128), (128, 128), (128, 128), (128, 16).
+ * Normalisation record (lbm_surrogate_get/set_norm), 96 floats: feature mean
+ * [16], feature std [16], target mean [16], target std [16], envelope min
+ * [16], envelope max [16].  Raw features: T [K], p [Pa], Y_k (9), dt [s];
+ * targets: dT [K] and dY_k (9) over one step.
+ */
+typedef struct {
+    int64_t captured;        /* rows in the capture buffer */
+    int64_t predicted;       /* cell updates whose chemistry came from the network */
+    int64_t fallback;        /* cell updates outside the envelope (plain integrator) */
+    int64_t audited;         /* network results recomputed by the plain integrator */
+    int64_t audit_failures;  /* audits above the tolerance (plain result kept) */
+    double max_audit_error;  /* largest audit error, in target standard deviations */
+} lbm_surrogate_stats;
+
+/* Number of FP32 parameters (weights and biases, padded layout). */
+int64_t lbm_surrogate_param_count(void);
+/* (Re)initialises the network (Glorot-uniform from `seed`, zero biases and
+ * Adam moments) and forgets the normalisation record. */
+int lbm_surrogate_init(lbm_ctx* ctx, uint64_t seed);
+/* Samples one cell in `stride` (hashed per step) of every plain-integrator
+ * chemistry call into a capture buffer of `max_rows` rows; max_rows = 0 stops
+ * and releases the buffer. Rows already captured are kept while it grows. */
+int lbm_surrogate_capture(lbm_ctx* ctx, int64_t max_rows, int stride);
+/* Copies up to max_rows captured rows [rows][32] (16 raw features | 16 raw
+ * targets) to the host; *rows_out receives the count. */
+int lbm_surrogate_get_capture(lbm_ctx* ctx, float* host, int64_t max_rows, int64_t* rows_out);
+/* `iters` Adam steps (learning rate `lr`) on mini-batches of `batch` rows
+ * (multiple of 16) drawn from the capture with `seed`; losses[iters] (may be
+ * null) receives each batch's mean squared error in normalised units.  The
+ * first call after lbm_surrogate_init derives the normalisation record from
+ * the capture. */
+int lbm_surrogate_train(lbm_ctx* ctx, int iters, int batch, float lr, uint64_t seed, float* losses);
+/* Switches Collide_Species between the network (on != 0) and the plain integrator. */
+int lbm_surrogate_enable(lbm_ctx* ctx, int on, int audit_stride, double audit_tol);
+int lbm_surrogate_read_stats(lbm_ctx* ctx, lbm_surrogate_stats* out);
+int lbm_surrogate_get_params(lbm_ctx* ctx, float* host);
+int lbm_surrogate_set_params(lbm_ctx* ctx, const float* host);
+int lbm_surrogate_get_norm(lbm_ctx* ctx, float* host96);
+int lbm_surrogate_set_norm(lbm_ctx* ctx, const float* host96);
+/* Network outputs Y [m][16] for normalised inputs X [m][16] with the in-step
+ * inference kernel. */
+int lbm_surrogate_forward(lbm_ctx* ctx, const float* X, int64_t m, float* Y);
+/* Mean squared error over the 10 outputs of m rows (m multiple of 16) and its
+ * gradient in the parameter layout, from the training kernels. */
+int lbm_surrogate_gradient(lbm_ctx* ctx, const float* X, const float* T, int64_t m, float* grad, float* loss);
 
 #ifdef __cplusplus
 }

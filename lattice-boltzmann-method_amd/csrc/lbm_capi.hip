@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -11,6 +12,7 @@
 #include "lbm_amd.h"
 #include "lbm_internal.h"
 #include "lbm_multi.h"
+#include "surrogate.h"
 
 using namespace lbm;
 
@@ -70,6 +72,31 @@ struct lbm_ctx {
     int* tms_dir = nullptr;
     int tms_count = 0;
     std::string tms_unsupported;
+    double* src = nullptr;  // [NS][n] chemistry mass sources
+
+    // chemistry surrogate
+    sur::Mlp net;
+    bool net_ready = false, norm_ready = false, sur_on = false;
+    int adam_step = 0;
+    sur::Work work;
+    float* sur_stats = nullptr;       // [SS_SIZE]
+    double* stats_part = nullptr;
+    unsigned long long* sur_count = nullptr;  // [SUR_NCOUNT]
+    unsigned* chem_list = nullptr;
+    unsigned* chem_count = nullptr;
+    double* tchem = nullptr;
+    float* cap_buf = nullptr;
+    unsigned long long* cap_count = nullptr;
+    long long cap_max = 0;
+    int cap_stride = 0;
+    bool capturing = false;
+    int audit_stride = 0;
+    float audit_tol = 0.f;
+    unsigned step = 0;
+    uint16_t* bx = nullptr;  // training batch, BF16 features
+    float* bt = nullptr;     // training batch, targets
+    float* bloss = nullptr;  // per-iteration losses
+    int bcap = 0, lcap = 0;
 
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -157,6 +184,19 @@ MultiView<double> make_mview(lbm_ctx* c, int src, int dst)
     v.u = c->units;
     v.reaction = c->cfg.reaction;
     v.tau_count = c->tau_count;
+    v.src = c->src;
+    v.tchem = c->tchem;
+    v.step = c->step;
+    if (c->capturing) {
+        v.cap_buf = c->cap_buf;
+        v.cap_count = c->cap_count;
+        v.cap_max = c->cap_max;
+        v.cap_stride = c->cap_stride;
+    }
+    v.sur_stats = c->sur_stats;
+    v.sur_count = c->sur_count;
+    v.audit_stride = c->audit_stride;
+    v.audit_tol = c->audit_tol;
     return v;
 }
 
@@ -244,11 +284,15 @@ void release(lbm_ctx* c)
         (void)hipEventDestroy(r.b);
     }
     c->pending.clear();
-    void* ptrs[] = {c->type, c->nbflag, c->pf[0], c->pf[1], c->pg[0], c->pg[1], c->fld_block, c->qd_block,
-                    c->tau_count, c->diag_part, c->diag_out, c->sf[0], c->sf[1], c->sp_block, c->tms_cell,
-                    c->tms_dir};
+    void* ptrs[] = {c->type,      c->nbflag,    c->pf[0],      c->pf[1],     c->pg[0],     c->pg[1],
+                    c->fld_block, c->qd_block,  c->tau_count,  c->diag_part, c->diag_out,  c->sf[0],
+                    c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
+                    c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
+                    c->cap_count, c->bx,        c->bt,         c->bloss};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->net_ready) sur::mlp_free(c->net);
+    if (c->work.cap) sur::work_free(c->work);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -365,7 +409,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         const size_t fb = sizeof(double) * (size_t)n * c->ns * 5;
         if ((e = hipMalloc((void**)&c->sp_block, fb)) != hipSuccess) return bail("hipMalloc species fields", e);
         (void)hipMemsetAsync(c->sp_block, 0, fb, c->stream);
-        c->bytes += 2 * sbytes + fb;
+        const size_t srcb = sizeof(double) * (size_t)n * c->ns;
+        if ((e = hipMalloc((void**)&c->src, srcb)) != hipSuccess) return bail("hipMalloc chemistry sources", e);
+        (void)hipMemsetAsync(c->src, 0, srcb, c->stream);
+        c->bytes += 2 * sbytes + fb + srcb;
     }
     c->diag_npart = diag_partials(c->d);
     if ((e = hipMalloc(&c->diag_part, sizeof(double) * 8 * c->diag_npart)) != hipSuccess) return bail("hipMalloc", e);
@@ -678,7 +725,24 @@ int lbm_collide_species(lbm_ctx* c)
     if (!c) return LBM_E_ARG;
     if (!multi(c)) return LBM_OK;  // Collide_Species exists only with species (lbm-main.cpp:156-158)
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
-    return run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.reaction) {
+        int rc;
+        if (c->sur_on) {
+            const MultiView<double> v = make_mview(c, 0, 1);
+            rc = launch(c, "mc_chemistry_surrogate",
+                        [&] { launch_mc_sur_chemistry<double>(v, c->net, c->chem_list, c->chem_count, c->stream); });
+            if (!rc)
+                rc = launch(c, "mc_chemistry_fallback",
+                            [&] { launch_mc_chem_list<double>(v, c->chem_list, c->chem_count, c->stream); });
+        } else {
+            rc = run_mc(c, McKernel::Chemistry, "mc_chemistry_plain", 0, 1);
+        }
+        if (rc) return rc;
+    }
+    const int rc = run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+    ++c->step;
+    return rc;
 }
 
 int lbm_collide(lbm_ctx* c)
@@ -784,6 +848,11 @@ int lbm_diagnostics(lbm_ctx* c, lbm_diag* out)
     out->max_mach = h[5];
     out->tau_warnings = (int64_t)tau;
     out->fallback_count = 0;
+    if (c->sur_count) {
+        unsigned long long cnt[SUR_NCOUNT];
+        HIP_TRY(c, hipMemcpy(cnt, c->sur_count, sizeof(cnt), hipMemcpyDeviceToHost));
+        out->fallback_count = (int64_t)(cnt[SUR_FALLBACK] + cnt[SUR_AUDIT_FAIL]);
+    }
     return LBM_OK;
 }
 
@@ -828,5 +897,341 @@ int lbm_profile_read(lbm_ctx* c, const char* name, double* total_ms, int64_t* la
 }
 
 int64_t lbm_device_bytes(const lbm_ctx* c) { return c ? c->bytes : 0; }
+
+}  // extern "C"
+
+// ------------------------------------------------------ chemistry surrogate
+namespace {
+
+constexpr uint64_t SUR_DEFAULT_SEED = 20261202ull;
+
+int sur_common(lbm_ctx* c)
+{
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (!c->sur_stats) {
+        HIP_TRY(c, hipMalloc((void**)&c->sur_stats, sizeof(float) * sur::SS_SIZE));
+        HIP_TRY(c, hipMemsetAsync(c->sur_stats, 0, sizeof(float) * sur::SS_SIZE, c->stream));
+        HIP_TRY(c, hipMalloc((void**)&c->stats_part, sizeof(double) * sur::STATS_BLOCKS * 32 * 4));
+        HIP_TRY(c, hipMalloc((void**)&c->sur_count, sizeof(unsigned long long) * SUR_NCOUNT));
+        HIP_TRY(c, hipMemsetAsync(c->sur_count, 0, sizeof(unsigned long long) * SUR_NCOUNT, c->stream));
+    }
+    if (!c->net_ready) {
+        HIP_TRY(c, sur::mlp_alloc(c->net, SUR_DEFAULT_SEED, c->stream));
+        c->net_ready = true;
+        c->adam_step = 0;
+    }
+    return LBM_OK;
+}
+
+// Training scratch for batches of up to `rows` rows.
+int sur_work(lbm_ctx* c, int rows)
+{
+    if (c->work.cap >= rows) return LBM_OK;
+    if (c->work.cap) sur::work_free(c->work);
+    HIP_TRY(c, sur::work_alloc(c->work, rows));
+    return LBM_OK;
+}
+
+int sur_batch(lbm_ctx* c, int rows)
+{
+    if (c->bcap >= rows) return LBM_OK;
+    if (c->bx) (void)hipFree(c->bx);
+    if (c->bt) (void)hipFree(c->bt);
+    c->bx = nullptr;
+    c->bt = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&c->bx, sizeof(uint16_t) * 16 * (size_t)rows));
+    HIP_TRY(c, hipMalloc((void**)&c->bt, sizeof(float) * 16 * (size_t)rows));
+    c->bcap = rows;
+    return LBM_OK;
+}
+
+int copy_params(lbm_ctx* c, float* host, bool to_host)
+{
+    size_t off = 0;
+    for (int L = 0; L < sur::NL; ++L) {
+        const size_t nw = (size_t)sur::DIMS[L] * sur::DIMS[L + 1], nb = sur::DIMS[L + 1];
+        if (to_host) {
+            HIP_TRY(c, hipMemcpyAsync(host + off, c->net.W[L], 4 * nw, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(host + off + nw, c->net.b[L], 4 * nb, hipMemcpyDeviceToHost, c->stream));
+        } else {
+            HIP_TRY(c, hipMemcpyAsync(c->net.W[L], host + off, 4 * nw, hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(c->net.b[L], host + off + nw, 4 * nb, hipMemcpyHostToDevice, c->stream));
+        }
+        off += nw + nb;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int need_reacting(lbm_ctx* c)
+{
+    if (!multi(c)) return fail(c, LBM_E_STATE, "the chemistry surrogate needs a multicomponent context");
+    if (!c->cfg.reaction) return fail(c, LBM_E_STATE, "the chemistry surrogate needs reaction = 1");
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t lbm_surrogate_param_count(void) { return sur::NPARAM; }
+
+int lbm_surrogate_init(lbm_ctx* c, uint64_t seed)
+{
+    if (!c) return LBM_E_ARG;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->net_ready) sur::mlp_free(c->net);
+    c->net_ready = false;
+    HIP_TRY(c, sur::mlp_alloc(c->net, seed, c->stream));
+    c->net_ready = true;
+    c->adam_step = 0;
+    c->norm_ready = false;
+    c->sur_on = false;
+    return sur_common(c);
+}
+
+int lbm_surrogate_capture(lbm_ctx* c, int64_t max_rows, int stride)
+{
+    if (!c) return LBM_E_ARG;
+    if (int rc = need_reacting(c)) return rc;
+    if (max_rows < 0 || max_rows > ((int64_t)1 << 31) || (max_rows > 0 && stride < 1))
+        return fail(c, LBM_E_ARG, "capture needs 0 <= max_rows <= 2^31 and stride >= 1");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (max_rows == 0) {
+        if (c->cap_buf) (void)hipFree(c->cap_buf);
+        if (c->cap_count) (void)hipFree(c->cap_count);
+        c->cap_buf = nullptr;
+        c->cap_count = nullptr;
+        c->cap_max = 0;
+        c->capturing = false;
+        return LBM_OK;
+    }
+    if (!c->cap_count) {
+        HIP_TRY(c, hipMalloc((void**)&c->cap_count, sizeof(unsigned long long)));
+        HIP_TRY(c, hipMemset(c->cap_count, 0, sizeof(unsigned long long)));
+    }
+    if (max_rows > c->cap_max) {
+        float* nb = nullptr;
+        HIP_TRY(c, hipMalloc((void**)&nb, sizeof(float) * sur::CAP_STRIDE * (size_t)max_rows));
+        if (c->cap_buf) {
+            unsigned long long have = 0;
+            HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
+            const size_t keep = (size_t)std::min<long long>((long long)have, c->cap_max);
+            HIP_TRY(c, hipMemcpy(nb, c->cap_buf, sizeof(float) * sur::CAP_STRIDE * keep, hipMemcpyDeviceToDevice));
+            (void)hipFree(c->cap_buf);
+        }
+        c->cap_buf = nb;
+        c->cap_max = max_rows;
+    }
+    c->cap_stride = stride;
+    c->capturing = true;
+    return LBM_OK;
+}
+
+int lbm_surrogate_get_capture(lbm_ctx* c, float* host, int64_t max_rows, int64_t* rows_out)
+{
+    if (!c || !rows_out || (max_rows > 0 && !host)) return fail(c, LBM_E_ARG, "null argument");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    unsigned long long have = 0;
+    if (c->cap_count) HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
+    const int64_t rows = std::min<int64_t>((int64_t)have, c->cap_max);
+    const int64_t m = std::min<int64_t>(rows, max_rows);
+    if (m > 0) HIP_TRY(c, hipMemcpy(host, c->cap_buf, sizeof(float) * sur::CAP_STRIDE * m, hipMemcpyDeviceToHost));
+    *rows_out = rows;
+    return LBM_OK;
+}
+
+int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t seed, float* losses)
+{
+    if (!c) return LBM_E_ARG;
+    if (iters < 0 || batch < 16 || batch % 16 || batch > (1 << 20) || !(lr > 0.f))
+        return fail(c, LBM_E_ARG, "train needs iters >= 0, batch a multiple of 16 in [16, 2^20], lr > 0");
+    if (int rc = sur_common(c)) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    unsigned long long have = 0;
+    if (c->cap_count) HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
+    const int64_t rows = std::min<int64_t>((int64_t)have, c->cap_max);
+    if (rows < 1) return fail(c, LBM_E_STATE, "no captured rows to train on (lbm_surrogate_capture, then step)");
+    if (!c->norm_ready) {
+        sur::capture_stats(c->cap_buf, rows, c->stats_part, c->sur_stats, c->stream);
+        c->norm_ready = true;
+    }
+    if (int rc = sur_work(c, batch)) return rc;
+    if (int rc = sur_batch(c, batch)) return rc;
+    if (iters > c->lcap) {
+        if (c->bloss) (void)hipFree(c->bloss);
+        c->bloss = nullptr;
+        HIP_TRY(c, hipMalloc((void**)&c->bloss, sizeof(float) * iters));
+        c->lcap = iters;
+    }
+    for (int it = 0; it < iters; ++it) {
+        sur::gather_batch(c->cap_buf, rows, c->sur_stats, seed, it, batch, c->bx, c->bt, c->stream);
+        sur::mlp_grad(c->net, c->bx, c->bt, batch, c->work, c->stream);
+        sur::loss_sum(c->work.loss_rows, batch, 1.f / ((float)batch * sur::NOUT), c->bloss + it, c->stream);
+        sur::mlp_adam(c->net, c->work, lr, ++c->adam_step, c->stream);
+    }
+    HIP_TRY(c, hipGetLastError());
+    if (losses && iters > 0)
+        HIP_TRY(c, hipMemcpyAsync(losses, c->bloss, sizeof(float) * iters, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_surrogate_enable(lbm_ctx* c, int on, int audit_stride, double audit_tol)
+{
+    if (!c) return LBM_E_ARG;
+    if (!on) {
+        c->sur_on = false;
+        return LBM_OK;
+    }
+    if (int rc = need_reacting(c)) return rc;
+    if (audit_stride < 0 || !(audit_tol >= 0.0)) return fail(c, LBM_E_ARG, "audit_stride >= 0 and audit_tol >= 0");
+    if (int rc = sur_common(c)) return rc;
+    if (!c->norm_ready) return fail(c, LBM_E_STATE, "train the surrogate (or set its normalisation) first");
+    const int64_t n = c->d.n;
+    if (!c->chem_list) {
+        HIP_TRY(c, hipMalloc((void**)&c->chem_list, sizeof(unsigned) * n));
+        HIP_TRY(c, hipMalloc((void**)&c->chem_count, sizeof(unsigned)));
+        HIP_TRY(c, hipMalloc((void**)&c->tchem, sizeof(double) * n));
+        c->bytes += (sizeof(unsigned) + sizeof(double)) * n;
+    }
+    c->audit_stride = audit_stride;
+    c->audit_tol = (float)audit_tol;
+    c->sur_on = true;
+    return LBM_OK;
+}
+
+int lbm_surrogate_read_stats(lbm_ctx* c, lbm_surrogate_stats* out)
+{
+    if (!c || !out) return fail(c, LBM_E_ARG, "null argument");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *out = lbm_surrogate_stats{};
+    unsigned long long have = 0;
+    if (c->cap_count) HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
+    out->captured = std::min<int64_t>((int64_t)have, c->cap_max);
+    if (c->sur_count) {
+        unsigned long long cnt[SUR_NCOUNT];
+        HIP_TRY(c, hipMemcpy(cnt, c->sur_count, sizeof(cnt), hipMemcpyDeviceToHost));
+        out->predicted = (int64_t)cnt[SUR_PREDICTED];
+        out->fallback = (int64_t)cnt[SUR_FALLBACK];
+        out->audited = (int64_t)cnt[SUR_AUDITED];
+        out->audit_failures = (int64_t)cnt[SUR_AUDIT_FAIL];
+        const unsigned bits = (unsigned)cnt[SUR_MAX_ERR];
+        float e;
+        std::memcpy(&e, &bits, 4);
+        out->max_audit_error = e;
+    }
+    return LBM_OK;
+}
+
+int lbm_surrogate_get_params(lbm_ctx* c, float* host)
+{
+    if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = sur_common(c)) return rc;
+    return copy_params(c, host, true);
+}
+
+int lbm_surrogate_set_params(lbm_ctx* c, const float* host)
+{
+    if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = sur_common(c)) return rc;
+    if (int rc = copy_params(c, const_cast<float*>(host), false)) return rc;
+    for (int L = 0; L < sur::NL; ++L) {
+        const size_t nw = (size_t)sur::DIMS[L] * sur::DIMS[L + 1], nb = sur::DIMS[L + 1];
+        HIP_TRY(c, hipMemsetAsync(c->net.mW[L], 0, 4 * nw, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->net.vW[L], 0, 4 * nw, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->net.mb[L], 0, 4 * nb, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->net.vb[L], 0, 4 * nb, c->stream));
+    }
+    c->adam_step = 0;
+    sur::refresh_bf16(c->net, c->stream);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_surrogate_get_norm(lbm_ctx* c, float* host96)
+{
+    if (!c || !host96) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = sur_common(c)) return rc;
+    if (!c->norm_ready) return fail(c, LBM_E_STATE, "no normalisation record yet");
+    HIP_TRY(c, hipMemcpyAsync(host96, c->sur_stats, sizeof(float) * sur::SS_SIZE, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_surrogate_set_norm(lbm_ctx* c, const float* host96)
+{
+    if (!c || !host96) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = sur_common(c)) return rc;
+    for (int q = 0; q < 16; ++q)
+        if (!(host96[sur::SS_FSTD + q] > 0.f) || !(host96[sur::SS_TSTD + q] > 0.f))
+            return fail(c, LBM_E_ARG, "standard deviations must be > 0");
+    HIP_TRY(c, hipMemcpyAsync(c->sur_stats, host96, sizeof(float) * sur::SS_SIZE, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->norm_ready = true;
+    return LBM_OK;
+}
+
+int lbm_surrogate_forward(lbm_ctx* c, const float* X, int64_t m, float* Y)
+{
+    if (!c || !X || !Y) return fail(c, LBM_E_ARG, "null argument");
+    if (m < 1 || m > (1 << 24)) return fail(c, LBM_E_ARG, "m out of range [1, 2^24]");
+    if (int rc = sur_common(c)) return rc;
+    float *dx = nullptr, *dy = nullptr;
+    const size_t bytes = sizeof(float) * 16 * (size_t)m;
+    HIP_TRY(c, hipMalloc((void**)&dx, bytes));
+    hipError_t e = hipMalloc((void**)&dy, bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, X, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        sur::mlp_infer(c->net, dx, (int)m, dy, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(Y, dy, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return fail(c, LBM_E_HIP, std::string("surrogate forward: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_surrogate_gradient(lbm_ctx* c, const float* X, const float* T, int64_t m, float* grad, float* loss)
+{
+    if (!c || !X || !T || !grad || !loss) return fail(c, LBM_E_ARG, "null argument");
+    if (m < 16 || m % 16 || m > (1 << 20)) return fail(c, LBM_E_ARG, "m must be a multiple of 16 in [16, 2^20]");
+    if (int rc = sur_common(c)) return rc;
+    if (int rc = sur_work(c, (int)m)) return rc;
+    if (int rc = sur_batch(c, (int)m)) return rc;
+    if (c->lcap < 1) {
+        HIP_TRY(c, hipMalloc((void**)&c->bloss, sizeof(float)));
+        c->lcap = 1;
+    }
+    float* xf = nullptr;
+    const size_t bytes = sizeof(float) * 16 * (size_t)m;
+    HIP_TRY(c, hipMalloc((void**)&xf, bytes));
+    hipError_t e = hipMemcpyAsync(xf, X, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->bt, T, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        sur::to_bf16(xf, c->bx, 16 * m, c->stream);
+        sur::mlp_grad(c->net, c->bx, c->bt, (int)m, c->work, c->stream);
+        sur::loss_sum(c->work.loss_rows, (int)m, 1.f / ((float)m * sur::NOUT), c->bloss, c->stream);
+        e = hipGetLastError();
+    }
+    size_t off = 0;
+    for (int L = 0; L < sur::NL && e == hipSuccess; ++L) {
+        const size_t nw = (size_t)sur::DIMS[L] * sur::DIMS[L + 1], nb = sur::DIMS[L + 1];
+        e = hipMemcpyAsync(grad + off, c->work.gW[L], 4 * nw, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(grad + off + nw, c->work.gb[L], 4 * nb, hipMemcpyDeviceToHost, c->stream);
+        off += nw + nb;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, c->bloss, sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(xf);
+    if (e != hipSuccess) return fail(c, LBM_E_HIP, std::string("surrogate gradient: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 #pragma once
 #include "chem_device.h"
 #include "lbm_internal.h"
+#include "surrogate.h"
 
 namespace lbm {
 
@@ -21,24 +22,27 @@ struct MultiView {
     chem::Units u;
     int reaction;
     unsigned long long* tau_count;
+    double* src;    // [NS][n] species mass added by the chemistry over the step (lattice units)
+    double* tchem;  // [n] temperature after the chemistry [K] (surrogate audit only)
     // chemistry surrogate (all null when unused)
-    unsigned step;                 // step counter for sampling hashes
-    float* cap_buf;                // capture rows [cap_max][32]: features 16 | targets 16
+    unsigned step;                 // step counter for the sampling hashes
+    float* cap_buf;                // capture rows [cap_max][sur::CAP_STRIDE]
     unsigned long long* cap_count;
     long long cap_max;
     int cap_stride;
-    const float* sur_pred;         // [n][16] normalised network outputs
-    const uint8_t* sur_flag;       // [n] 1: outside the training envelope
-    const float* sur_stats;        // feature mean/std [16]x2, target mean/std [16]x2, envelope lo/hi [16]x2
-    unsigned long long* sur_count; // fallback, audited, audit failures, max audit error (float bits)
-    int audit_stride;
-    float audit_tol;
+    const float* sur_stats;        // normalisation and envelope record [sur::SS_SIZE]
+    unsigned long long* sur_count; // SUR_* counters
+    int audit_stride;              // one 32-cell chunk in audit_stride is audited
+    float audit_tol;               // audit tolerance in target standard deviations
 };
 
-// layout of sur_stats
-enum { SS_FMEAN = 0, SS_FSTD = 16, SS_TMEAN = 32, SS_TSTD = 48, SS_LO = 64, SS_HI = 80, SS_SIZE = 96 };
+// counters of the surrogate path (cumulative)
+enum { SUR_PREDICTED = 0, SUR_FALLBACK = 1, SUR_AUDITED = 2, SUR_AUDIT_FAIL = 3, SUR_MAX_ERR = 4, SUR_NCOUNT = 8 };
 
-enum class McKernel { Init, CollideSpecies, Collide, Stream, Moments };
+// bit 31 of a chemistry-list entry marks an audit (else a fallback)
+constexpr unsigned CHEM_AUDIT = 0x80000000u;
+
+enum class McKernel { Init, Chemistry, CollideSpecies, Collide, Stream, Moments };
 
 template <typename P>
 void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);
@@ -51,8 +55,16 @@ constexpr double NSCBC_SIGMA_OUT = 0.25;
 template <typename P>
 void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
 
-// Normalised surrogate features [n][16] and envelope flags of every fluid cell.
+// Surrogate chemistry of every fluid cell: features, network (BF16 MFMA),
+// decoded sources into v.src / v.hrr / v.tchem; cells outside the training
+// envelope and the audited chunks are appended to `list` (count in *count,
+// zeroed first on the stream).
 template <typename P>
-void launch_mc_sur_features(const MultiView<P>& v, float* X, uint8_t* flag, hipStream_t s);
+void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, unsigned* list, unsigned* count,
+                             hipStream_t s);
+// Plain chemistry of the listed cells: fallbacks are overwritten, audits are
+// compared with the network's result and overwritten above the tolerance.
+template <typename P>
+void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsigned* count, hipStream_t s);
 
 }  // namespace lbm

@@ -15,6 +15,7 @@
 //   calculate_moment lbm-moment.cpp:193-377
 #include "chem_device.h"
 #include "lbm_multi.h"
+#include "surrogate_device.h"
 
 namespace lbm {
 
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_init(MultiView<P> v)
     }
 }
 
-// ------------------------------------------------------- Collide_Species
+// ------------------------------------------------------------ chemistry
 __device__ inline unsigned mix_hash(unsigned c, unsigned step)
 {
     unsigned h = c * 0x9E3779B1u ^ (step * 0x85EBCA77u + 0x165667B1u);
@@ -115,34 +116,244 @@ __device__ inline unsigned mix_hash(unsigned c, unsigned step)
     return h;
 }
 
+// Thermochemical state of a cell as Collide_Species sees it
+// (lbm-collision.cpp:350-369): species densities, normalised mass fractions,
+// SI temperature and density, and the SI internal energy without the
+// species diffusion kinetic energy.
+struct ChemState {
+    double rho, rho_si, Tsi, usi;
+    double ra[NS], Y0[NS];
+};
+
+template <typename P>
+__device__ inline void chem_state(const MultiView<P>& v, int c, bool with_energy, ChemState& s)
+{
+    const int64_t n = v.d.n;
+    s.rho = v.fld[LBM_FIELD_RHO][c];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        s.ra[a] = v.rhoa[a * n + c];
+        s.Y0[a] = s.ra[a] / s.rho;
+    }
+    normalize(s.Y0);
+    s.Tsi = v.u.si_temp(v.fld[LBM_FIELD_T][c]);
+    s.rho_si = v.u.si_rho(s.rho);
+    s.usi = 0.0;
+    if (with_energy) {
+        double Uint = v.fld[LBM_FIELD_RHOE][c] / s.rho;
+        for (int a = 0; a < NS; ++a) {
+            const double ux = v.ua[(0 * NS + a) * n + c], uy = v.ua[(1 * NS + a) * n + c],
+                         uz = v.ua[(2 * NS + a) * n + c];
+            Uint = Uint - 0.5 * s.ra[a] / s.rho * (ux * ux + uy * uy + uz * uz);
+        }
+        s.usi = v.u.si_energy_mass(Uint);
+    }
+}
+
 // The reference's chemistry source (lbm-collision.cpp:370-393): two explicit
 // Euler sub-steps of dt/2 of the net production rates, each followed by a
 // UV temperature re-solve at the new composition.  src[a] is the species
 // mass added over the step, *Tout the final temperature [K].
 template <typename P>
-__device__ inline void plain_chemistry(const MultiView<P>& v, double rho, double rho_si, const double ra[NS],
-                                       const double Y0[NS], double Tsi, double usi, double src[NS], double* Tout)
+__device__ inline void plain_chemistry(const MultiView<P>& v, const ChemState& s, double src[NS], double* Tout)
 {
-    double Tcur = Tsi, Ycur[NS];
+    double Tcur = s.Tsi, Ycur[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
-        Ycur[a] = Y0[a];
+        Ycur[a] = s.Y0[a];
         src[a] = 0.0;
     }
     for (int sub = 0; sub < 2; ++sub) {
         double wdot[NS];
-        net_production(Tcur, rho_si, Ycur, wdot);
+        net_production(Tcur, s.rho_si, Ycur, wdot);
 #pragma unroll
         for (int a = 0; a < NS; ++a) {
             src[a] += 1.0 / 2.0 * (double)v.u.rho_dot(wdot[a] * W[a]);
-            Ycur[a] = (ra[a] + src[a]) / rho;
+            Ycur[a] = (s.ra[a] + src[a]) / s.rho;
         }
         normalize(Ycur);
-        Tcur = T_from_UV(usi, Ycur, Tcur);
+        Tcur = T_from_UV(s.usi, Ycur, Tcur);
     }
     *Tout = Tcur;
 }
 
+// HRR = -sum_a h_a(T) rho_dot_a (lbm-collision.cpp:395-399)
+template <typename P>
+__device__ inline double heat_release(const MultiView<P>& v, const double src[NS], double T)
+{
+    double hrr = 0;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) hrr += -1.0 * (double)v.u.energy_mass(h_RT(a, T) * R_UNIV * T / W[a]) * src[a];
+    return hrr;
+}
+
+template <typename P>
+__device__ inline void store_chem(const MultiView<P>& v, int c, const double src[NS], double T)
+{
+    const int64_t n = v.d.n;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) v.src[a * n + c] = src[a];
+    v.hrr[c] = heat_release(v, src, T);
+    if (v.tchem) v.tchem[c] = T;
+}
+
+// Plain chemistry of every fluid cell; optionally samples (state, result)
+// rows for the surrogate's training set.
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_chemistry(MultiView<P> v)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    ChemState s;
+    chem_state(v, c, true, s);
+    double src[NS], Tcur;
+    plain_chemistry(v, s, src, &Tcur);
+    store_chem(v, c, src, Tcur);
+    if (v.cap_buf && mix_hash(c, v.step) % (unsigned)v.cap_stride == 0) {
+        const unsigned long long slot = atomicAdd(v.cap_count, 1ull);
+        if ((long long)slot < v.cap_max) {
+            float* row = v.cap_buf + slot * sur::CAP_STRIDE;
+            row[0] = (float)s.Tsi;
+            row[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) row[2 + a] = (float)s.Y0[a];
+            row[11] = v.u.s;
+#pragma unroll
+            for (int q = 12; q < 16; ++q) row[q] = 0.f;
+            row[16] = (float)(Tcur - s.Tsi);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / s.rho);
+#pragma unroll
+            for (int q = 26; q < 32; ++q) row[q] = 0.f;
+        }
+    }
+}
+
+// Surrogate chemistry: 512 threads = 8 wavefronts x 32 cells per pass, the
+// network's weights staged in LDS once per block.
+constexpr int SUR_WAVES = 8;
+
+template <typename P>
+__global__ __launch_bounds__(512) void k_mc_sur_chemistry(MultiView<P> v, sur::Mlp net, unsigned* list,
+                                                          unsigned* count)
+{
+    using namespace sur;
+    __shared__ __align__(16) uint16_t Wf[FRAG_TOTAL];
+    __shared__ float bias[BIAS_TOTAL];
+    __shared__ __align__(16) uint16_t xin[SUR_WAVES][32 * 16];
+    __shared__ float yout[SUR_WAVES][32 * 16];
+    stage_net(net, Wf, bias);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n = v.d.n;
+    const float* st = v.sur_stats;
+    for (int64_t base = (int64_t)blockIdx.x * SUR_WAVES * 32; base < n; base += (int64_t)gridDim.x * SUR_WAVES * 32) {
+        const int64_t chunk0 = base + wave * 32;
+        const int c = (int)(chunk0 + (lane & 31));
+        const bool mine = lane < 32 && c < n && v.type[c] == TYPE_F;
+        bool outside = false;
+        __syncthreads();
+        if (lane < 32) {
+            float raw[NIN] = {};
+            if (mine) {
+                ChemState s;
+                chem_state(v, c, false, s);
+                raw[0] = (float)s.Tsi;
+                raw[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
+#pragma unroll
+                for (int a = 0; a < NS; ++a) raw[2 + a] = (float)s.Y0[a];
+                raw[11] = v.u.s;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                float x = 0.f;
+                if (q < NIN && mine) {
+                    outside |= raw[q] < st[SS_LO + q] || raw[q] > st[SS_HI + q];
+                    x = (raw[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q];
+                }
+                xin[wave][lane * 16 + q] = f2bf(x);
+            }
+        }
+        __syncthreads();
+        mlp_rows32(Wf, bias, xin[wave], yout[wave]);
+        __syncthreads();
+        bool audit = false, fallback = false;
+        if (mine) {
+            ChemState s;
+            chem_state(v, c, false, s);
+            const float* pr = yout[wave] + lane * 16;
+            double src[NS], sum = 0;
+            bool finite = true;
+#pragma unroll
+            for (int a = 0; a < NS; ++a) {
+                src[a] = s.rho * ((double)pr[1 + a] * st[SS_TSTD + 1 + a] + st[SS_TMEAN + 1 + a]);
+                sum += src[a];
+            }
+#pragma unroll
+            for (int a = 0; a < NS; ++a) src[a] -= s.Y0[a] * sum;  // the sources carry no net mass
+            const double Tcur = s.Tsi + ((double)pr[0] * st[SS_TSTD] + st[SS_TMEAN]);
+            finite = isfinite(sum) && isfinite(Tcur) && Tcur > 0.0;
+            fallback = outside || !finite;
+            if (!finite) {
+#pragma unroll
+                for (int a = 0; a < NS; ++a) src[a] = 0.0;
+            }
+            store_chem(v, c, src, finite ? Tcur : s.Tsi);
+            audit = !fallback && v.audit_stride > 0 &&
+                    mix_hash((unsigned)(chunk0 >> 5), v.step) % (unsigned)v.audit_stride == 0;
+        }
+        // wavefront-aggregated append of the cells the plain integrator must see
+        const unsigned long long need = __ballot(fallback || audit);
+        const unsigned long long used = __ballot(mine && !fallback);
+        const unsigned long long fb = __ballot(fallback);
+        unsigned slot0 = 0;
+        if (lane == 0) {
+            if (need) slot0 = atomicAdd(count, (unsigned)__popcll(need));
+            if (used) atomicAdd(&v.sur_count[SUR_PREDICTED], (unsigned long long)__popcll(used));
+            if (fb) atomicAdd(&v.sur_count[SUR_FALLBACK], (unsigned long long)__popcll(fb));
+        }
+        slot0 = __shfl(slot0, 0);
+        if (fallback || audit) {
+            const unsigned below = (unsigned)__popcll(need & ((1ull << lane) - 1ull));
+            list[slot0 + below] = (unsigned)c | (audit ? CHEM_AUDIT : 0u);
+        }
+    }
+}
+
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const unsigned* list, const unsigned* count)
+{
+    using namespace sur;
+    const unsigned total = *count;
+    const int64_t n = v.d.n;
+    const float* st = v.sur_stats;
+    for (unsigned e = blockIdx.x * BLOCK + threadIdx.x; e < total; e += gridDim.x * BLOCK) {
+        const unsigned ent = list[e];
+        const int c = (int)(ent & ~CHEM_AUDIT);
+        ChemState s;
+        chem_state(v, c, true, s);
+        double src[NS], Tp;
+        plain_chemistry(v, s, src, &Tp);
+        if (!(ent & CHEM_AUDIT)) {
+            store_chem(v, c, src, Tp);
+            continue;
+        }
+        float err = (float)(fabs(Tp - v.tchem[c]) / st[SS_TSTD]);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            const float ea = (float)(fabs(src[a] - v.src[a * n + c]) / (s.rho * st[SS_TSTD + 1 + a]));
+            err = ea > err ? ea : err;
+        }
+        if (!(err == err)) err = INFINITY;
+        atomicAdd(&v.sur_count[SUR_AUDITED], 1ull);
+        atomicMax(&v.sur_count[SUR_MAX_ERR], (unsigned long long)__float_as_uint(err));
+        if (err > v.audit_tol) {  // the audit overrules the network for this cell
+            atomicAdd(&v.sur_count[SUR_AUDIT_FAIL], 1ull);
+            store_chem(v, c, src, Tp);
+        }
+    }
+}
+
+// ------------------------------------------------------- Collide_Species
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
 {
@@ -159,81 +370,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
         ra[a] = v.rhoa[a * n + c];
         Xa[a] = v.X[a * n + c];
         Y0[a] = ra[a] / rho;
-        src[a] = 0.0;
+        src[a] = v.reaction ? v.src[a * n + c] : 0.0;
     }
     normalize(Y0);
     const double Tsi = v.u.si_temp(T);
     const double rho_si = v.u.si_rho(rho);
-    if (v.reaction) {
-        double Uint = Fd[LBM_FIELD_RHOE][c] / rho;
-        for (int a = 0; a < NS; ++a) {
-            const double ux = v.ua[(0 * NS + a) * n + c], uy = v.ua[(1 * NS + a) * n + c],
-                         uz = v.ua[(2 * NS + a) * n + c];
-            Uint = Uint - 0.5 * ra[a] / rho * (ux * ux + uy * uy + uz * uz);
-        }
-        const double usi = v.u.si_energy_mass(Uint);
-        double Tcur = Tsi;
-        bool plain = true;
-        if (v.sur_pred) {
-            // learned source: d_a = src_a / rho and dT predicted from (T, p, Y, dt)
-            const float* pr = v.sur_pred + (int64_t)c * 16;
-            const float* st = v.sur_stats;
-            double sum = 0;
-#pragma unroll
-            for (int a = 0; a < NS; ++a) {
-                src[a] = rho * ((double)pr[1 + a] * st[SS_TSTD + 1 + a] + st[SS_TMEAN + 1 + a]);
-                sum += src[a];
-            }
-#pragma unroll
-            for (int a = 0; a < NS; ++a) src[a] -= Y0[a] * sum;  // keep sum_a src_a = 0
-            Tcur = Tsi + ((double)pr[0] * st[SS_TSTD] + st[SS_TMEAN]);
-            plain = v.sur_flag[c] != 0;
-            const bool audit = v.audit_stride > 0 && mix_hash(c, v.step) % (unsigned)v.audit_stride == 0;
-            if (plain) atomicAdd(&v.sur_count[0], 1ull);
-            if (audit && !plain) {
-                double sp[NS], Tp;
-                plain_chemistry(v, rho, rho_si, ra, Y0, Tsi, usi, sp, &Tp);
-                float err = (float)(fabs(Tp - Tcur) / st[SS_TSTD]);
-#pragma unroll
-                for (int a = 0; a < NS; ++a) {
-                    const float e = (float)(fabs(sp[a] - src[a]) / (rho * st[SS_TSTD + 1 + a]));
-                    err = e > err ? e : err;
-                }
-                atomicAdd(&v.sur_count[1], 1ull);
-                atomicMax(&v.sur_count[3], (unsigned long long)__float_as_uint(err));
-                if (err > v.audit_tol) {  // the audit overrules the network for this cell
-                    atomicAdd(&v.sur_count[2], 1ull);
-#pragma unroll
-                    for (int a = 0; a < NS; ++a) src[a] = sp[a];
-                    Tcur = Tp;
-                }
-            }
-        }
-        if (plain) plain_chemistry(v, rho, rho_si, ra, Y0, Tsi, usi, src, &Tcur);
-        if (v.cap_buf && !v.sur_pred && mix_hash(c, v.step) % (unsigned)v.cap_stride == 0) {
-            const unsigned long long slot = atomicAdd(v.cap_count, 1ull);
-            if ((long long)slot < v.cap_max) {
-                float* row = v.cap_buf + slot * 32;
-                row[0] = (float)Tsi;
-                row[1] = (float)pressure(Tsi, rho_si, Y0);
-#pragma unroll
-                for (int a = 0; a < NS; ++a) row[2 + a] = (float)Y0[a];
-                row[11] = v.u.s;
-#pragma unroll
-                for (int q = 12; q < 16; ++q) row[q] = 0.f;
-                row[16] = (float)(Tcur - Tsi);
-#pragma unroll
-                for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / rho);
-#pragma unroll
-                for (int q = 26; q < 32; ++q) row[q] = 0.f;
-            }
-        }
-        double hrr = 0;
-#pragma unroll
-        for (int a = 0; a < NS; ++a)
-            hrr += -1.0 * (double)v.u.energy_mass(h_RT(a, Tcur) * R_UNIV * Tcur / W[a]) * src[a];
-        v.hrr[c] = hrr;
-    }
     // species viscosities and Wilke weights at (T, rho)
     const double lnT = log(Tsi);
     double mu[NS], den[NS];
@@ -844,47 +985,6 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     }
 }
 
-// ------------------------------------------------ surrogate features
-template <typename P>
-__global__ __launch_bounds__(BLOCK) void k_mc_sur_features(MultiView<P> v, float* X, uint8_t* flag)
-{
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= v.d.n) return;
-    float* row = X + (int64_t)c * 16;
-    if (v.type[c] != TYPE_F) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) row[q] = 0.f;
-        flag[c] = 0;
-        return;
-    }
-    const int64_t n = v.d.n;
-    const double rho = v.fld[LBM_FIELD_RHO][c];
-    double Y0[NS];
-#pragma unroll
-    for (int a = 0; a < NS; ++a) Y0[a] = v.rhoa[a * n + c] / rho;
-    normalize(Y0);
-    const double Tsi = v.u.si_temp(v.fld[LBM_FIELD_T][c]);
-    const double rho_si = v.u.si_rho(rho);
-    float raw[16];
-    raw[0] = (float)Tsi;
-    raw[1] = (float)pressure(Tsi, rho_si, Y0);
-#pragma unroll
-    for (int a = 0; a < NS; ++a) raw[2 + a] = (float)Y0[a];
-    raw[11] = v.u.s;
-    const float* st = v.sur_stats;
-    bool out = false;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        if (q < 12) {
-            out |= raw[q] < st[SS_LO + q] || raw[q] > st[SS_HI + q];
-            row[q] = (raw[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q];
-        } else {
-            row[q] = 0.f;
-        }
-    }
-    flag[c] = out ? 1 : 0;
-}
-
 // ------------------------------------------------------------- launchers
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
@@ -893,6 +993,7 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
 {
     switch (k) {
     case McKernel::Init: hipLaunchKernelGGL(k_mc_init<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::Chemistry: hipLaunchKernelGGL(k_mc_chemistry<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     case McKernel::CollideSpecies:
         hipLaunchKernelGGL(k_mc_collide_species<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);
         break;
@@ -908,14 +1009,35 @@ void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int
     if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
 }
 
-template <typename P>
-void launch_mc_sur_features(const MultiView<P>& v, float* X, uint8_t* flag, hipStream_t s)
+static int cu_count()
 {
-    hipLaunchKernelGGL(k_mc_sur_features<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, X, flag);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return cus;
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
+template <typename P>
+void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, unsigned* list, unsigned* count,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, count);
+    const int64_t groups = (v.d.n + SUR_WAVES * 32 - 1) / (SUR_WAVES * 32);
+    const int grid = (int)(groups < cu_count() ? groups : cu_count());
+    hipLaunchKernelGGL(k_mc_sur_chemistry<P>, dim3(grid), dim3(512), 0, s, v, net, list, count);
+}
+
+template <typename P>
+void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsigned* count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mc_chem_list<P>, dim3(4 * cu_count()), dim3(BLOCK), 0, s, v, list, count);
 }
 
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
-template void launch_mc_sur_features<double>(const MultiView<double>&, float*, uint8_t*, hipStream_t);
+template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const sur::Mlp&, unsigned*, unsigned*,
+                                              hipStream_t);
+template void launch_mc_chem_list<double>(const MultiView<double>&, const unsigned*, const unsigned*, hipStream_t);
 template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
 
 }  // namespace lbm

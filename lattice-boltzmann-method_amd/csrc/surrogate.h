@@ -16,6 +16,7 @@ constexpr int NOUT = 10;                           // dT, dY_k (9)
 struct Mlp {
     float *W[NL] = {}, *b[NL] = {}, *mW[NL] = {}, *vW[NL] = {}, *mb[NL] = {}, *vb[NL] = {};
     uint16_t *Wb[NL] = {}, *WbT[NL] = {};
+    uint16_t* Wf = nullptr;  // all layers in MFMA fragment order (in-step inference)
 };
 
 // Activations and gradient scratch for batches of up to `cap` rows.
@@ -30,6 +31,13 @@ struct Work {
     float* loss_rows = nullptr;
 };
 
+// Capture rows [rows][32]: raw features T [K], p [Pa], Y_k (9), dt [s], 0 x4 |
+// targets dT [K], dY_k (9) over one step, 0 x6.
+constexpr int CAP_STRIDE = 32;
+// Normalisation and envelope record [96]: feature mean, std, target mean,
+// std, feature min, max (16 each).
+enum { SS_FMEAN = 0, SS_FSTD = 16, SS_TMEAN = 32, SS_TSTD = 48, SS_LO = 64, SS_HI = 80, SS_SIZE = 96 };
+
 hipError_t mlp_alloc(Mlp& net, uint64_t seed, hipStream_t s);
 void mlp_free(Mlp& net);
 hipError_t work_alloc(Work& w, int cap);
@@ -43,8 +51,20 @@ void mlp_forward(const Mlp& net, const uint16_t* X, int M, Work& w, hipStream_t 
 void mlp_grad(const Mlp& net, const uint16_t* X, const float* T, int M, Work& w, hipStream_t s);
 // One Adam step (step >= 1) with the gradients in w.
 void mlp_adam(Mlp& net, Work& w, float lr, int step, hipStream_t s);
-// Fused inference: Y [M][16] FP32 from X [M][16] FP32 (any M), activations in LDS.
+// In-step style inference: Y [M][16] FP32 from normalised X [M][16] FP32 (any
+// M), activations in registers, weights staged in LDS.
 void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s);
+// Column statistics of `rows` capture rows into `stats` [SS_SIZE] (device);
+// `part` holds STATS_BLOCKS * 32 * 4 doubles.
+constexpr int STATS_BLOCKS = 256;
+void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s);
+// Training batch of B rows drawn (hash of seed, iteration, row) from the
+// capture: BF16 normalised features [B][16] and FP32 normalised targets [B][16].
+void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, int it, int B, uint16_t* X,
+                  float* T, hipStream_t s);
+// out[0] = sum(loss_rows[0..B)) * scale, summed in a fixed order.
+void loss_sum(const float* loss_rows, int B, float scale, float* out, hipStream_t s);
+constexpr int NPARAM = 16 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 16 + 16;
 
 }  // namespace sur
 }  // namespace lbm

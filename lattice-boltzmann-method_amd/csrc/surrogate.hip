@@ -3,41 +3,19 @@
 // FP32 accumulation, Adam on FP32 master weights.
 //
 // Layers: 16 (12 features, zero-padded) -> 128 -> 128 -> 128 -> 16 (10
-// outputs, zero-padded), GELU on the hidden layers.  Every GEMM is
+// outputs, zero-padded), GELU on the hidden layers.  Every training GEMM is
 // C[m][n] = sum_k A[m][k] B[n][k] (+ bias) on 16x16 output tiles, one
 // wavefront per tile, v_mfma_f32_16x16x16_bf16 with 4 BF16 of A and of B per
-// lane per K-step of 16:
-//   A fragment: row m0 + (lane % 16), k = k0 + 4 (lane / 16) + 0..3
-//   B fragment: col n0 + (lane % 16), k = k0 + 4 (lane / 16) + 0..3
-//   D fragment: col n0 + (lane % 16), row m0 + 4 (lane / 16) + 0..3
+// lane per K-step of 16 (operand layout in surrogate_device.h).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
 #include "surrogate.h"
+#include "surrogate_device.h"
 
 namespace lbm {
 namespace sur {
-
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__device__ inline uint16_t f2bf(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);                                   // round to nearest even
-    return (uint16_t)(u >> 16);
-}
-__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ inline float gelu_grad(float x)
-{
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-    return cdf + x * pdf;
-}
 
 // One 16x16 tile of C = A B^T with A [M][lda] and B [N][ldb] in BF16 (K
 // multiple of 16, M and N multiples of 16).
@@ -185,6 +163,18 @@ __global__ void k_to_bf(const float* in, uint16_t* out, int64_t n)
     if (i < n) out[i] = f2bf(in[i]);
 }
 
+// W[L] (FP32, [N][K]) into the fragment-ordered BF16 block of layer L.
+__global__ void k_to_frag(const float* W, int N, int K, uint16_t* Wf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * K) return;
+    const int KS = K / 16;
+    const int q = i & 3, lane = (i >> 2) & 63, ts = i >> 8;
+    const int t = ts / KS, s = ts % KS;
+    const int n = 16 * t + (lane & 15), k = 16 * s + 4 * (lane >> 4) + q;
+    Wf[i] = f2bf(W[n * K + k]);
+}
+
 __global__ void k_to_bf_T(const float* in, int rows, int cols, uint16_t* out, uint16_t* outT)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,68 +187,168 @@ __global__ void k_to_bf_T(const float* in, int rows, int cols, uint16_t* out, ui
 // ------------------------------------------------------------------ host
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// ------------------------------------------------- fused inference kernel
-// One block of 256 threads (4 wavefronts) evaluates the whole network for 64
-// rows with the activations kept in LDS: each wavefront owns 16 rows and all
-// output tiles of a layer.  Rows are fed by `features(row, out16)`.
-constexpr int IROWS = 64;
+// ------------------------------------------------------ inference kernel
+// 512 threads (8 wavefronts x 32 rows), weights and biases staged in LDS
+// once per block, grid-stride over groups of 256 rows.
+constexpr int IWAVES = 8, IROWS = IWAVES * 32;
 
-__device__ inline void lds_layer(const uint16_t* in, int K, const uint16_t* W, const float* b, int N, uint16_t* out,
-                                 float* outf, bool act)
+__global__ __launch_bounds__(512) void k_mlp_infer(Mlp net, const float* X, int M, float* Y)
 {
+    __shared__ __align__(16) uint16_t Wf[FRAG_TOTAL];
+    __shared__ float bias[BIAS_TOTAL];
+    __shared__ __align__(16) uint16_t xin[IWAVES][32 * 16];
+    __shared__ float yout[IWAVES][32 * 16];
+    stage_net(net, Wf, bias);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    const int m0 = wave * 16;
-    for (int n0 = 0; n0 < N; n0 += 16) {
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            const uint16_t* pa = in + (m0 + r) * K + k0 + kq;
-            const uint16_t* pb = W + (size_t)(n0 + r) * K + k0 + kq;
-            v4s a, bb;
-            a[0] = (short)pa[0]; a[1] = (short)pa[1]; a[2] = (short)pa[2]; a[3] = (short)pa[3];
-            bb[0] = (short)pb[0]; bb[1] = (short)pb[1]; bb[2] = (short)pb[2]; bb[3] = (short)pb[3];
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
+    for (int base = blockIdx.x * IROWS; base < M; base += gridDim.x * IROWS) {
+        __syncthreads();
+        for (int i = lane; i < 32 * 16; i += 64) {
+            const int row = base + wave * 32 + i / 16;
+            xin[wave][i] = f2bf(row < M ? X[(size_t)row * 16 + (i % 16)] : 0.f);
         }
-        const int col = n0 + r;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = m0 + (lane >> 4) * 4 + v;
-            const float z = acc[v] + b[col];
-            if (out) out[row * N + col] = f2bf(act ? gelu(z) : z);
-            if (outf) outf[row * N + col] = z;
+        __syncthreads();
+        mlp_rows32(Wf, bias, xin[wave], yout[wave]);
+        __syncthreads();
+        for (int i = lane; i < 32 * 16; i += 64) {
+            const int row = base + wave * 32 + i / 16;
+            if (row < M) Y[(size_t)row * 16 + (i % 16)] = yout[wave][i];
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(Mlp net, const float* X, int M, float* Y)
+int infer_grid(int M)
 {
-    __shared__ uint16_t h0[IROWS * 16];
-    __shared__ uint16_t h1[IROWS * 128];
-    __shared__ uint16_t h2[IROWS * 128];
-    __shared__ float yout[IROWS * 16];
-    const int base = blockIdx.x * IROWS;
-    for (int i = threadIdx.x; i < IROWS * 16; i += 256) {
-        const int row = base + i / 16;
-        h0[i] = f2bf(row < M ? X[(size_t)row * 16 + (i % 16)] : 0.f);
-    }
-    __syncthreads();
-    lds_layer(h0, 16, net.Wb[0], net.b[0], 128, h1, nullptr, true);
-    __syncthreads();
-    lds_layer(h1, 128, net.Wb[1], net.b[1], 128, h2, nullptr, true);
-    __syncthreads();
-    lds_layer(h2, 128, net.Wb[2], net.b[2], 128, h1, nullptr, true);
-    __syncthreads();
-    lds_layer(h1, 128, net.Wb[3], net.b[3], 16, nullptr, yout, false);
-    __syncthreads();
-    for (int i = threadIdx.x; i < IROWS * 16; i += 256) {
-        const int row = base + i / 16;
-        if (row < M) Y[(size_t)row * 16 + (i % 16)] = yout[i];
-    }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int groups = cdiv(M, IROWS);
+    return groups < cus ? (groups > 0 ? groups : 1) : cus;
 }
 
 void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(cdiv(M, IROWS)), dim3(256), 0, s, net, X, M, Y);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(infer_grid(M)), dim3(512), 0, s, net, X, M, Y);
+}
+
+// ---------------------------------------------------- training data
+// Per-block partial sum, sum of squares, min and max of the 32 capture
+// columns: thread t owns column t % 32 of rows (t / 32) + 8 (block + k grid).
+__global__ __launch_bounds__(256) void k_cap_stats(const float* cap, int64_t rows, double* part)
+{
+    __shared__ double red[256][4];
+    const int q = threadIdx.x & 31, g = threadIdx.x >> 5;
+    double s = 0, s2 = 0, lo = INFINITY, hi = -INFINITY;
+    for (int64_t r = (int64_t)blockIdx.x * 8 + g; r < rows; r += (int64_t)gridDim.x * 8) {
+        const double x = cap[r * CAP_STRIDE + q];
+        s += x;
+        s2 += x * x;
+        lo = x < lo ? x : lo;
+        hi = x > hi ? x : hi;
+    }
+    red[threadIdx.x][0] = s;
+    red[threadIdx.x][1] = s2;
+    red[threadIdx.x][2] = lo;
+    red[threadIdx.x][3] = hi;
+    __syncthreads();
+    if (g == 0) {
+        for (int k = 1; k < 8; ++k) {
+            const double* o = red[k * 32 + q];
+            s += o[0];
+            s2 += o[1];
+            lo = o[2] < lo ? o[2] : lo;
+            hi = o[3] > hi ? o[3] : hi;
+        }
+        double* p = part + ((size_t)blockIdx.x * 32 + q) * 4;
+        p[0] = s;
+        p[1] = s2;
+        p[2] = lo;
+        p[3] = hi;
+    }
+}
+
+// Final reduction (fixed order) into the normalisation record.  A column with
+// no spread (dt, inert species) gets std 1.
+__global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, float* st)
+{
+    const int q = threadIdx.x;
+    if (q >= 32) return;
+    double s = 0, s2 = 0, lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < nblk; ++b) {
+        const double* p = part + ((size_t)b * 32 + q) * 4;
+        s += p[0];
+        s2 += p[1];
+        lo = p[2] < lo ? p[2] : lo;
+        hi = p[3] > hi ? p[3] : hi;
+    }
+    const double mean = s / (double)rows;
+    const double var = s2 / (double)rows - mean * mean;
+    double sd = var > 0 ? sqrt(var) : 0.0;
+    if (!(sd > 1e-6 * fabs(mean)) || !(sd > 1e-30)) sd = 1.0;
+    if (q < 16) {
+        const bool used = q < NIN;
+        st[SS_FMEAN + q] = used ? (float)mean : 0.f;
+        st[SS_FSTD + q] = used ? (float)sd : 1.f;
+        st[SS_LO + q] = used ? (float)lo : 0.f;
+        st[SS_HI + q] = used ? (float)hi : 0.f;
+    } else {
+        const int t = q - 16;
+        const bool used = t < NOUT;
+        st[SS_TMEAN + t] = used ? (float)mean : 0.f;
+        st[SS_TSTD + t] = used ? (float)sd : 1.f;
+    }
+}
+
+void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cap_stats, dim3(STATS_BLOCKS), dim3(256), 0, s, cap, rows, part);
+    hipLaunchKernelGGL(k_cap_stats_final, dim3(1), dim3(64), 0, s, part, STATS_BLOCKS, rows, stats);
+}
+
+__device__ inline uint64_t splitmix(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_gather(const float* cap, int64_t rows, const float* st, uint64_t seed, int it, int B, uint16_t* X,
+                         float* T)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const uint64_t h = splitmix(seed ^ splitmix(((uint64_t)it << 32) | (uint64_t)b));
+    const int64_t r = (int64_t)(h % (uint64_t)rows);
+    const float* row = cap + r * CAP_STRIDE;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        X[(size_t)b * 16 + q] = f2bf(q < NIN ? (row[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q] : 0.f);
+        T[(size_t)b * 16 + q] = q < NOUT ? (row[16 + q] - st[SS_TMEAN + q]) / st[SS_TSTD + q] : 0.f;
+    }
+}
+
+void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, int it, int B, uint16_t* X,
+                  float* T, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(B, 256)), dim3(256), 0, s, cap, rows, stats, seed, it, B, X, T);
+}
+
+__global__ __launch_bounds__(256) void k_loss_sum(const float* lr, int B, float scale, float* out)
+{
+    __shared__ float red[256];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < B; i += 256) s += lr[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+void loss_sum(const float* loss_rows, int B, float scale, float* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, loss_rows, B, scale, out);
 }
 
 void mlp_forward(const Mlp& net, const uint16_t* X, int M, Work& w, hipStream_t s)
@@ -306,6 +396,8 @@ void mlp_adam(Mlp& net, Work& w, float lr, int step, hipStream_t s)
         hipLaunchKernelGGL(k_adam, dim3(cdiv(DIMS[L + 1], 256)), dim3(256), 0, s, net.b[L], w.gb[L], net.mb[L],
                            net.vb[L], DIMS[L + 1], lr, b1, b2, eps, c1, c2, (uint16_t*)nullptr, (uint16_t*)nullptr, 0,
                            1);
+        hipLaunchKernelGGL(k_to_frag, dim3(cdiv(n, 256)), dim3(256), 0, s, net.W[L], DIMS[L + 1], DIMS[L],
+                           net.Wf + frag_offset(L));
     }
 }
 
@@ -320,6 +412,8 @@ void refresh_bf16(Mlp& net, hipStream_t s)
         const int n = DIMS[L] * DIMS[L + 1];
         hipLaunchKernelGGL(k_to_bf_T, dim3(cdiv(n, 256)), dim3(256), 0, s, net.W[L], DIMS[L + 1], DIMS[L], net.Wb[L],
                            net.WbT[L]);
+        hipLaunchKernelGGL(k_to_frag, dim3(cdiv(n, 256)), dim3(256), 0, s, net.W[L], DIMS[L + 1], DIMS[L],
+                           net.Wf + frag_offset(L));
     }
 }
 
@@ -341,6 +435,7 @@ hipError_t mlp_alloc(Mlp& net, uint64_t seed, hipStream_t s)
         return (double)(z >> 11) * (1.0 / 9007199254740992.0);
     };
     hipError_t e;
+    if ((e = hipMalloc((void**)&net.Wf, 2 * FRAG_TOTAL)) != hipSuccess) return e;
     for (int L = 0; L < NL; ++L) {
         const int rows = DIMS[L + 1], cols = DIMS[L], n = rows * cols;
         float** f32s[] = {&net.W[L], &net.mW[L], &net.vW[L]};
@@ -380,6 +475,7 @@ void mlp_free(Mlp& net)
         for (void* p : ps)
             if (p) (void)hipFree(p);
     }
+    if (net.Wf) (void)hipFree(net.Wf);
     net = Mlp{};
 }
 

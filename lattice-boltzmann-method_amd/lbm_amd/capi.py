@@ -28,8 +28,17 @@ EXPORTS = [
     "lbm_get_field", "lbm_set_populations", "lbm_get_populations", "lbm_init", "lbm_collide_species",
     "lbm_collide", "lbm_stream", "lbm_tms_bc", "lbm_moments", "lbm_step", "lbm_diagnostics", "lbm_sync",
     "lbm_get_stream", "lbm_profile_enable", "lbm_profile_read", "lbm_device_bytes", "lbm_set_species_field",
-    "lbm_get_species_field",
+    "lbm_get_species_field", "lbm_surrogate_param_count", "lbm_surrogate_init", "lbm_surrogate_capture",
+    "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
+    "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
+    "lbm_surrogate_forward", "lbm_surrogate_gradient",
 ]
+
+# surrogate network (include/lbm_amd.h): layer widths, real inputs / outputs
+SUR_DIMS = (16, 128, 128, 128, 16)
+SUR_NIN, SUR_NOUT = 12, 10
+SUR_FEATURES = ["T", "p"] + [f"Y_{s}" for s in ("H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2")] + ["dt"]
+SUR_TARGETS = ["dT"] + [f"dY_{s}" for s in ("H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2")]
 
 
 class LbmError(RuntimeError):
@@ -42,6 +51,15 @@ class Config(ctypes.Structure):
                 ("nu", ctypes.c_double), ("gas_const", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("prtl", ctypes.c_double), ("ref_rho", ctypes.c_double), ("ref_T", ctypes.c_double),
                 ("dx_si", ctypes.c_double), ("dt_si", ctypes.c_double), ("reaction", ctypes.c_int)]
+
+
+class SurrogateStats(ctypes.Structure):
+    _fields_ = [("captured", ctypes.c_int64), ("predicted", ctypes.c_int64), ("fallback", ctypes.c_int64),
+                ("audited", ctypes.c_int64), ("audit_failures", ctypes.c_int64),
+                ("max_audit_error", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Diag(ctypes.Structure):
@@ -88,12 +106,30 @@ def load_library(path=LIB_PATH):
     L.lbm_profile_read.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64)]
     L.lbm_device_bytes.argtypes = [vp]
     L.lbm_device_bytes.restype = ctypes.c_int64
+    fp, i64, u64 = ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_uint64
+    L.lbm_surrogate_param_count.argtypes = []
+    L.lbm_surrogate_param_count.restype = i64
+    L.lbm_surrogate_init.argtypes = [vp, u64]
+    L.lbm_surrogate_capture.argtypes = [vp, i64, i32]
+    L.lbm_surrogate_get_capture.argtypes = [vp, fp, i64, ctypes.POINTER(i64)]
+    L.lbm_surrogate_train.argtypes = [vp, i32, i32, ctypes.c_float, u64, fp]
+    L.lbm_surrogate_enable.argtypes = [vp, i32, i32, ctypes.c_double]
+    L.lbm_surrogate_read_stats.argtypes = [vp, ctypes.POINTER(SurrogateStats)]
+    for fn in ("lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm",
+               "lbm_surrogate_set_norm"):
+        getattr(L, fn).argtypes = [vp, fp]
+    L.lbm_surrogate_forward.argtypes = [vp, fp, i64, fp]
+    L.lbm_surrogate_gradient.argtypes = [vp, fp, fp, i64, fp, fp]
     _LIB = L
     return L
 
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 class Lattice:
@@ -280,6 +316,51 @@ class Lattice:
     def device_bytes(self):
         return self._L.lbm_device_bytes(self._h)
 
+    # -- chemistry surrogate network (the network calls work on any context)
+    def surrogate_init(self, seed):
+        self._check(self._L.lbm_surrogate_init(self._h, seed), "surrogate_init")
+
+    def surrogate_params(self):
+        out = np.empty(self._L.lbm_surrogate_param_count(), np.float32)
+        self._check(self._L.lbm_surrogate_get_params(self._h, _fp(out)), "surrogate_get_params")
+        return out
+
+    def set_surrogate_params(self, params):
+        a = np.ascontiguousarray(params, np.float32)
+        if a.size != self._L.lbm_surrogate_param_count():
+            raise ValueError("parameter vector has the wrong length")
+        self._check(self._L.lbm_surrogate_set_params(self._h, _fp(a)), "surrogate_set_params")
+
+    def surrogate_norm(self):
+        out = np.empty(96, np.float32)
+        self._check(self._L.lbm_surrogate_get_norm(self._h, _fp(out)), "surrogate_get_norm")
+        return out
+
+    def set_surrogate_norm(self, norm):
+        a = np.ascontiguousarray(norm, np.float32)
+        if a.size != 96:
+            raise ValueError("normalisation record has 96 entries")
+        self._check(self._L.lbm_surrogate_set_norm(self._h, _fp(a)), "surrogate_set_norm")
+
+    def surrogate_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        if X.ndim != 2 or X.shape[1] != 16:
+            raise ValueError("X must be [m][16]")
+        Y = np.empty_like(X)
+        self._check(self._L.lbm_surrogate_forward(self._h, _fp(X), X.shape[0], _fp(Y)), "surrogate_forward")
+        return Y
+
+    def surrogate_gradient(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        if X.shape != T.shape or X.ndim != 2 or X.shape[1] != 16:
+            raise ValueError("X and T must both be [m][16]")
+        g = np.empty(self._L.lbm_surrogate_param_count(), np.float32)
+        loss = ctypes.c_float()
+        self._check(self._L.lbm_surrogate_gradient(self._h, _fp(X), _fp(T), X.shape[0], _fp(g), ctypes.byref(loss)),
+                    "surrogate_gradient")
+        return float(loss.value), g
+
 
 class MixtureLattice(Lattice):
     """Multicomponent (9-species H2-air) lattice, mirroring
@@ -293,3 +374,32 @@ class MixtureLattice(Lattice):
         cfg = Config(nx, ny, nz, len(self.SPECIES), precision, mode, device, 0.0, 0.0, gamma, 0.0, 1.0, 1.0,
                      dx_si, dt_si, 1 if reaction else 0)
         self._create(cfg)
+
+    # -- chemistry surrogate in the step
+    def surrogate_capture(self, max_rows, stride=1):
+        self._check(self._L.lbm_surrogate_capture(self._h, int(max_rows), int(stride)), "surrogate_capture")
+
+    def surrogate_captured(self, max_rows=None):
+        n = ctypes.c_int64()
+        self._check(self._L.lbm_surrogate_get_capture(self._h, None, 0, ctypes.byref(n)), "surrogate_get_capture")
+        m = n.value if max_rows is None else min(n.value, max_rows)
+        out = np.empty((max(m, 0), 32), np.float32)
+        if m > 0:
+            self._check(self._L.lbm_surrogate_get_capture(self._h, _fp(out), m, ctypes.byref(n)),
+                        "surrogate_get_capture")
+        return out
+
+    def surrogate_train(self, iters, batch=4096, lr=1e-3, seed=1):
+        losses = np.zeros(max(iters, 1), np.float32)
+        self._check(self._L.lbm_surrogate_train(self._h, int(iters), int(batch), float(lr), int(seed), _fp(losses)),
+                    "surrogate_train")
+        return losses[:iters]
+
+    def surrogate_enable(self, on=True, audit_stride=64, audit_tol=0.5):
+        self._check(self._L.lbm_surrogate_enable(self._h, 1 if on else 0, int(audit_stride), float(audit_tol)),
+                    "surrogate_enable")
+
+    def surrogate_stats(self):
+        st = SurrogateStats()
+        self._check(self._L.lbm_surrogate_read_stats(self._h, ctypes.byref(st)), "surrogate_read_stats")
+        return st.as_dict()

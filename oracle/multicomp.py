@@ -247,7 +247,11 @@ class MultiState:
             self.f[a, :, c] = feq(ra, vel, th_a, zero).T
 
     # --------------------------------------------------- Collide_Species
-    def collide_species(self, reaction=True):
+    def collide_species(self, reaction=True, chemistry=None):
+        """Collide_Species; `chemistry(c, Tsi, rho_si, Y0, rho, u_si)` may
+        return (src [NS][nc], T_final) to replace the plain integration (the
+        surrogate path); the sources used are kept in self.src_chem and the
+        final temperature in self.T_chem."""
         U_, m = self.units, self.mech
         c = np.nonzero(self.type == TYPE_F)[0]
         nc = len(c)
@@ -264,13 +268,21 @@ class MultiState:
             for a in range(self.ns):
                 Uint = Uint - 0.5 * rho_a[a] / rho * (ua[0, a] ** 2 + ua[1, a] ** 2 + ua[2, a] ** 2)
             u_si = U_.si_energy_mass(Uint).astype(np.float64)
-            Tcur, Ycur = Tsi.copy(), Y0.copy()
-            for _ in range(2):
-                wdot = m.net_production(Tcur, rho_si, Ycur)
-                for a in range(self.ns):
-                    src[a] += 1.0 / 2.0 * U_.rho_dot(wdot[:, a] * m.W[a]).astype(np.float64)
-                Ycur = m.normalize_Y(((rho_a + src) / rho).T)
-                Tcur = m.T_from_UV(u_si, Ycur, Tcur)
+            if chemistry is None:
+                Tcur, Ycur = Tsi.copy(), Y0.copy()
+                for _ in range(2):
+                    wdot = m.net_production(Tcur, rho_si, Ycur)
+                    for a in range(self.ns):
+                        src[a] += 1.0 / 2.0 * U_.rho_dot(wdot[:, a] * m.W[a]).astype(np.float64)
+                    Ycur = m.normalize_Y(((rho_a + src) / rho).T)
+                    Tcur = m.T_from_UV(u_si, Ycur, Tcur)
+            else:
+                src, Tcur = chemistry(c, Tsi, rho_si, Y0, rho, u_si)
+                src = np.array(src, np.float64)
+            self.src_chem = np.zeros((self.ns, self.n))
+            self.src_chem[:, c] = src
+            self.T_chem = np.zeros(self.n)
+            self.T_chem[c] = Tcur
             hm = m.h_molar(Tcur)
             hrr = np.zeros(nc)
             for a in range(self.ns):
