@@ -8,10 +8,18 @@ Collide, Streaming, TMS_BC, calculate_moment) over the whole synthetic box;
 the timed region starts with the state already resident in HBM.
 
 Workloads
-  turb256   (default) thermal compressible decaying turbulence, 256^3
-            periodic, single component: the FP32 shifted-population path is
-            `value`; the FP64 plain path is timed beside it for the speedup
-            and the FP32 error against FP64 after the timed steps.
+  mixing256 reacting H2-air / hot-air temporal mixing layer, 256x128x128,
+            9 species, periodic x/z, NSCBC outflow on both y faces.  `value`
+            is the FP32 shifted-population path with the learned chemistry
+            surrogate (trained on the GPU, inside this run, on states the
+            plain integrator produced during the warm-up; training time is
+            reported, not timed); the FP64 plain path is timed beside it for
+            the speedup, and both runs' fields are compared after the same
+            number of steps (flame temperature, T and species errors).
+  turb256   thermal compressible decaying turbulence, 256^3 periodic, single
+            component: the FP32 shifted-population path is `value`; the FP64
+            plain path is timed beside it for the speedup and the FP32 error
+            against FP64 after the timed steps.
   tgv64     3-D Taylor-Green vortex, 64^3, FP64 plain path.
 """
 import argparse
@@ -39,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="turb256", choices=["turb256", "tgv64"])
+    ap.add_argument("--workload", default="mixing256", choices=["mixing256", "turb256", "tgv64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -56,7 +64,12 @@ def make_lattice(fl, pr, precision, mode):
     return lat
 
 
-def timed(lat, steps, warmup):
+SINGLE_KERNELS = ("fused_collide_stream", "fused_moments", "collide_explicit")
+MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_fallback", "mc_collide_species",
+                 "mc_collide", "mc_stream", "mc_tms_outflow", "mc_moments")
+
+
+def timed(lat, steps, warmup, names=SINGLE_KERNELS):
     """Warm-up, then exactly `steps` steps between two device synchronisations;
     kernel launches are bracketed by HIP events on the library's stream."""
     lat.step(warmup)
@@ -67,7 +80,7 @@ def timed(lat, steps, warmup):
     lat.sync()
     t1 = time.perf_counter()
     kernels = {}
-    for name in ("fused_collide_stream", "fused_moments", "collide_explicit"):
+    for name in names:
         ms, n = lat.profile_read(name)
         if n:
             kernels[name] = {"avg_ms": ms / n, "launches": n, "total_ms": ms}
@@ -105,6 +118,99 @@ def cpu_baseline(threads):
                       f"{N}^3 periodic box, {steps} steps, {r['seconds']:.1f} s, {r['threads']} OpenMP threads, {cpu}"}
 
 
+# Algorithmic HBM bytes per cell of the multicomponent kernels: the
+# populations each one reads and writes (9 species x 27 f, 27 g), per byte
+# of storage.  Moment and property planes are excluded.
+MC_POP_TRAFFIC = {"mc_collide_species": 2 * 9 * 27, "mc_collide": 2 * 27, "mc_stream": 2 * 10 * 27,
+                  "mc_moments": 10 * 27}
+
+
+def mixing_layer_bench(args):
+    """The reacting mixing layer: FP64 plain beside FP32 + surrogate."""
+    from lbm_amd import cases
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, MixtureLattice
+    nx, ny, nz = 256, 128, 128
+    cells = nx * ny * nz
+    fl = cases.mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
+    F = (fl["type"] == 0).ravel()
+    cap_steps, cap_stride = 4, 4          # ~4.2 M training states
+    train_iters, batch = 2000, 4096
+    steps = args.steps
+
+    def fields(lat):
+        rho_a = lat.get_species_field("rho").reshape(9, -1)[:, F]
+        return lat.get_field("T").ravel()[F], rho_a / rho_a.sum(axis=0, keepdims=True)
+
+    # FP32 shifted populations + learned chemistry
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED)
+    lat.load_case(fl)
+    lat.Init()
+    lat.calculate_moment()
+    lat.surrogate_capture(cap_steps * cells // cap_stride + 4096, cap_stride)
+    lat.step(cap_steps)
+    lat.surrogate_capture_stop()
+    lat.sync()
+    t0 = time.perf_counter()
+    losses = np.concatenate([lat.surrogate_train(train_iters * 3 // 4, batch=batch, lr=2e-3, seed=1),
+                             lat.surrogate_train(train_iters // 4, batch=batch, lr=3e-4, seed=2)])
+    train_s = time.perf_counter() - t0
+    lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
+    sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS)
+    T32, Y32 = fields(lat)
+    sstats = lat.surrogate_stats()
+    diag = lat.diagnostics()
+    dev_bytes = lat.device_bytes()
+    lat.close()
+    mlups = cells * steps / sec / 1e6
+
+    extra = {"kernels": {n: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]} for n, v in kern.items()},
+             "surrogate": {"train_s": round(train_s, 2), "train_iters": train_iters, "batch": batch,
+                           "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-50:].mean()),
+                           "captured_states": None, **sstats},
+             "device_bytes_fp32": dev_bytes, "nan_count": diag["nan_count"]}
+    extra["surrogate"]["captured_states"] = extra["surrogate"].pop("captured")
+    extra["fallback_rate"] = round((sstats["fallback"] + sstats["audit_failures"]) / max(sstats["predicted"] +
+                                                                                        sstats["fallback"], 1), 6)
+
+    # dominant kernel's roofline (FP32 storage)
+    roofline = None
+    dom = max((n for n in kern if n in MC_POP_TRAFFIC), key=lambda n: kern[n]["avg_ms"], default=None)
+    if dom:
+        bpl = MC_POP_TRAFFIC[dom] * 4
+        achieved = bpl * cells / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "bytes_per_launch": bpl * cells, "avg_ms": round(kern[dom]["avg_ms"], 4)}
+
+    if not args.no_compare:
+        ref = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP64)
+        ref.load_case(fl)
+        ref.Init()
+        ref.calculate_moment()
+        sec64, kern64 = timed(ref, steps, args.warmup + cap_steps, MULTI_KERNELS)
+        T64, Y64 = fields(ref)
+        ref.close()
+        mlups64 = cells * steps / sec64 / 1e6
+        extra["fp64_mlups"] = round(mlups64, 2)
+        extra["speedup_fp32_surrogate_vs_fp64"] = round(mlups / mlups64, 3)
+        extra["fp64_kernels"] = {n: {"avg_ms": round(v["avg_ms"], 4)} for n, v in kern64.items()}
+        extra["error_vs_fp64"] = {
+            "flame_T_rel": float(f"{abs(T32.max() - T64.max()) / T64.max():.3e}"),
+            "T_max_rel": float(f"{np.max(np.abs(T32 - T64) / T64):.3e}"),
+            "Y_L2": float(f"{np.sqrt(np.mean((Y32 - Y64) ** 2)):.3e}"),
+            "Y_max_abs": float(f"{np.max(np.abs(Y32 - Y64)):.3e}"),
+        }
+    rec = {
+        "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(sec / steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "reacting_mixing_layer_256x128x128_9sp_fp32_shifted_surrogate", "cells": cells,
+                   "grid": [nx, ny, nz], "mode": "strict", "parallelism": "none"},
+        "roofline": roofline, "cpu_baseline": None, "extra": extra, "host": platform.node(),
+    }
+    return rec
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -115,6 +221,12 @@ def main():
     from lbm_amd import cases
     from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED
 
+    if args.workload == "mixing256":
+        rec = mixing_layer_bench(args)
+        if not args.no_cpu_baseline:
+            rec["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+        print(json.dumps(rec))
+        return
     if args.workload == "turb256":
         N = 256
         fl, pr = cases.decaying_turbulence(N, nu=0.005)

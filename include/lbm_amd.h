@@ -188,8 +188,9 @@ int64_t lbm_surrogate_param_count(void);
  * Adam moments) and forgets the normalisation record. */
 int lbm_surrogate_init(lbm_ctx* ctx, uint64_t seed);
 /* Samples one cell in `stride` (hashed per step) of every plain-integrator
- * chemistry call into a capture buffer of `max_rows` rows; max_rows = 0 stops
- * and releases the buffer. Rows already captured are kept while it grows. */
+ * chemistry call into a capture buffer of `max_rows` rows.  stride = 0 stops
+ * sampling and keeps the rows; max_rows = 0 releases the buffer.  Rows
+ * already captured are kept while the buffer grows. */
 int lbm_surrogate_capture(lbm_ctx* ctx, int64_t max_rows, int stride);
 /* Copies up to max_rows captured rows [rows][32] (16 raw features | 16 raw
  * targets) to the host; *rows_out receives the count. */

@@ -73,6 +73,10 @@ struct lbm_ctx {
     int tms_count = 0;
     std::string tms_unsupported;
     double* src = nullptr;  // [NS][n] chemistry mass sources
+    // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
+    double mfref[mech::NS * Q] = {}, mgref[Q] = {};
+    double* mref_dev = nullptr;  // [NS*27 + 27]
+    double* sums_part = nullptr;
 
     // chemistry surrogate
     sur::Mlp net;
@@ -166,15 +170,18 @@ SingleView<P> make_view(lbm_ctx* c, int src, int dst)
     return v;
 }
 
-MultiView<double> make_mview(lbm_ctx* c, int src, int dst)
+template <typename P>
+MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
 {
-    MultiView<double> v{};
+    MultiView<P> v{};
     v.d = c->d;
     v.type = c->type;
-    v.f = static_cast<double*>(c->sf[src]);
-    v.fo = static_cast<double*>(c->sf[dst]);
-    v.g = static_cast<double*>(c->pg[src]);
-    v.go = static_cast<double*>(c->pg[dst]);
+    v.f = static_cast<P*>(c->sf[src]);
+    v.fo = static_cast<P*>(c->sf[dst]);
+    v.g = static_cast<P*>(c->pg[src]);
+    v.go = static_cast<P*>(c->pg[dst]);
+    v.fref = c->mref_dev;
+    v.gref = c->mref_dev ? c->mref_dev + mech::NS * Q : nullptr;
     for (int i = 0; i < LBM_FIELD_COUNT; ++i) v.fld[i] = c->fld[i];
     v.hrr = c->fld[LBM_FIELD_HRR];
     const int64_t n = c->d.n;
@@ -200,14 +207,15 @@ MultiView<double> make_mview(lbm_ctx* c, int src, int dst)
     return v;
 }
 
-int run_mc(lbm_ctx* c, McKernel k, const char* name, int src, int dst)
-{
-    return launch(c, name, [&] { launch_mc<double>(k, make_mview(c, src, dst), c->stream); });
-}
-
 bool multi(const lbm_ctx* c) { return c->ns > 0; }
 
 bool fp32(const lbm_ctx* c) { return c->cfg.precision == LBM_FP32_SHIFTED; }
+
+int run_mc(lbm_ctx* c, McKernel k, const char* name, int src, int dst)
+{
+    if (fp32(c)) return launch(c, name, [&] { launch_mc<float>(k, make_mview<float>(c, src, dst), c->stream); });
+    return launch(c, name, [&] { launch_mc<double>(k, make_mview<double>(c, src, dst), c->stream); });
+}
 bool fused(const lbm_ctx* c) { return c->cfg.mode == LBM_MODE_FUSED; }
 
 // Runs `fn.template operator()<P>()` with the context's population type.
@@ -277,6 +285,49 @@ void set_reference_state(lbm_ctx* c)
     }
 }
 
+// Rest state of the FP32 multicomponent deviations: the zero-velocity
+// equilibria (as Init builds them, lbm-init.cpp:85-146) of the mean TYPE_F
+// temperature, pressure and composition of the set-up fields.
+int set_multi_reference_state(lbm_ctx* c)
+{
+    const int64_t n = c->d.n;
+    launch_mc_state_sums(c->type, c->fld[LBM_FIELD_T], c->fld[LBM_FIELD_P], c->sp_block, n, c->sums_part, c->stream);
+    std::vector<double> part((size_t)MC_SUMS * MC_SUM_BLOCKS);
+    HIP_TRY(c, hipMemcpyAsync(part.data(), c->sums_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double sum[MC_SUMS] = {};
+    for (int b = 0; b < MC_SUM_BLOCKS; ++b)
+        for (int q = 0; q < MC_SUMS; ++q) sum[q] += part[(size_t)b * MC_SUMS + q];
+    if (!(sum[0] > 0)) return fail(c, LBM_E_STATE, "no TYPE_F cells");
+    using namespace chem;
+    double X[NS], Y[NS], xs = 0;
+    for (int a = 0; a < NS; ++a) xs += (X[a] = std::max(sum[3 + a], 0.0));
+    if (!(xs > 0)) return fail(c, LBM_E_STATE, "mole fractions of the set-up are all zero");
+    for (int a = 0; a < NS; ++a) X[a] /= xs;
+    Y_from_X(X, Y);
+    const Units& u = c->units;
+    const double Tsi = u.si_temp(sum[1] / sum[0]);
+    const double psi = u.si_p(sum[2] / sum[0]);
+    if (!(Tsi > 0) || !(psi > 0)) return fail(c, LBM_E_STATE, "set-up temperature and pressure must be > 0");
+    const double Wm = mean_W(Y);
+    const double rho = u.rho(psi * Wm / (R_UNIV * Tsi));
+    const double U = u.energy_mass(u_mass(Tsi, Y));
+    const double zero[3] = {0, 0, 0};
+    const GeqAxes<double> G = geq_axes<double>(u.energy_mass(R_UNIV * Tsi / Wm), zero);
+    const double theta = u.energy_mass(R_UNIV * Tsi / Wm);
+    for (int l = 0; l < Q; ++l) c->mgref[l] = geq_dir<double>(G, l, rho, U - 1.5 * theta);
+    for (int a = 0; a < NS; ++a) {
+        const double ra = std::max(Y[a] * rho, SPECIES_MIN);
+        const Axis3<double> z = psi_axis<double>(0.0, u.energy_mass(R_UNIV * Tsi / W[a]));
+        for (int l = 0; l < Q; ++l) c->mfref[a * Q + l] = ra * pick(z, cx(l)) * pick(z, cy(l)) * pick(z, cz(l));
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->mref_dev, c->mfref, sizeof(double) * NS * Q, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->mref_dev + NS * Q, c->mgref, sizeof(double) * Q, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
 void release(lbm_ctx* c)
 {
     for (auto& r : c->pending) {
@@ -288,7 +339,7 @@ void release(lbm_ctx* c)
                     c->fld_block, c->qd_block,  c->tau_count,  c->diag_part, c->diag_out,  c->sf[0],
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
-                    c->cap_count, c->bx,        c->bt,         c->bloss};
+                    c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -326,8 +377,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return LBM_E_UNSUPPORTED;
     }
     if (cfg->n_species == mech::NS) {
-        if (cfg->mode != LBM_MODE_STRICT || cfg->precision != LBM_FP64) {
-            g_create_error = "the multicomponent path runs in LBM_MODE_STRICT with LBM_FP64";
+        if (cfg->mode != LBM_MODE_STRICT) {
+            g_create_error = "the multicomponent path runs in LBM_MODE_STRICT";
             return LBM_E_UNSUPPORTED;
         }
         if (!(cfg->dx_si > 0.0) || !(cfg->dt_si > 0.0)) {
@@ -401,7 +452,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     if (cfg->n_species > 0) {
         c->ns = cfg->n_species;
         c->units = chem::make_units(cfg->dx_si, cfg->dt_si);
-        const size_t sbytes = (size_t)n * Q * c->ns * sizeof(double);
+        const size_t sbytes = (size_t)n * Q * c->ns * c->esize;
         for (int b = 0; b < 2; ++b) {
             if ((e = hipMalloc(&c->sf[b], sbytes)) != hipSuccess) return bail("hipMalloc species populations", e);
             (void)hipMemsetAsync(c->sf[b], 0, sbytes, c->stream);
@@ -412,6 +463,13 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         const size_t srcb = sizeof(double) * (size_t)n * c->ns;
         if ((e = hipMalloc((void**)&c->src, srcb)) != hipSuccess) return bail("hipMalloc chemistry sources", e);
         (void)hipMemsetAsync(c->src, 0, srcb, c->stream);
+        if (fp32(c)) {
+            if ((e = hipMalloc((void**)&c->mref_dev, sizeof(double) * (mech::NS + 1) * Q)) != hipSuccess)
+                return bail("hipMalloc", e);
+            (void)hipMemsetAsync(c->mref_dev, 0, sizeof(double) * (mech::NS + 1) * Q, c->stream);
+            if ((e = hipMalloc((void**)&c->sums_part, sizeof(double) * MC_SUMS * MC_SUM_BLOCKS)) != hipSuccess)
+                return bail("hipMalloc", e);
+        }
         c->bytes += 2 * sbytes + fb + srcb;
     }
     c->diag_npart = diag_partials(c->d);
@@ -586,8 +644,18 @@ int lbm_set_populations(lbm_ctx* c, int which, const double* host)
     if (multi(c)) {
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
-        const size_t bytes = (size_t)c->d.n * Q * (sp ? c->ns : 1) * sizeof(double);
-        HIP_TRY(c, hipMemcpyAsync(sp ? c->sf[slot] : c->pg[slot], host, bytes, hipMemcpyHostToDevice, c->stream));
+        void* dev = sp ? c->sf[slot] : c->pg[slot];
+        const size_t count = (size_t)c->d.n * Q * (sp ? c->ns : 1);
+        if (fp32(c)) {
+            const int64_t n = c->d.n;
+            const double* ref = sp ? c->mfref : c->mgref;
+            std::vector<float> tmp(count);
+            for (size_t pl = 0; pl < count / n; ++pl)
+                for (int64_t i = 0; i < n; ++i) tmp[pl * n + i] = (float)(host[pl * n + i] - ref[pl]);
+            HIP_TRY(c, hipMemcpyAsync(dev, tmp.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+        } else {
+            HIP_TRY(c, hipMemcpyAsync(dev, host, count * 8, hipMemcpyHostToDevice, c->stream));
+        }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         return LBM_OK;
     }
@@ -634,9 +702,20 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
     if (multi(c)) {
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
-        const size_t bytes = (size_t)c->d.n * Q * (sp ? c->ns : 1) * sizeof(double);
-        HIP_TRY(c, hipMemcpyAsync(host, sp ? c->sf[slot] : c->pg[slot], bytes, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const void* dev = sp ? c->sf[slot] : c->pg[slot];
+        const size_t count = (size_t)c->d.n * Q * (sp ? c->ns : 1);
+        if (fp32(c)) {
+            const int64_t n = c->d.n;
+            const double* ref = sp ? c->mfref : c->mgref;
+            std::vector<float> tmp(count);
+            HIP_TRY(c, hipMemcpyAsync(tmp.data(), dev, count * 4, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            for (size_t pl = 0; pl < count / n; ++pl)
+                for (int64_t i = 0; i < n; ++i) host[pl * n + i] = (double)tmp[pl * n + i] + ref[pl];
+        } else {
+            HIP_TRY(c, hipMemcpyAsync(host, dev, count * 8, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
         return LBM_OK;
     }
     void* buf;
@@ -706,6 +785,8 @@ int lbm_init(lbm_ctx* c)
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede lbm_init");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (multi(c)) {
+        if (fp32(c))
+            if (int rc = set_multi_reference_state(c)) return rc;
         if (int rc = run_mc(c, McKernel::Init, "mc_init", 0, 0)) return rc;
         c->fields_valid = true;
         return LBM_OK;
@@ -729,12 +810,16 @@ int lbm_collide_species(lbm_ctx* c)
     if (c->cfg.reaction) {
         int rc;
         if (c->sur_on) {
-            const MultiView<double> v = make_mview(c, 0, 1);
-            rc = launch(c, "mc_chemistry_surrogate",
-                        [&] { launch_mc_sur_chemistry<double>(v, c->net, c->chem_list, c->chem_count, c->stream); });
-            if (!rc)
-                rc = launch(c, "mc_chemistry_fallback",
-                            [&] { launch_mc_chem_list<double>(v, c->chem_list, c->chem_count, c->stream); });
+            rc = DISPATCH(c, [&] {
+                const MultiView<P> v = make_mview<P>(c, 0, 1);
+                int r = launch(c, "mc_chemistry_surrogate", [&] {
+                    launch_mc_sur_chemistry<P>(v, c->net, c->chem_list, c->chem_count, c->stream);
+                });
+                if (!r)
+                    r = launch(c, "mc_chemistry_fallback",
+                               [&] { launch_mc_chem_list<P>(v, c->chem_list, c->chem_count, c->stream); });
+                return r;
+            }());
         } else {
             rc = run_mc(c, McKernel::Chemistry, "mc_chemistry_plain", 0, 1);
         }
@@ -782,9 +867,9 @@ int lbm_tms_bc(lbm_ctx* c)
     if (multi(c)) {
         if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
         if (!c->tms_unsupported.empty()) return fail(c, LBM_E_UNSUPPORTED, c->tms_unsupported);
-        return launch(c, "mc_tms_outflow", [&] {
-            launch_mc_tms<double>(make_mview(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
-        });
+        return DISPATCH(c, launch(c, "mc_tms_outflow", [&] {
+            launch_mc_tms<P>(make_mview<P>(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
+        }));
     }
     if (c->needs_tms)
         return fail(c, LBM_E_UNSUPPORTED,
@@ -995,10 +1080,14 @@ int lbm_surrogate_capture(lbm_ctx* c, int64_t max_rows, int stride)
 {
     if (!c) return LBM_E_ARG;
     if (int rc = need_reacting(c)) return rc;
-    if (max_rows < 0 || max_rows > ((int64_t)1 << 31) || (max_rows > 0 && stride < 1))
-        return fail(c, LBM_E_ARG, "capture needs 0 <= max_rows <= 2^31 and stride >= 1");
+    if (max_rows < 0 || max_rows > ((int64_t)1 << 31) || stride < 0)
+        return fail(c, LBM_E_ARG, "capture needs 0 <= max_rows <= 2^31 and stride >= 0");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (stride == 0 && max_rows > 0) {  // pause: keep what was captured
+        c->capturing = false;
+        return LBM_OK;
+    }
     if (max_rows == 0) {
         if (c->cap_buf) (void)hipFree(c->cap_buf);
         if (c->cap_count) (void)hipFree(c->cap_count);

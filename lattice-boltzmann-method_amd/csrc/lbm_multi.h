@@ -22,6 +22,8 @@ struct MultiView {
     chem::Units u;
     int reaction;
     unsigned long long* tau_count;
+    const double* fref;  // FP32 storage: rest-state shifts [NS][27] of the species populations
+    const double* gref;  // and [27] of the energy populations (unused for FP64)
     double* src;    // [NS][n] species mass added by the chemistry over the step (lattice units)
     double* tchem;  // [n] temperature after the chemistry [K] (surrogate audit only)
     // chemistry surrogate (all null when unused)
@@ -66,5 +68,11 @@ void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, unsigne
 // compared with the network's result and overwritten above the tolerance.
 template <typename P>
 void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsigned* count, hipStream_t s);
+
+// Per-block sums (MC_SUM_BLOCKS x MC_SUMS doubles into `part`) over TYPE_F
+// cells of 1, T, p and X_a.
+constexpr int MC_SUMS = 3 + chem::NS, MC_SUM_BLOCKS = 512;
+void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const double* X, int64_t n,
+                          double* part, hipStream_t s);
 
 }  // namespace lbm

@@ -47,6 +47,47 @@ __device__ inline P& spop(P* base, int a, int l, int64_t n, int c)
     return base[((int64_t)a * Q + l) * n + c];
 }
 
+// Population values.  FP32 storage holds deviations from the rest-state
+// equilibria fref [NS][27] and gref [27]; FP64 storage holds the values.
+template <typename P>
+__device__ inline double fshift(const MultiView<P>& v, int a, int l)
+{
+    if constexpr (sizeof(P) == 4) return v.fref[a * Q + l];
+    else return 0.0;
+}
+template <typename P>
+__device__ inline double gshift(const MultiView<P>& v, int l)
+{
+    if constexpr (sizeof(P) == 4) return v.gref[l];
+    else return 0.0;
+}
+template <typename P>
+__device__ inline double ldf(const MultiView<P>& v, const P* base, int a, int l, int c)
+{
+    const double x = (double)base[((int64_t)a * Q + l) * v.d.n + c];
+    if constexpr (sizeof(P) == 4) return x + fshift(v, a, l);
+    else return x;
+}
+template <typename P>
+__device__ inline void stf(const MultiView<P>& v, P* base, int a, int l, int c, double x)
+{
+    if constexpr (sizeof(P) == 4) x -= fshift(v, a, l);
+    base[((int64_t)a * Q + l) * v.d.n + c] = (P)x;
+}
+template <typename P>
+__device__ inline double ldg(const MultiView<P>& v, const P* base, int l, int c)
+{
+    const double x = (double)base[(int64_t)l * v.d.n + c];
+    if constexpr (sizeof(P) == 4) return x + gshift(v, l);
+    else return x;
+}
+template <typename P>
+__device__ inline void stg(const MultiView<P>& v, P* base, int l, int c, double x)
+{
+    if constexpr (sizeof(P) == 4) x -= gshift(v, l);
+    base[(int64_t)l * v.d.n + c] = (P)x;
+}
+
 // product-form equilibrium of one direction from the per-axis factors
 template <typename T>
 __device__ inline T feq_dir(const Axis3<T> F[3], int l, T rho)
@@ -90,7 +131,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_init(MultiView<P> v)
     const double theta = v.u.energy_mass(R_UNIV * Tsi / Wm);
     const GeqAxes<double> G = geq_axes<double>(theta, vel);
 #pragma unroll
-    for (int l = 0; l < Q; ++l) v.go[(int64_t)l * n + c] = (P)geq_dir<double>(G, l, rho, U - 1.5 * theta);
+    for (int l = 0; l < Q; ++l) stg(v, v.go, l, c, geq_dir<double>(G, l, rho, U - 1.5 * theta));
     const double zero[3] = {0, 0, 0};
     for (int a = 0; a < NS; ++a) {
         const double yr = Y[a] * rho;
@@ -102,7 +143,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_init(MultiView<P> v)
         Axis3<double> F[3];
         feq_axes<double>(F, vel, th, zero);
 #pragma unroll
-        for (int l = 0; l < Q; ++l) spop(v.fo, a, l, n, c) = (P)feq_dir<double>(F, l, ra);
+        for (int l = 0; l < Q; ++l) stf(v, v.fo, a, l, c, feq_dir<double>(F, l, ra));
     }
 }
 
@@ -398,38 +439,39 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
         const double visc = Xa[a] * (double)v.u.mu(mu[a]) / den[a];
         om[a] = 2 * Xa[a] * p / (Xa[a] * p + 2 * visc);
     }
-    // Maxwell-Stefan friction with the binary diffusivities of the state
     const double p_si = pressure(Tsi, rho_si, Y0);
-    double Fa[3][NS];
-    for (int a = 0; a < NS; ++a) {
-        double acc[3] = {0, 0, 0};
-        const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
-                     uaz = v.ua[(2 * NS + a) * n + c];
-        for (int b = 0; b < NS; ++b) {
-            const double D = v.u.nu(binary_diff(a, b, lnT, p_si));
-            const double k = -1.0 * p * Xa[a] * Xa[b] / D;
-            acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
-            acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
-            acc[2] += k * (uaz - v.ua[(2 * NS + b) * n + c]);
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) Fa[d][a] = acc[d] * 1.0 / ra[a];
-    }
     const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
     const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
     const double* Tf = Fd[LBM_FIELD_T];
     for (int a = 0; a < NS; ++a) {
+        // Maxwell-Stefan friction on species a with the binary diffusivities of the state
+        double Fa[3];
+        {
+            double acc[3] = {0, 0, 0};
+            const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
+                         uaz = v.ua[(2 * NS + a) * n + c];
+            for (int b = 0; b < NS; ++b) {
+                const double D = v.u.nu(binary_diff(a, b, lnT, p_si));
+                const double k = -1.0 * p * Xa[a] * Xa[b] / D;
+                acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
+                acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
+                acc[2] += k * (uaz - v.ua[(2 * NS + b) * n + c]);
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) Fa[d] = acc[d] * 1.0 / ra[a];
+        }
         P fl[Q];
         double mom[3] = {0, 0, 0};
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
             fl[l] = spop(v.f, a, l, n, c);
-            if (cx(l)) mom[0] += cx(l) * (double)fl[l];
-            if (cy(l)) mom[1] += cy(l) * (double)fl[l];
-            if (cz(l)) mom[2] += cz(l) * (double)fl[l];
+            const double f = (double)fl[l] + fshift(v, a, l);
+            if (cx(l)) mom[0] += cx(l) * f;
+            if (cy(l)) mom[1] += cy(l) * f;
+            if (cz(l)) mom[2] += cz(l) * f;
         }
         const double uraw[3] = {mom[0] / ra[a], mom[1] / ra[a], mom[2] / ra[a]};
-        const double udu[3] = {uraw[0] + Fa[0][a], uraw[1] + Fa[1][a], uraw[2] + Fa[2][a]};
+        const double udu[3] = {uraw[0] + Fa[0], uraw[1] + Fa[1], uraw[2] + Fa[2]};
         const double Ra = v.u.cp(R_UNIV / W[a]);
         const double theta = Ra * T;
         double dQ[3];
@@ -456,8 +498,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
             const double fdu = feq_dir<double>(Fu, l, ra[a]);
             const double fr = feq_dir<double>(Fr, l, src[a]);
             if (low && fr <= 0.0) continue;  // post-collision value left as it was
-            const double f = fl[l];
-            spop(v.fo, a, l, n, c) = (P)(f + om[a] * (fe - f) + (fdu - fe) + fr);
+            const double f = (double)fl[l] + fshift(v, a, l);
+            stf(v, v.fo, a, l, c, f + om[a] * (fe - f) + (fdu - fe) + fr);
         }
     }
 }
@@ -553,8 +595,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
             const double ge = geq_dir<double>(G, l, rho, Uint - 1.5 * theta);
             const double cdq = 0.5 * (cx(l) * dq[0] + cy(l) * dq[1] + cz(l) * dq[2]);
             const double gs = (c2(l) == 1) ? ge + cdq : ge;
-            const double g = v.g[(int64_t)l * n + c];
-            v.go[(int64_t)l * n + c] = (P)(g + om1 * (ge - g) + (om - om1) * (ge - gs));
+            const double g = ldg(v, v.g, l, c);
+            stg(v, v.go, l, c, g + om1 * (ge - g) + (om - om1) * (ge - gs));
         }
     }
     const unsigned long long m = __ballot(warn);
@@ -563,7 +605,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
 }
 
 // ------------------------------------------------------------- Streaming
-// v.f/v.g: post-collision (source); v.fo/v.go: streamed (destination).
+// v.f/v.g: post-collision (source); v.fo/v.go: streamed (destination).  The
+// rest-state shifts of FP32 storage are the same for opposite directions, so
+// stored values move unchanged.
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
 {
@@ -655,7 +699,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
     double re = 0, qx = 0, qy = 0, qz = 0;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        const double g = v.g[(int64_t)l * n + c];
+        const double g = ldg(v, v.g, l, c);
         re += g;
         if (cx(l)) qx += cx(l) * g;
         if (cy(l)) qy += cy(l) * g;
@@ -669,7 +713,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
         double r = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
-            const double f = spop(v.f, a, l, n, c);
+            const double f = ldf(v, v.f, a, l, c);
             r += f;
             if (cx(l)) mx += cx(l) * f;
             if (cy(l)) my += cy(l) * f;
@@ -921,7 +965,7 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
         double r = 0, m0 = 0, m1 = 0, m2 = 0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out[a]) : (double)spop(v.f, a, q, n, c);
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out[a]) : ldf(v, v.f, a, q, c);
             r += f;
             if (cx(q)) m0 += cx(q) * f;
             if (cy(q)) m1 += cy(q) * f;
@@ -936,7 +980,7 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         re_loc += (unknown >> q & 1u) ? geq_dir<double>(Go, q, rho_out, U_out - 1.5 * th_out)
-                                      : (double)v.g[(int64_t)q * n + c];
+                                      : ldg(v, v.g, q, c);
     for (int a = 0; a < NS; ++a) {
         rho_loc += ra_loc[a];
         mloc[0] += ma[a][0];
@@ -971,8 +1015,8 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
         for (int q = 0; q < Q; ++q) {
             const double fo = feq_dir<double>(Fo, q, ra_out[a]);
             const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
-            P& dst = spop(v.f, a, q, n, c);
-            dst = (unknown >> q & 1u) ? (P)(2 * fo - flc) : (P)(fo + (double)dst - flc);
+            const double fin = ldf(v, v.f, a, q, c);
+            stf(v, v.f, a, q, c, (unknown >> q & 1u) ? 2 * fo - flc : fo + fin - flc);
         }
     }
     const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
@@ -980,9 +1024,43 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     for (int q = 0; q < Q; ++q) {
         const double go = geq_dir<double>(Go, q, rho_out, U_out - 1.5 * th_out);
         const double gl = geq_dir<double>(Gl, q, rho_loc, U_loc - 1.5 * th_loc);
-        P& dst = v.g[(int64_t)q * n + c];
-        dst = (unknown >> q & 1u) ? (P)(2 * go - gl) : (P)(go + (double)dst - gl);
+        const double gin = ldg(v, v.g, q, c);
+        stg(v, v.g, q, c, (unknown >> q & 1u) ? 2 * go - gl : go + gin - gl);
     }
+}
+
+// ------------------------------------------- rest state of FP32 storage
+// Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
+// the mean state the FP32 deviations are taken from.
+__global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, const double* T, const double* p,
+                                                         const double* X, int64_t n, double* part)
+{
+    __shared__ double red[BLOCK];
+    double acc[MC_SUMS] = {};
+    for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < n; c += (int64_t)gridDim.x * BLOCK) {
+        if (type[c] != TYPE_F) continue;
+        acc[0] += 1.0;
+        acc[1] += T[c];
+        acc[2] += p[c];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) acc[3 + a] += X[a * n + c];
+    }
+    for (int q = 0; q < MC_SUMS; ++q) {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int w = BLOCK / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * MC_SUMS + q] = red[0];
+        __syncthreads();
+    }
+}
+
+void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const double* X, int64_t n,
+                          double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mc_state_sums, dim3(MC_SUM_BLOCKS), dim3(BLOCK), 0, s, type, T, p, X, n, part);
 }
 
 // ------------------------------------------------------------- launchers
@@ -1035,6 +1113,11 @@ void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsi
 }
 
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
+template void launch_mc<float>(McKernel, const MultiView<float>&, hipStream_t);
+template void launch_mc_tms<float>(const MultiView<float>&, const int*, const int*, int, hipStream_t);
+template void launch_mc_sur_chemistry<float>(const MultiView<float>&, const sur::Mlp&, unsigned*, unsigned*,
+                                             hipStream_t);
+template void launch_mc_chem_list<float>(const MultiView<float>&, const unsigned*, const unsigned*, hipStream_t);
 template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const sur::Mlp&, unsigned*, unsigned*,
                                               hipStream_t);
 template void launch_mc_chem_list<double>(const MultiView<double>&, const unsigned*, const unsigned*, hipStream_t);

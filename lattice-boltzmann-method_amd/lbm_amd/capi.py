@@ -377,7 +377,12 @@ class MixtureLattice(Lattice):
 
     # -- chemistry surrogate in the step
     def surrogate_capture(self, max_rows, stride=1):
+        """Samples one cell in `stride` per plain chemistry call into a buffer of
+        `max_rows` rows; stride 0 pauses (rows kept), max_rows 0 releases."""
         self._check(self._L.lbm_surrogate_capture(self._h, int(max_rows), int(stride)), "surrogate_capture")
+
+    def surrogate_capture_stop(self):
+        self._check(self._L.lbm_surrogate_capture(self._h, 1, 0), "surrogate_capture")
 
     def surrogate_captured(self, max_rows=None):
         n = ctypes.c_int64()

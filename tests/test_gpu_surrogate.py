@@ -181,7 +181,9 @@ def test_train_step_matches_oracle_adam():
     assert np.max(np.abs(step)) <= lr * (1 + 1e-5)
     big = np.abs(g) > 1e-3 * np.max(np.abs(g))
     assert big.sum() > 0.5 * g.size
-    np.testing.assert_allclose(step[big], -lr * np.sign(g[big]), rtol=1e-4)
+    # first Adam step: m / c1 = g and sqrt(v / c2) = |g|, so w -= lr g / (|g| + eps); the
+    # comparison is limited by the FP32 rounding of w (|w| < 1)
+    np.testing.assert_allclose(step[big], -lr * g[big] / (np.abs(g[big]) + 1e-8), rtol=1e-4, atol=1.2e-7)
     # padded outputs never move; the fragment copies follow the master weights
     W3 = S.unpack(new)[3][0]
     assert np.all(W3[S.NOUT:] == 0)
@@ -322,7 +324,7 @@ def test_trained_surrogate_gain_over_plain_integrator():
     lat.calculate_moment()
     lat.surrogate_capture(1 << 20, 1)
     lat.step(24)
-    lat.surrogate_capture(0, 1)
+    lat.surrogate_capture_stop()
     losses = np.concatenate([lat.surrogate_train(1500, batch=4096, lr=2e-3, seed=1),
                              lat.surrogate_train(1000, batch=4096, lr=3e-4, seed=2)])
     assert losses[-100:].mean() < 0.05 * losses[:10].mean(), (losses[:10].mean(), losses[-100:].mean())
