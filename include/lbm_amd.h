@@ -153,7 +153,7 @@ int64_t lbm_device_bytes(const lbm_ctx* ctx);
  * source of Collide_Species (lbm-collision.cpp:370-399) every step.  These
  * calls let the per-cell MLP (T, p, Y_k, dt) -> (dT, dY_k) stand in for that
  * integration: 12 inputs (zero-padded to 16) -> 128 -> 128 -> 128 -> 10
- * outputs (padded to 16), GELU hidden layers, BF16 weights with FP32
+ * outputs (padded to 16), GELU (tanh form) hidden layers, BF16 weights with FP32
  * accumulation on the matrix cores, Adam on FP32 master weights.
  *
  * Workflow: lbm_surrogate_capture while stepping with the plain integrator
@@ -169,9 +169,12 @@ int64_t lbm_device_bytes(const lbm_ctx* ctx);
  * for each layer L = 0..3, W_L [N_L][K_L] row-major then b_L [N_L], with
  * (K, N) = (16, 128), (128, 128), (128, 128), (128, 16).
  * Normalisation record (lbm_surrogate_get/set_norm), 96 floats: feature mean
- * [16], feature std [16], target mean [16], target std [16], envelope min
- * [16], envelope max [16].  Raw features: T [K], p [Pa], Y_k (9), dt [s];
- * targets: dT [K] and dY_k (9) over one step.
+ * [16], feature scale [16], target mean [16], target scale [16], envelope
+ * min [16], envelope max [16].  Raw features: T [K], p [Pa], Y_k (9), dt [s];
+ * targets: dT [K] (from the exact initial temperature) and dY_k (9) over one
+ * step.  A target scale of 0 marks an output that is not learned (it decodes
+ * to its mean); the predicted sources' net mass is removed in proportion to
+ * the target scales.
  */
 typedef struct {
     int64_t captured;        /* rows in the capture buffer */

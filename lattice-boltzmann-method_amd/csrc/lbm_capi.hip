@@ -89,6 +89,10 @@ struct lbm_ctx {
     unsigned* chem_list = nullptr;
     unsigned* chem_count = nullptr;
     double* tchem = nullptr;
+    uint16_t* sur_x = nullptr;  // in-step network inputs [sur_rows][16] BF16
+    float* sur_y = nullptr;     // outputs [sur_rows][16]
+    uint8_t* sur_flag = nullptr;
+    int64_t sur_rows = 0;
     float* cap_buf = nullptr;
     unsigned long long* cap_count = nullptr;
     long long cap_max = 0;
@@ -339,7 +343,8 @@ void release(lbm_ctx* c)
                     c->fld_block, c->qd_block,  c->tau_count,  c->diag_part, c->diag_out,  c->sf[0],
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
-                    c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part};
+                    c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
+                    c->sur_x,     c->sur_y,     c->sur_flag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -813,7 +818,8 @@ int lbm_collide_species(lbm_ctx* c)
             rc = DISPATCH(c, [&] {
                 const MultiView<P> v = make_mview<P>(c, 0, 1);
                 int r = launch(c, "mc_chemistry_surrogate", [&] {
-                    launch_mc_sur_chemistry<P>(v, c->net, c->chem_list, c->chem_count, c->stream);
+                    launch_mc_sur_chemistry<P>(v, c->net, c->sur_x, c->sur_y, c->sur_flag, c->sur_rows,
+                                               c->chem_list, c->chem_count, c->stream);
                 });
                 if (!r)
                     r = launch(c, "mc_chemistry_fallback",
@@ -1182,10 +1188,16 @@ int lbm_surrogate_enable(lbm_ctx* c, int on, int audit_stride, double audit_tol)
     if (!c->norm_ready) return fail(c, LBM_E_STATE, "train the surrogate (or set its normalisation) first");
     const int64_t n = c->d.n;
     if (!c->chem_list) {
+        c->sur_rows = (n + 255) / 256 * 256;
         HIP_TRY(c, hipMalloc((void**)&c->chem_list, sizeof(unsigned) * n));
         HIP_TRY(c, hipMalloc((void**)&c->chem_count, sizeof(unsigned)));
         HIP_TRY(c, hipMalloc((void**)&c->tchem, sizeof(double) * n));
-        c->bytes += (sizeof(unsigned) + sizeof(double)) * n;
+        HIP_TRY(c, hipMalloc((void**)&c->sur_x, sizeof(uint16_t) * 16 * c->sur_rows));
+        HIP_TRY(c, hipMalloc((void**)&c->sur_y, sizeof(float) * 16 * c->sur_rows));
+        HIP_TRY(c, hipMalloc((void**)&c->sur_flag, n));
+        HIP_TRY(c, hipMemsetAsync(c->sur_x, 0, sizeof(uint16_t) * 16 * c->sur_rows, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->bytes += (sizeof(unsigned) + sizeof(double) + 1) * n + (2 + 4) * 16 * c->sur_rows;
     }
     c->audit_stride = audit_stride;
     c->audit_tol = (float)audit_tol;
@@ -1257,8 +1269,8 @@ int lbm_surrogate_set_norm(lbm_ctx* c, const float* host96)
     if (!c || !host96) return fail(c, LBM_E_ARG, "null argument");
     if (int rc = sur_common(c)) return rc;
     for (int q = 0; q < 16; ++q)
-        if (!(host96[sur::SS_FSTD + q] > 0.f) || !(host96[sur::SS_TSTD + q] > 0.f))
-            return fail(c, LBM_E_ARG, "standard deviations must be > 0");
+        if (!(host96[sur::SS_FSTD + q] > 0.f) || !(host96[sur::SS_TSTD + q] >= 0.f))
+            return fail(c, LBM_E_ARG, "feature scales must be > 0 and target scales >= 0");
     HIP_TRY(c, hipMemcpyAsync(c->sur_stats, host96, sizeof(float) * sur::SS_SIZE, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->norm_ready = true;

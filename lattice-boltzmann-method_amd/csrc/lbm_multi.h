@@ -57,13 +57,14 @@ constexpr double NSCBC_SIGMA_OUT = 0.25;
 template <typename P>
 void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
 
-// Surrogate chemistry of every fluid cell: features, network (BF16 MFMA),
-// decoded sources into v.src / v.hrr / v.tchem; cells outside the training
-// envelope and the audited chunks are appended to `list` (count in *count,
-// zeroed first on the stream).
+// Surrogate chemistry of every fluid cell: features (BF16 [rows][16] in X,
+// envelope flags), network (BF16 MFMA, FP32 outputs [rows][16] in Y; rows >=
+// n, a multiple of 256, rows past n zero), decoded sources into v.src /
+// v.hrr / v.tchem; cells outside the training envelope and the audited
+// chunks are appended to `list` (count in *count, zeroed first on the stream).
 template <typename P>
-void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, unsigned* list, unsigned* count,
-                             hipStream_t s);
+void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, uint16_t* X, float* Y, uint8_t* flag,
+                             int64_t rows, unsigned* list, unsigned* count, hipStream_t s);
 // Plain chemistry of the listed cells: fallbacks are overwritten, audits are
 // compared with the network's result and overwritten above the tolerance.
 template <typename P>

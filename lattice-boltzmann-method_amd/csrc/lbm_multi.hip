@@ -261,7 +261,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chemistry(MultiView<P> v)
             row[11] = v.u.s;
 #pragma unroll
             for (int q = 12; q < 16; ++q) row[q] = 0.f;
-            row[16] = (float)(Tcur - s.Tsi);
+            // dT from the exact temperature of the initial state (Tsi carries
+            // the binary32 rounding of the lattice-unit crossing)
+            row[16] = (float)(Tcur - T_from_UV(s.usi, s.Y0, s.Tsi));
 #pragma unroll
             for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / s.rho);
 #pragma unroll
@@ -270,93 +272,111 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chemistry(MultiView<P> v)
     }
 }
 
-// Surrogate chemistry: 512 threads = 8 wavefronts x 32 cells per pass, the
-// network's weights staged in LDS once per block.
-constexpr int SUR_WAVES = 8;
-
+// Surrogate chemistry in three passes, so that the FP64 feature and decode
+// work runs at full occupancy and the network pass is pure matrix-core work:
+//   k_mc_sur_features: normalised BF16 features [n][16] and envelope flags;
+//   sur::mlp_infer_bf16: the network, outputs [n][16] FP32;
+//   k_mc_sur_decode: sources, HRR and final temperature; cells outside the
+//   envelope and audited 32-cell chunks go to the plain integrator's list.
 template <typename P>
-__global__ __launch_bounds__(512) void k_mc_sur_chemistry(MultiView<P> v, sur::Mlp net, unsigned* list,
-                                                          unsigned* count)
+__global__ __launch_bounds__(BLOCK) void k_mc_sur_features(MultiView<P> v, uint16_t* X, uint8_t* flag)
 {
     using namespace sur;
-    __shared__ __align__(16) uint16_t Wf[FRAG_TOTAL];
-    __shared__ float bias[BIAS_TOTAL];
-    __shared__ __align__(16) uint16_t xin[SUR_WAVES][32 * 16];
-    __shared__ float yout[SUR_WAVES][32 * 16];
-    stage_net(net, Wf, bias);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n) return;
+    const float* st = v.sur_stats;
+    float raw[NIN] = {};
+    const bool mine = v.type[c] == TYPE_F;
+    if (mine) {
+        ChemState s;
+        chem_state(v, c, false, s);
+        raw[0] = (float)s.Tsi;
+        raw[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) raw[2 + a] = (float)s.Y0[a];
+        raw[11] = v.u.s;
+    }
+    bool outside = false;
+    uint32_t packed[8];
+#pragma unroll
+    for (int q = 0; q < 16; q += 2) {
+        float x0 = 0.f, x1 = 0.f;
+        if (mine && q < NIN) {
+            outside |= raw[q] < st[SS_LO + q] || raw[q] > st[SS_HI + q];
+            x0 = (raw[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q];
+        }
+        if (mine && q + 1 < NIN) {
+            outside |= raw[q + 1] < st[SS_LO + q + 1] || raw[q + 1] > st[SS_HI + q + 1];
+            x1 = (raw[q + 1] - st[SS_FMEAN + q + 1]) / st[SS_FSTD + q + 1];
+        }
+        packed[q / 2] = (uint32_t)f2bf(x0) | ((uint32_t)f2bf(x1) << 16);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(X + (int64_t)c * 16);
+    dst[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    dst[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+    flag[c] = outside ? 1 : 0;
+}
+
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_sur_decode(MultiView<P> v, const float* Y, const uint8_t* flag,
+                                                         unsigned* list, unsigned* count)
+{
+    using namespace sur;
+    __shared__ unsigned blk_used, blk_fb;
     const int64_t n = v.d.n;
     const float* st = v.sur_stats;
-    for (int64_t base = (int64_t)blockIdx.x * SUR_WAVES * 32; base < n; base += (int64_t)gridDim.x * SUR_WAVES * 32) {
-        const int64_t chunk0 = base + wave * 32;
-        const int c = (int)(chunk0 + (lane & 31));
-        const bool mine = lane < 32 && c < n && v.type[c] == TYPE_F;
-        bool outside = false;
-        __syncthreads();
-        if (lane < 32) {
-            float raw[NIN] = {};
-            if (mine) {
-                ChemState s;
-                chem_state(v, c, false, s);
-                raw[0] = (float)s.Tsi;
-                raw[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) blk_used = blk_fb = 0u;
+    __syncthreads();
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const bool mine = c < n && v.type[c] == TYPE_F;
+    bool audit = false, fallback = false;
+    if (mine) {
+        ChemState s;
+        chem_state(v, c, false, s);
+        const float4* pr4 = reinterpret_cast<const float4*>(Y + (int64_t)c * 16);
+        const float4 y0 = pr4[0], y1 = pr4[1], y2 = pr4[2];
+        const float pr[12] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w, y2.x, y2.y, y2.z, y2.w};
+        // the imbalance of the predicted sources is removed in proportion to each
+        // species' target scale (an inert species, scale 0, keeps its mean)
+        double src[NS], sum = 0, wsum = 0;
 #pragma unroll
-                for (int a = 0; a < NS; ++a) raw[2 + a] = (float)s.Y0[a];
-                raw[11] = v.u.s;
-            }
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                float x = 0.f;
-                if (q < NIN && mine) {
-                    outside |= raw[q] < st[SS_LO + q] || raw[q] > st[SS_HI + q];
-                    x = (raw[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q];
-                }
-                xin[wave][lane * 16 + q] = f2bf(x);
-            }
+        for (int a = 0; a < NS; ++a) {
+            src[a] = s.rho * ((double)pr[1 + a] * st[SS_TSTD + 1 + a] + st[SS_TMEAN + 1 + a]);
+            sum += src[a];
+            wsum += st[SS_TSTD + 1 + a];
         }
-        __syncthreads();
-        mlp_rows32(Wf, bias, xin[wave], yout[wave]);
-        __syncthreads();
-        bool audit = false, fallback = false;
-        if (mine) {
-            ChemState s;
-            chem_state(v, c, false, s);
-            const float* pr = yout[wave] + lane * 16;
-            double src[NS], sum = 0;
-            bool finite = true;
 #pragma unroll
-            for (int a = 0; a < NS; ++a) {
-                src[a] = s.rho * ((double)pr[1 + a] * st[SS_TSTD + 1 + a] + st[SS_TMEAN + 1 + a]);
-                sum += src[a];
-            }
+        for (int a = 0; a < NS; ++a) src[a] -= st[SS_TSTD + 1 + a] / wsum * sum;  // no net mass
+        const double Tcur = s.Tsi + ((double)pr[0] * st[SS_TSTD] + st[SS_TMEAN]);
+        const bool finite = isfinite(sum) && isfinite(Tcur) && Tcur > 0.0;
+        fallback = flag[c] != 0 || !finite;
+        if (!finite) {
 #pragma unroll
-            for (int a = 0; a < NS; ++a) src[a] -= s.Y0[a] * sum;  // the sources carry no net mass
-            const double Tcur = s.Tsi + ((double)pr[0] * st[SS_TSTD] + st[SS_TMEAN]);
-            finite = isfinite(sum) && isfinite(Tcur) && Tcur > 0.0;
-            fallback = outside || !finite;
-            if (!finite) {
-#pragma unroll
-                for (int a = 0; a < NS; ++a) src[a] = 0.0;
-            }
-            store_chem(v, c, src, finite ? Tcur : s.Tsi);
-            audit = !fallback && v.audit_stride > 0 &&
-                    mix_hash((unsigned)(chunk0 >> 5), v.step) % (unsigned)v.audit_stride == 0;
+            for (int a = 0; a < NS; ++a) src[a] = 0.0;
         }
-        // wavefront-aggregated append of the cells the plain integrator must see
-        const unsigned long long need = __ballot(fallback || audit);
-        const unsigned long long used = __ballot(mine && !fallback);
-        const unsigned long long fb = __ballot(fallback);
-        unsigned slot0 = 0;
-        if (lane == 0) {
-            if (need) slot0 = atomicAdd(count, (unsigned)__popcll(need));
-            if (used) atomicAdd(&v.sur_count[SUR_PREDICTED], (unsigned long long)__popcll(used));
-            if (fb) atomicAdd(&v.sur_count[SUR_FALLBACK], (unsigned long long)__popcll(fb));
-        }
-        slot0 = __shfl(slot0, 0);
-        if (fallback || audit) {
-            const unsigned below = (unsigned)__popcll(need & ((1ull << lane) - 1ull));
-            list[slot0 + below] = (unsigned)c | (audit ? CHEM_AUDIT : 0u);
-        }
+        store_chem(v, c, src, finite ? Tcur : s.Tsi);
+        audit = !fallback && v.audit_stride > 0 && mix_hash((unsigned)(c >> 5), v.step) % (unsigned)v.audit_stride == 0;
+    }
+    // wavefront-aggregated append of the cells the plain integrator must see
+    const unsigned long long need = __ballot(fallback || audit);
+    const unsigned used = (unsigned)__popcll(__ballot(mine && !fallback));
+    const unsigned fb = (unsigned)__popcll(__ballot(fallback));
+    unsigned slot0 = 0;
+    if (lane == 0) {
+        if (need) slot0 = atomicAdd(count, (unsigned)__popcll(need));
+        if (used) atomicAdd(&blk_used, used);
+        if (fb) atomicAdd(&blk_fb, fb);
+    }
+    slot0 = __shfl(slot0, 0);
+    if (fallback || audit) {
+        const unsigned below = (unsigned)__popcll(need & ((1ull << lane) - 1ull));
+        list[slot0 + below] = (unsigned)c | (audit ? CHEM_AUDIT : 0u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one update per block
+        if (blk_used) atomicAdd(&v.sur_count[SUR_PREDICTED], (unsigned long long)blk_used);
+        if (blk_fb) atomicAdd(&v.sur_count[SUR_FALLBACK], (unsigned long long)blk_fb);
     }
 }
 
@@ -367,6 +387,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const un
     const unsigned total = *count;
     const int64_t n = v.d.n;
     const float* st = v.sur_stats;
+    unsigned long long n_aud = 0, n_fail = 0;
+    float emax = 0.f;
     for (unsigned e = blockIdx.x * BLOCK + threadIdx.x; e < total; e += gridDim.x * BLOCK) {
         const unsigned ent = list[e];
         const int c = (int)(ent & ~CHEM_AUDIT);
@@ -378,19 +400,34 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const un
             store_chem(v, c, src, Tp);
             continue;
         }
-        float err = (float)(fabs(Tp - v.tchem[c]) / st[SS_TSTD]);
+        // errors in target standard deviations; the temperature's scale is at
+        // least 1e-6 T (the binary32 rounding of the initial temperature)
+        const double tscale = fmax((double)st[SS_TSTD], 1e-6 * s.Tsi);
+        float err = (float)(fabs(Tp - v.tchem[c]) / tscale);
 #pragma unroll
         for (int a = 0; a < NS; ++a) {
+            if (!(st[SS_TSTD + 1 + a] > 0.f)) continue;  // no spread: not predicted
             const float ea = (float)(fabs(src[a] - v.src[a * n + c]) / (s.rho * st[SS_TSTD + 1 + a]));
             err = ea > err ? ea : err;
         }
         if (!(err == err)) err = INFINITY;
-        atomicAdd(&v.sur_count[SUR_AUDITED], 1ull);
-        atomicMax(&v.sur_count[SUR_MAX_ERR], (unsigned long long)__float_as_uint(err));
+        ++n_aud;
+        emax = err > emax ? err : emax;
         if (err > v.audit_tol) {  // the audit overrules the network for this cell
-            atomicAdd(&v.sur_count[SUR_AUDIT_FAIL], 1ull);
+            ++n_fail;
             store_chem(v, c, src, Tp);
         }
+    }
+    // one update per wavefront
+    for (int o = 32; o > 0; o >>= 1) {
+        n_aud += __shfl_down(n_aud, o, 64);
+        n_fail += __shfl_down(n_fail, o, 64);
+        emax = fmaxf(emax, __shfl_down(emax, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && n_aud) {
+        atomicAdd(&v.sur_count[SUR_AUDITED], n_aud);
+        atomicAdd(&v.sur_count[SUR_AUDIT_FAIL], n_fail);
+        atomicMax(&v.sur_count[SUR_MAX_ERR], (unsigned long long)__float_as_uint(emax));
     }
 }
 
@@ -1097,13 +1134,13 @@ static int cu_count()
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
 template <typename P>
-void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, unsigned* list, unsigned* count,
-                             hipStream_t s)
+void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, uint16_t* X, float* Y, uint8_t* flag,
+                             int64_t rows, unsigned* list, unsigned* count, hipStream_t s)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, count);
-    const int64_t groups = (v.d.n + SUR_WAVES * 32 - 1) / (SUR_WAVES * 32);
-    const int grid = (int)(groups < cu_count() ? groups : cu_count());
-    hipLaunchKernelGGL(k_mc_sur_chemistry<P>, dim3(grid), dim3(512), 0, s, v, net, list, count);
+    hipLaunchKernelGGL(k_mc_sur_features<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, X, flag);
+    sur::mlp_infer_bf16(net, X, rows, Y, s);
+    hipLaunchKernelGGL(k_mc_sur_decode<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, Y, flag, list, count);
 }
 
 template <typename P>
@@ -1115,11 +1152,11 @@ void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsi
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
 template void launch_mc<float>(McKernel, const MultiView<float>&, hipStream_t);
 template void launch_mc_tms<float>(const MultiView<float>&, const int*, const int*, int, hipStream_t);
-template void launch_mc_sur_chemistry<float>(const MultiView<float>&, const sur::Mlp&, unsigned*, unsigned*,
-                                             hipStream_t);
+template void launch_mc_sur_chemistry<float>(const MultiView<float>&, const sur::Mlp&, uint16_t*, float*,
+                                             uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);
 template void launch_mc_chem_list<float>(const MultiView<float>&, const unsigned*, const unsigned*, hipStream_t);
-template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const sur::Mlp&, unsigned*, unsigned*,
-                                              hipStream_t);
+template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const sur::Mlp&, uint16_t*, float*,
+                                              uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);
 template void launch_mc_chem_list<double>(const MultiView<double>&, const unsigned*, const unsigned*, hipStream_t);
 template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
 

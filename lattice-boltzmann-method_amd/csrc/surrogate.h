@@ -54,6 +54,9 @@ void mlp_adam(Mlp& net, Work& w, float lr, int step, hipStream_t s);
 // In-step style inference: Y [M][16] FP32 from normalised X [M][16] FP32 (any
 // M), activations in registers, weights staged in LDS.
 void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s);
+// The same network on BF16 inputs X [rows][16] (rows a multiple of 256) into
+// FP32 outputs Y [rows][16], weights staged in LDS.
+void mlp_infer_bf16(const Mlp& net, const uint16_t* X, int64_t rows, float* Y, hipStream_t s);
 // Column statistics of `rows` capture rows into `stats` [SS_SIZE] (device);
 // `part` holds STATS_BLOCKS * 32 * 4 doubles.
 constexpr int STATS_BLOCKS = 256;

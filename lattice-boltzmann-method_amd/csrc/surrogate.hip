@@ -3,7 +3,7 @@
 // FP32 accumulation, Adam on FP32 master weights.
 //
 // Layers: 16 (12 features, zero-padded) -> 128 -> 128 -> 128 -> 16 (10
-// outputs, zero-padded), GELU on the hidden layers.  Every training GEMM is
+// outputs, zero-padded), GELU (tanh form) on the hidden layers.  Every training GEMM is
 // C[m][n] = sum_k A[m][k] B[n][k] (+ bias) on 16x16 output tiles, one
 // wavefront per tile, v_mfma_f32_16x16x16_bf16 with 4 BF16 of A and of B per
 // lane per K-step of 16 (operand layout in surrogate_device.h).
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(512) void k_mlp_infer(Mlp net, const float* X, int 
     __shared__ __align__(16) uint16_t Wf[FRAG_TOTAL];
     __shared__ float bias[BIAS_TOTAL];
     __shared__ __align__(16) uint16_t xin[IWAVES][32 * 16];
-    __shared__ float yout[IWAVES][32 * 16];
+    __shared__ __align__(16) float yout[IWAVES][32 * 16];
     stage_net(net, Wf, bias);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int base = blockIdx.x * IROWS; base < M; base += gridDim.x * IROWS) {
@@ -227,6 +227,33 @@ int infer_grid(int M)
 void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mlp_infer, dim3(infer_grid(M)), dim3(512), 0, s, net, X, M, Y);
+}
+
+// In-step network pass: inputs already BF16 in HBM, 8 wavefronts x 32 rows
+// per pass of a 512-thread block, grid-stride; rows is a multiple of 256.
+__global__ __launch_bounds__(512) void k_mlp_rows(Mlp net, const uint16_t* X, int64_t rows, float* Y)
+{
+    __shared__ __align__(16) uint16_t Wf[FRAG_TOTAL];
+    __shared__ float bias[BIAS_TOTAL];
+    stage_net(net, Wf, bias);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    for (int64_t base = (int64_t)blockIdx.x * IROWS; base < rows; base += (int64_t)gridDim.x * IROWS) {
+        // keeps the weight fragments' LDS reads inside the loop (hoisted out
+        // of it they would need 576 registers)
+        asm volatile("" ::: "memory");
+        const int64_t r0 = base + wave * 32;
+        mlp_rows32(Wf, bias, X + r0 * 16, Y + r0 * 16);
+    }
+}
+
+void mlp_infer_bf16(const Mlp& net, const uint16_t* X, int64_t rows, float* Y, hipStream_t s)
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t groups = rows / IROWS;
+    const int grid = (int)(groups < 2 * cus ? (groups > 0 ? groups : 1) : 2 * cus);
+    hipLaunchKernelGGL(k_mlp_rows, dim3(grid), dim3(512), 0, s, net, X, rows, Y);
 }
 
 // ---------------------------------------------------- training data
@@ -265,8 +292,12 @@ __global__ __launch_bounds__(256) void k_cap_stats(const float* cap, int64_t row
     }
 }
 
-// Final reduction (fixed order) into the normalisation record.  A column with
-// no spread (dt, inert species) gets std 1.
+// Final reduction (fixed order) into the normalisation record.  Feature
+// scales are at least 1e-3 of the feature's mean, so that rounding-level
+// spread (a composition that barely varies) maps to small network inputs;
+// the envelope is the captured [min, max] widened by 10 % of its width plus
+// 1e-3 of the mean.  A target column with no spread (an inert species) gets
+// scale 0: it is not learned and decodes to its mean.
 __global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, float* st)
 {
     const int q = threadIdx.x;
@@ -282,18 +313,21 @@ __global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, fl
     const double mean = s / (double)rows;
     const double var = s2 / (double)rows - mean * mean;
     double sd = var > 0 ? sqrt(var) : 0.0;
-    if (!(sd > 1e-6 * fabs(mean)) || !(sd > 1e-30)) sd = 1.0;
     if (q < 16) {
         const bool used = q < NIN;
+        sd = fmax(sd, 1e-3 * fabs(mean));
+        if (!(sd > 1e-30)) sd = 1.0;
+        const double margin = 0.1 * (hi - lo) + 1e-3 * fabs(mean);
         st[SS_FMEAN + q] = used ? (float)mean : 0.f;
         st[SS_FSTD + q] = used ? (float)sd : 1.f;
-        st[SS_LO + q] = used ? (float)lo : 0.f;
-        st[SS_HI + q] = used ? (float)hi : 0.f;
+        st[SS_LO + q] = used ? (float)(lo - margin) : 0.f;
+        st[SS_HI + q] = used ? (float)(hi + margin) : 0.f;
     } else {
+        if (!(sd > 1e-6 * fabs(mean)) || !(sd > 1e-30)) sd = 0.0;
         const int t = q - 16;
         const bool used = t < NOUT;
         st[SS_TMEAN + t] = used ? (float)mean : 0.f;
-        st[SS_TSTD + t] = used ? (float)sd : 1.f;
+        st[SS_TSTD + t] = used ? (float)sd : 0.f;
     }
 }
 
@@ -322,7 +356,8 @@ __global__ void k_gather(const float* cap, int64_t rows, const float* st, uint64
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         X[(size_t)b * 16 + q] = f2bf(q < NIN ? (row[q] - st[SS_FMEAN + q]) / st[SS_FSTD + q] : 0.f);
-        T[(size_t)b * 16 + q] = q < NOUT ? (row[16 + q] - st[SS_TMEAN + q]) / st[SS_TSTD + q] : 0.f;
+        T[(size_t)b * 16 + q] =
+            (q < NOUT && st[SS_TSTD + q] > 0.f) ? (row[16 + q] - st[SS_TMEAN + q]) / st[SS_TSTD + q] : 0.f;
     }
 }
 

@@ -26,30 +26,34 @@ namespace sur {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16, round to nearest even (NaN and Inf keep their class).
-__host__ __device__ inline uint16_t f2bf(float x)
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ uint16_t f2bf(float x)
 {
-    uint32_t u;
-    memcpy(&u, &x, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const __bf16 h = (__bf16)x;
+    uint16_t u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
 }
-__host__ __device__ inline float bf2f(uint16_t h)
-{
-    const uint32_t u = (uint32_t)h << 16;
-    float x;
-    memcpy(&x, &u, 4);
-    return x;
-}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// exact GELU, x Phi(x), and its derivative Phi(x) + x phi(x)
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ inline float gelu_grad(float x)
+// GELU in its tanh form, x (1 + tanh(v)) / 2 with v = sqrt(2/pi) (x + 0.044715
+// x^3), written as x / (1 + exp(-2 v)); and its derivative s + x s (1 - s) 2 v'
+// with s = 1 / (1 + exp(-2 v)).
+constexpr float GELU_C = 1.5957691216057308f;  // 2 sqrt(2 / pi)
+constexpr float GELU_A = 0.044715f;
+// (v_exp_f32 and v_rcp_f32: ~1 ulp, far below the BF16 rounding that follows;
+// exp(-u) = 2^(x (K1 + K2 x^2)))
+constexpr float GELU_K1 = -GELU_C * 1.4426950408889634f, GELU_K2 = -GELU_C * GELU_A * 1.4426950408889634f;
+__device__ __forceinline__ float gelu(float x)
 {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-    return cdf + x * pdf;
+    const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(GELU_K2, x * x, GELU_K1));
+    return x * __builtin_amdgcn_rcpf(1.0f + e);
+}
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float u = GELU_C * x * (1.0f + GELU_A * x * x);
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-u));
+    return s + x * s * (1.0f - s) * (GELU_C * (1.0f + 3.0f * GELU_A * x * x));
 }
 
 // Fragment-ordered BF16 weights: for layer L with NT = N/16 output tiles and
@@ -71,7 +75,7 @@ constexpr int bias_offset(int L)
     return off;
 }
 
-__device__ inline v4s load_frag(const uint16_t* p)
+__device__ __forceinline__ v4s load_frag(const uint16_t* p)
 {
     const uint2 w = *reinterpret_cast<const uint2*>(p);
     v4s r;
@@ -82,20 +86,35 @@ __device__ inline v4s load_frag(const uint16_t* p)
     return r;
 }
 
-// BF16(GELU(acc + bias)) for the 4 neurons of one D fragment.
-__device__ inline v4s act_frag(const v4f acc, const float* bias, int n0, int lane)
+// The bias of the 4 neurons of one D fragment (the accumulator's start).
+__device__ __forceinline__ v4f bias_frag(const float* bias, int n0, int lane)
 {
+    const float* b = bias + n0 + 4 * (lane >> 4);
+    return v4f{b[0], b[1], b[2], b[3]};
+}
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// BF16(GELU(z)) for the 4 neurons of one D fragment, converted in pairs.
+__device__ __forceinline__ v4s act_frag(const v4f z)
+{
+    const bf16x2 lo = __builtin_convertvector((f32x2){gelu(z[0]), gelu(z[1])}, bf16x2);
+    const bf16x2 hi = __builtin_convertvector((f32x2){gelu(z[2]), gelu(z[3])}, bf16x2);
+    uint32_t w[2];
+    __builtin_memcpy(&w[0], &lo, 4);
+    __builtin_memcpy(&w[1], &hi, 4);
     v4s h;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) h[q] = (short)f2bf(gelu(acc[q] + bias[n0 + 4 * (lane >> 4) + q]));
+    __builtin_memcpy(&h, w, 8);
     return h;
 }
 
 // Evaluates the network for 32 rows (two batch tiles) held by one wavefront.
-// `Wf` and `bias` are the fragment-ordered weights and the biases (LDS or
-// global), `xin` the BF16 inputs [32][16] of this wavefront and `yout` its
-// FP32 outputs [32][16] (both LDS).
-__device__ inline void mlp_rows32(const uint16_t* Wf, const float* bias, const uint16_t* xin, float* yout)
+// `Wf` and `bias` are the fragment-ordered weights and the biases (LDS), `xin`
+// the BF16 inputs [32][16] of this wavefront and `yout` its FP32 outputs
+// [32][16] (LDS or global; 16-byte aligned).  Each accumulator starts from
+// the layer's bias.
+__device__ __forceinline__ void mlp_rows32(const uint16_t* Wf, const float* bias, const uint16_t* xin, float* yout)
 {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, kq = (lane >> 4) * 4;
@@ -110,12 +129,10 @@ __device__ inline void mlp_rows32(const uint16_t* Wf, const float* bias, const u
 #pragma unroll
         for (int t = 0; t < H; ++t) {
             const v4s a = load_frag(W0 + (t * 64 + lane) * 4);
+            const v4f b = bias_frag(bias + bias_offset(0), 16 * t, lane);
 #pragma unroll
-            for (int bt = 0; bt < 2; ++bt) {
-                v4f acc = {0.f, 0.f, 0.f, 0.f};
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, x[bt], acc, 0, 0, 0);
-                act[bt][t] = act_frag(acc, bias + bias_offset(0), 16 * t, lane);
-            }
+            for (int bt = 0; bt < 2; ++bt)
+                act[bt][t] = act_frag(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, x[bt], b, 0, 0, 0));
         }
     }
     // layers 1 and 2: 128 -> 128
@@ -125,15 +142,15 @@ __device__ inline void mlp_rows32(const uint16_t* Wf, const float* bias, const u
         const float* bL = bias + bias_offset(L);
 #pragma unroll
         for (int t = 0; t < H; ++t) {
-            v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+            v4f acc0 = bias_frag(bL, 16 * t, lane), acc1 = acc0;
 #pragma unroll
             for (int s = 0; s < H; ++s) {
                 const v4s a = load_frag(WL + ((t * H + s) * 64 + lane) * 4);
                 acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[0][s], acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[1][s], acc1, 0, 0, 0);
             }
-            nxt[0][t] = act_frag(acc0, bL, 16 * t, lane);
-            nxt[1][t] = act_frag(acc1, bL, 16 * t, lane);
+            nxt[0][t] = act_frag(acc0);
+            nxt[1][t] = act_frag(acc1);
         }
 #pragma unroll
         for (int t = 0; t < H; ++t) {
@@ -144,23 +161,19 @@ __device__ inline void mlp_rows32(const uint16_t* Wf, const float* bias, const u
     // layer 3: 128 -> 16, no activation
     const uint16_t* W3 = Wf + frag_offset(3);
     const float* b3 = bias + bias_offset(3);
-    v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    v4f acc0 = bias_frag(b3, 0, lane), acc1 = acc0;
 #pragma unroll
     for (int s = 0; s < H; ++s) {
         const v4s a = load_frag(W3 + (s * 64 + lane) * 4);
         acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[0][s], acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[1][s], acc1, 0, 0, 0);
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int nrn = kq + q;
-        yout[r * 16 + nrn] = acc0[q] + b3[nrn];
-        yout[(16 + r) * 16 + nrn] = acc1[q] + b3[nrn];
-    }
+    *reinterpret_cast<v4f*>(yout + r * 16 + kq) = acc0;
+    *reinterpret_cast<v4f*>(yout + (16 + r) * 16 + kq) = acc1;
 }
 
 // Stages the fragment-ordered weights and biases of `net` into LDS.
-__device__ inline void stage_net(const Mlp& net, uint16_t* Wf, float* bias)
+__device__ __forceinline__ void stage_net(const Mlp& net, uint16_t* Wf, float* bias)
 {
     const uint4* src = reinterpret_cast<const uint4*>(net.Wf);
     uint4* dst = reinterpret_cast<uint4*>(Wf);

@@ -283,6 +283,9 @@ class MultiState:
             self.src_chem[:, c] = src
             self.T_chem = np.zeros(self.n)
             self.T_chem[c] = Tcur
+            # exact temperature of the initial state (the surrogate's dT origin)
+            self.T_exact0 = np.zeros(self.n)
+            self.T_exact0[c] = m.T_from_UV(u_si, Y0, Tsi)
             hm = m.h_molar(Tcur)
             hrr = np.zeros(nc)
             for a in range(self.ns):

@@ -8,7 +8,7 @@ whose plain restatement is MultiState.collide_species in oracle/multicomp.py.
 This module restates the network the HIP kernels implement:
 
   * layers 16 -> 128 -> 128 -> 128 -> 16 (12 real inputs, 10 real outputs,
-    zero padding), exact GELU x Phi(x) on the hidden layers;
+    zero padding), GELU (tanh form) on the hidden layers;
   * BF16 (round to nearest even) weights, inputs and hidden activations,
     FP32 pre-activations; `use_bf16=False` evaluates the same network in exact
     float64 arithmetic (used to pin the derivative formulas by finite
@@ -18,8 +18,10 @@ This module restates the network the HIP kernels implement:
     next dX product, as the training kernels do;
   * Adam (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 master weights;
   * Glorot-uniform initial weights from a SplitMix64 stream, zero biases;
-  * the normalisation record (feature / target mean, std, envelope min/max)
-    derived from capture rows, and the decoding of network outputs into the
+  * the normalisation record derived from capture rows: feature and target
+    means and scales (feature scales at least 1e-3 of the mean), and the
+    feature envelope (captured min/max widened by 10 % of the width plus 1e-3
+    of the mean); and the decoding of network outputs into the
     chemistry sources (species mass added over the step, final temperature).
 
 Parameter layout: for each layer W [N][K] row-major, then b [N].
@@ -67,15 +69,18 @@ def bf16(x):
     return (r.astype(np.uint32)).view(np.float32).reshape(x.shape)
 
 
-_erf = np.vectorize(math.erf, otypes=[np.float64])
+GELU_C = 2.0 * math.sqrt(2.0 / math.pi)
+GELU_A = 0.044715
 
 
 def gelu(x):
-    return 0.5 * x * (1.0 + _erf(x / math.sqrt(2.0)))
+    """GELU, tanh form: x (1 + tanh(v)) / 2, v = sqrt(2/pi) (x + 0.044715 x^3)."""
+    return 0.5 * x * (1.0 + np.tanh(0.5 * GELU_C * (x + GELU_A * x ** 3)))
 
 
 def gelu_grad(x):
-    return 0.5 * (1.0 + _erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+    t = np.tanh(0.5 * GELU_C * (x + GELU_A * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * 0.5 * GELU_C * (1.0 + 3.0 * GELU_A * x * x)
 
 
 def _splitmix_stream(seed):
@@ -175,16 +180,21 @@ def norm_record(rows):
     mean = r.mean(axis=0)
     var = (r * r).mean(axis=0) - mean * mean
     sd = np.sqrt(np.maximum(var, 0.0))
-    sd = np.where((sd > 1e-6 * np.abs(mean)) & (sd > 1e-30), sd, 1.0)
+    # features: scale at least 1e-3 of the mean; targets: no spread -> 1
+    sdf = np.maximum(sd[:16], 1e-3 * np.abs(mean[:16]))
+    sdf = np.where(sdf > 1e-30, sdf, 1.0)
+    sdt = np.where((sd[16:] > 1e-6 * np.abs(mean[16:])) & (sd[16:] > 1e-30), sd[16:], 0.0)
+    lo, hi = r[:, :16].min(axis=0), r[:, :16].max(axis=0)
+    margin = 0.1 * (hi - lo) + 1e-3 * np.abs(mean[:16])
     st = np.zeros(SS_SIZE, np.float32)
     used_f = np.arange(16) < NIN
     used_t = np.arange(16) < NOUT
     st[SS_FMEAN:SS_FMEAN + 16] = np.where(used_f, mean[:16], 0.0)
-    st[SS_FSTD:SS_FSTD + 16] = np.where(used_f, sd[:16], 1.0)
+    st[SS_FSTD:SS_FSTD + 16] = np.where(used_f, sdf, 1.0)
     st[SS_TMEAN:SS_TMEAN + 16] = np.where(used_t, mean[16:], 0.0)
-    st[SS_TSTD:SS_TSTD + 16] = np.where(used_t, sd[16:], 1.0)
-    st[SS_LO:SS_LO + 16] = np.where(used_f, r[:, :16].min(axis=0), 0.0)
-    st[SS_HI:SS_HI + 16] = np.where(used_f, r[:, :16].max(axis=0), 0.0)
+    st[SS_TSTD:SS_TSTD + 16] = np.where(used_t, sdt, 0.0)
+    st[SS_LO:SS_LO + 16] = np.where(used_f, lo - margin, 0.0)
+    st[SS_HI:SS_HI + 16] = np.where(used_f, hi + margin, 0.0)
     return st
 
 
@@ -201,17 +211,20 @@ def normalise_features(raw, st):
 def normalise_targets(raw_t, st):
     raw_t = np.asarray(raw_t, np.float32)
     T = np.zeros((raw_t.shape[0], 16), np.float32)
-    T[:, :NOUT] = (raw_t - st[SS_TMEAN:SS_TMEAN + NOUT]) / st[SS_TSTD:SS_TSTD + NOUT]
+    sd = st[SS_TSTD:SS_TSTD + NOUT]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        T[:, :NOUT] = np.where(sd > 0, (raw_t - st[SS_TMEAN:SS_TMEAN + NOUT]) / np.where(sd > 0, sd, 1), 0)
     return T
 
 
 def decode(pred, st, rho, Y0, Tsi):
     """Network outputs [m][16] -> species mass sources [NS][m] (summing to zero
-    by removing the net mass in proportion to Y) and the final temperature."""
+    by removing the net mass in proportion to the target scales; a scale-0
+    species keeps its mean) and the final temperature."""
     pred = np.asarray(pred, np.float64)
     tm = st[SS_TMEAN:SS_TMEAN + NOUT].astype(np.float64)
     ts = st[SS_TSTD:SS_TSTD + NOUT].astype(np.float64)
     src = rho[None, :] * (pred[:, 1:NOUT].T * ts[1:, None] + tm[1:, None])
-    src = src - Y0.T * src.sum(axis=0)[None, :]
+    src = src - (ts[1:] / ts[1:].sum())[:, None] * src.sum(axis=0)[None, :]
     T = Tsi + (pred[:, 0] * ts[0] + tm[0])
     return src, T

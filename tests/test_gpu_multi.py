@@ -231,7 +231,7 @@ def test_mc_fp32_shifted_tracks_fp64(case):
     a = _run_precision(fl, LBM_FP64, 10)
     b = _run_precision(fl, LBM_FP32_SHIFTED, 10)
     F = (fl["type"] == 0).ravel()
-    for k, tol in (("rho", 1e-6), ("T", 1e-6), ("p", 1e-6), ("u", 1e-3)):
+    for k, tol in (("rho", 1e-5), ("T", 1e-5), ("p", 1e-5), ("u", 1e-3)):
         e = relerr(b.get_field(k).ravel()[F], a.get_field(k).ravel()[F])
         assert e < tol, (k, e)
     ya = a.get_species_field("rho").reshape(9, -1)[:, F]

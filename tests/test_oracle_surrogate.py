@@ -112,14 +112,18 @@ def test_norm_record_statistics():
     st = S.norm_record(rows)
     assert st[S.SS_FMEAN] == pytest.approx(rows[:, 0].astype(np.float64).mean(), rel=1e-6)
     assert st[S.SS_FSTD] == pytest.approx(rows[:, 0].astype(np.float64).std(), rel=1e-5)
-    assert st[S.SS_FSTD + 11] == 1.0 and st[S.SS_FMEAN + 11] == np.float32(1e-9)
+    # constant column: scale 1e-3 of its value, envelope within 1e-3 of it
+    assert st[S.SS_FSTD + 11] == np.float32(1e-12) and st[S.SS_FMEAN + 11] == np.float32(1e-9)
+    assert st[S.SS_LO + 11] == np.float32(1e-9 - 1e-12) and st[S.SS_HI + 11] == np.float32(1e-9 + 1e-12)
     assert st[S.SS_FSTD + 1] == 1.0     # all-zero column
-    assert st[S.SS_LO] == rows[:, 0].min() and st[S.SS_HI] == rows[:, 0].max()
+    lo, hi = rows[:, 0].astype(np.float64).min(), rows[:, 0].astype(np.float64).max()
+    m = 0.1 * (hi - lo) + 1e-3 * rows[:, 0].astype(np.float64).mean()
+    assert st[S.SS_LO] == np.float32(lo - m) and st[S.SS_HI] == np.float32(hi + m)
     assert st[S.SS_TSTD] == pytest.approx(rows[:, 16].astype(np.float64).std(), rel=1e-5)
-    assert np.all(st[S.SS_FSTD + 12:S.SS_FSTD + 16] == 1) and np.all(st[S.SS_TSTD + 10:S.SS_TSTD + 16] == 1)
+    assert np.all(st[S.SS_FSTD + 12:S.SS_FSTD + 16] == 1) and np.all(st[S.SS_TSTD + 1:S.SS_TSTD + 16] == 0)
     X, outside = S.normalise_features(rows[:, :12], st)
     assert not outside.any()
-    _, outside = S.normalise_features(rows[:3, :12] * 1.5, st)
+    _, outside = S.normalise_features(rows[:3, :12] * 2.5, st)
     assert outside.all()
 
 
@@ -133,5 +137,11 @@ def test_decode_conserves_mass():
     Y0 = rng.dirichlet(np.ones(9), m)
     rho = rng.uniform(0.5, 1.5, m)
     src, T = S.decode(pred, st, rho, Y0, np.full(m, 1000.0))
+    assert np.max(np.abs(src.sum(axis=0))) < 1e-15
+    # a species with scale 0 keeps its mean exactly
+    st[S.SS_TSTD + 9] = 0.0
+    st[S.SS_TMEAN + 9] = 2e-3
+    src, _ = S.decode(pred, st, rho, Y0, np.full(m, 1000.0))
+    np.testing.assert_allclose(src[8], rho * np.float32(2e-3), rtol=1e-15)
     assert np.max(np.abs(src.sum(axis=0))) < 1e-15
     np.testing.assert_allclose(T, 1000.0 + pred[:, 0].astype(np.float64) * np.float32(1e-3) + 5.0, rtol=1e-15)
