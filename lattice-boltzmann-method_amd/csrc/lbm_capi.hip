@@ -71,6 +71,9 @@ struct lbm_ctx {
     int* tms_cell = nullptr;
     int* tms_dir = nullptr;
     int tms_count = 0;
+    int* tin_cell = nullptr;  // TMS_BC inflow interface cells (TYPE_I_C)
+    int* tin_dir = nullptr;
+    int tin_count = 0;
     std::string tms_unsupported;
     double* src = nullptr;  // [NS][n] chemistry mass sources
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
@@ -344,7 +347,7 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag};
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -528,7 +531,7 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                 if (keep) needs_tms = true;
                 flag[cidx] = (simple ? NB_SIMPLE : 0) | (keep ? NB_KEEP : 0);
             }
-    std::vector<int> tcell, tdir;
+    std::vector<int> tcell, tdir, icell, idir;
     std::string tms_unsupported;
     if (multi(c)) {
         for (int64_t i = 0; i < d.n; ++i)
@@ -544,9 +547,21 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                     if (at(i, j, k) != TYPE_F) continue;
                     for (int l = 0; l < Q; ++l) {
                         const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
-                        if (tn == TYPE_I || tn == TYPE_I_C || tn == TYPE_S)
-                            tms_unsupported = "inflow (TYPE_I, TYPE_I_C) and isothermal-wall (TYPE_S) TMS "
+                        if (tn == TYPE_I || tn == TYPE_S)
+                            tms_unsupported = "fixed-state inflow (TYPE_I) and isothermal-wall (TYPE_S) TMS "
                                               "reconstruction is not provided for the multicomponent state";
+                    }
+                    // the inflow sweep's interface detection (lbm-bc.cpp:479-494)
+                    for (int l = 0; l < Q; ++l) {
+                        const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
+                        const uint8_t tp = at(i + cx(l), j + cy(l), k + cz(l));
+                        if ((tn == TYPE_I || tn == TYPE_I_C) && tp == TYPE_F) {
+                            if (c2(l) == 1 && tn == TYPE_I_C) {
+                                icell.push_back((int)(((int64_t)i * d.ny + j) * d.nz + k));
+                                idir.push_back(l);
+                            }
+                            break;
+                        }
                     }
                     for (int l = 0; l < Q; ++l) {
                         const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
@@ -581,6 +596,16 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
             HIP_TRY(c, hipMalloc((void**)&c->tms_dir, sizeof(int) * tdir.size()));
             HIP_TRY(c, hipMemcpy(c->tms_cell, tcell.data(), sizeof(int) * tcell.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMemcpy(c->tms_dir, tdir.data(), sizeof(int) * tdir.size(), hipMemcpyHostToDevice));
+        }
+        if (c->tin_cell) (void)hipFree(c->tin_cell);
+        if (c->tin_dir) (void)hipFree(c->tin_dir);
+        c->tin_cell = c->tin_dir = nullptr;
+        c->tin_count = (int)icell.size();
+        if (c->tin_count) {
+            HIP_TRY(c, hipMalloc((void**)&c->tin_cell, sizeof(int) * icell.size()));
+            HIP_TRY(c, hipMalloc((void**)&c->tin_dir, sizeof(int) * idir.size()));
+            HIP_TRY(c, hipMemcpy(c->tin_cell, icell.data(), sizeof(int) * icell.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->tin_dir, idir.data(), sizeof(int) * idir.size(), hipMemcpyHostToDevice));
         }
         c->tms_unsupported = tms_unsupported;
         needs_tms = false;
@@ -873,8 +898,13 @@ int lbm_tms_bc(lbm_ctx* c)
     if (multi(c)) {
         if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
         if (!c->tms_unsupported.empty()) return fail(c, LBM_E_UNSUPPORTED, c->tms_unsupported);
-        return DISPATCH(c, launch(c, "mc_tms_outflow", [&] {
-            launch_mc_tms<P>(make_mview<P>(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
+        // outflow sweep, then inflow sweep (lbm-bc.cpp:126, 463)
+        if (int rc = DISPATCH(c, launch(c, "mc_tms_outflow", [&] {
+                launch_mc_tms<P>(make_mview<P>(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
+            })))
+            return rc;
+        return DISPATCH(c, launch(c, "mc_tms_inflow", [&] {
+            launch_mc_tms_in<P>(make_mview<P>(c, 0, 0), c->tin_cell, c->tin_dir, c->tin_count, c->stream);
         }));
     }
     if (c->needs_tms)

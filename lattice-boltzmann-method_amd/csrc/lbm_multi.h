@@ -57,6 +57,11 @@ constexpr double NSCBC_SIGMA_OUT = 0.25;
 template <typename P>
 void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
 
+// TMS_BC inflow reconstruction for TYPE_I_C ghosts (same list layout; runs
+// after the outflow sweep).
+template <typename P>
+void launch_mc_tms_in(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
+
 // Surrogate chemistry of every fluid cell: features (BF16 [rows][16] in X,
 // envelope flags), network (BF16 MFMA, FP32 outputs [rows][16] in Y; rows >=
 // n, a multiple of 256, rows past n zero), decoded sources into v.src /

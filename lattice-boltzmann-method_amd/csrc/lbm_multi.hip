@@ -1066,6 +1066,177 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     }
 }
 
+// ------------------------------------------------- TMS_BC inflow sweep
+// Inflow sweep of TMS_BC (lbm-bc.cpp:463-938) for TYPE_I_C ghosts: the
+// inflow target from the inline LODI relaxation with sigma = 5 towards the
+// ghost's state (lbm-bc.cpp:519-677), the density from setState_TP at the
+// target temperature and pressure (lbm-bc.cpp:672-680); then unknown
+// directions f_tgt + f_in - f_loc and known ones f_tgt + f - f_loc, with
+// f_in the equilibrium of the inflow state, f_loc that of the local moments
+// (inflow populations substituted) and f_tgt that of the local density and
+// composition at the inflow velocity and temperature.
+template <typename P>
+__global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cells, const int* dirs, int count)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t], l = dirs[t];
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    const int strides[3] = {v.d.ny * v.d.nz, v.d.nz, 1};
+    const int dims[3] = {v.d.nx, v.d.ny, v.d.nz};
+    const int cl[3] = {cx(l), cy(l), cz(l)};
+    const int idir = cl[0] ? 0 : (cl[1] ? 1 : 2);
+    const int isign = cl[idir];
+    const int st = strides[idir];
+    const int c1 = c + isign * st, cb = c - isign * st;
+    double* const* Fd = v.fld;
+    const double sigma = 5.0;
+    // ---- inline LODI target
+    const double rho = Fd[LBM_FIELD_RHO][c], p = Fd[LBM_FIELD_P][c];
+    const double vel[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
+    const double velb[3] = {Fd[LBM_FIELD_U][cb], Fd[LBM_FIELD_V][cb], Fd[LBM_FIELD_W][cb]};
+    double Yc[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yc[a] = v.rhoa[a * n + c] / rho;
+    double a_s, gam;
+    double Yn[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yn[a] = Yc[a];
+    sound_DP(v.u, rho, p, Yn, &a_s, &gam);  // normalises Yn
+    // fd_uw (FD.cpp:64-74) with the interface direction as the velocity
+    auto fd_uw = [&](double s0, double s1) { return isign < 0 ? (s0 - s1) / 1 : (-s0 + s1) / 1; };
+    const double du = fd_uw(Fd[LBM_FIELD_U + idir][c], Fd[LBM_FIELD_U + idir][c1]);
+    const double dp = fd_uw(p, Fd[LBM_FIELD_P][c1]);
+    const double rho1 = Fd[LBM_FIELD_RHO][c1];
+    const double un = vel[idir];
+    // tangential components as paired per normal: x -> (v, w), y -> (w, u), z -> (u, v)
+    const int t3 = idir == 0 ? 1 : (idir == 1 ? 2 : 0);
+    const int t4 = idir == 0 ? 2 : (idir == 1 ? 0 : 1);
+    const double Nd = dims[idir];  // ghost-inclusive length
+    const double Rmix = v.u.cp(R_UNIV / mean_W(Yn));
+    double L1 = (un - a_s) * (dp - rho * a_s * du);
+    const double L2 = sigma * a_s / Nd * rho * Rmix * (Fd[LBM_FIELD_T][c] - Fd[LBM_FIELD_T][cb]);
+    const double L3 = sigma * a_s / Nd * (vel[t3] - velb[t3]);
+    const double L4 = sigma * a_s / Nd * (vel[t4] - velb[t4]);
+    double L5 = (un + a_s) * (dp + rho * a_s * du);
+    const double K = sigma * a_s * a_s / Nd * rho * (1.0 - (un / a_s) * (un / a_s));
+    // the z-normal branch tests cy (lbm-bc.cpp:652), so it always takes the second form
+    const bool left = idir == 2 ? (cy(l) > 0) : (isign > 0);
+    if (left) L5 = K * (un - velb[idir]);
+    else L1 = K * (un - velb[idir]);
+    const double d1 = 1.0 / (a_s * a_s) * (L2 + 0.5 * (L5 + L1));
+    const double d2 = 1.0 / (2.0 * rho * a_s) * (L5 - L1);
+    const double d5 = 0.5 * (L5 + L1);
+    double rho_in = rho - 1 * d1;
+    double vin[3] = {vel[0], vel[1], vel[2]};
+    vin[idir] = vel[idir] - 1 * d2;
+    vin[t3] = vel[t3] - 1 * L3;
+    vin[t4] = vel[t4] - 1 * L4;
+    const double p_in = p - 1 * d5;
+    double ra_in[NS], Yi[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        const double dY = fd_uw(Yc[a], v.rhoa[a * n + c1] / rho1);
+        ra_in[a] = (Yc[a] - 1 * un * dY) * rho_in;
+        Yi[a] = ra_in[a] / rho_in;
+    }
+    double Tsi_in;
+    const double T_in = T_from_DP(v.u, rho_in, p_in, Yi, &Tsi_in);  // normalises Yi
+    // setState_TP(T_in, p_in): the inflow density follows
+    const double Tsi = v.u.si_temp(T_in);
+    rho_in = v.u.rho(v.u.si_p(p_in) * mean_W(Yi) / (R_UNIV * Tsi));
+#pragma unroll
+    for (int a = 0; a < NS; ++a) ra_in[a] = Yi[a] * rho_in;
+    // ---- equilibria of the inflow state (calculate_feq_geq, lbm-edf.cpp:198-209)
+    const double U_in = v.u.energy_mass(u_mass(Tsi, Yi));
+    const double th_in = v.u.energy_mass(R_UNIV * Tsi / mean_W(Yi));
+    uint32_t unknown = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint8_t tq = v.type[cell(v.d, x.i - cx(q), x.j - cy(q), x.k - cz(q))];
+        if (tq == TYPE_I || tq == TYPE_I_C) unknown |= 1u << q;
+    }
+    const double zero[3] = {0, 0, 0};
+    // ---- local moments with the inflow populations in the unknown directions
+    double ra_loc[NS], ma[NS][3], re_loc = 0;
+    double rho_loc = 0, mloc[3] = {0, 0, 0};
+    for (int a = 0; a < NS; ++a) {
+        const double th_a = v.u.energy_mass(R_UNIV * Tsi / W[a]);
+        Axis3<double> Fi[3];
+        feq_axes<double>(Fi, vin, th_a, zero);
+        double r = 0, m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fi, q, ra_in[a]) : ldf(v, v.f, a, q, c);
+            r += f;
+            if (cx(q)) m0 += cx(q) * f;
+            if (cy(q)) m1 += cy(q) * f;
+            if (cz(q)) m2 += cz(q) * f;
+        }
+        ra_loc[a] = r;
+        ma[a][0] = m0;
+        ma[a][1] = m1;
+        ma[a][2] = m2;
+    }
+    const GeqAxes<double> Gi = geq_axes<double>(th_in, vin);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        re_loc += (unknown >> q & 1u) ? geq_dir<double>(Gi, q, rho_in, U_in - 1.5 * th_in) : ldg(v, v.g, q, c);
+    for (int a = 0; a < NS; ++a) {
+        rho_loc += ra_loc[a];
+        mloc[0] += ma[a][0];
+        mloc[1] += ma[a][1];
+        mloc[2] += ma[a][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ma[a][d] = ra_loc[a] > 0.0 ? ma[a][d] / ra_loc[a] : 0.0;
+    }
+    double vloc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vloc[d] = rho_loc > 0 ? mloc[d] / rho_loc : 0.0;
+    double Uloc = re_loc / rho_loc;
+    for (int a = 0; a < NS; ++a)
+        Uloc = Uloc - 0.5 * ra_loc[a] / rho_loc * (ma[a][0] * ma[a][0] + ma[a][1] * ma[a][1] + ma[a][2] * ma[a][2]);
+    // calculate_temp (lbm-moment.cpp:540-553)
+    double Yl[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yl[a] = ra_loc[a] / rho_loc;
+    normalize(Yl);
+    const double T_loc = v.u.temp(T_from_UV(v.u.si_energy_mass(Uloc), Yl, v.u.si_temp(Fd[LBM_FIELD_T][c])));
+    const double Tsl = v.u.si_temp(T_loc);
+    const double U_loc = v.u.energy_mass(u_mass(Tsl, Yl));
+    const double th_loc = v.u.energy_mass(R_UNIV * Tsl / mean_W(Yl));
+    // target: local density and composition at the inflow velocity and temperature
+    const double U_tg = v.u.energy_mass(u_mass(Tsi, Yl));
+    const double th_tg = v.u.energy_mass(R_UNIV * Tsi / mean_W(Yl));
+    // ---- reconstruction
+    for (int a = 0; a < NS; ++a) {
+        const double thi = v.u.energy_mass(R_UNIV * Tsi / W[a]);
+        const double thl = v.u.energy_mass(R_UNIV * Tsl / W[a]);
+        Axis3<double> Fi[3], Fl[3];
+        feq_axes<double>(Fi, vin, thi, zero);
+        feq_axes<double>(Fl, ma[a], thl, zero);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double fi = feq_dir<double>(Fi, q, ra_in[a]);
+            const double ft = feq_dir<double>(Fi, q, ra_loc[a]);
+            const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
+            const double fin = ldf(v, v.f, a, q, c);
+            stf(v, v.f, a, q, c, (unknown >> q & 1u) ? ft + fi - flc : ft + fin - flc);
+        }
+    }
+    const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
+    const GeqAxes<double> Gt = geq_axes<double>(th_tg, vin);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double gi = geq_dir<double>(Gi, q, rho_in, U_in - 1.5 * th_in);
+        const double gt = geq_dir<double>(Gt, q, rho_loc, U_tg - 1.5 * th_tg);
+        const double gl = geq_dir<double>(Gl, q, rho_loc, U_loc - 1.5 * th_loc);
+        const double gin = ldg(v, v.g, q, c);
+        stg(v, v.g, q, c, (unknown >> q & 1u) ? gt + gi - gl : gt + gin - gl);
+    }
+}
+
 // ------------------------------------------- rest state of FP32 storage
 // Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
 // the mean state the FP32 deviations are taken from.
@@ -1124,6 +1295,12 @@ void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int
     if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
 }
 
+template <typename P>
+void launch_mc_tms_in(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s)
+{
+    if (count > 0) hipLaunchKernelGGL(k_mc_tms_in<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+}
+
 static int cu_count()
 {
     int dev = 0, cus = 256;
@@ -1159,5 +1336,7 @@ template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const su
                                               uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);
 template void launch_mc_chem_list<double>(const MultiView<double>&, const unsigned*, const unsigned*, hipStream_t);
 template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
+template void launch_mc_tms_in<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
+template void launch_mc_tms_in<float>(const MultiView<float>&, const int*, const int*, int, hipStream_t);
 
 }  // namespace lbm

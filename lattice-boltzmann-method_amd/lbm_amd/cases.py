@@ -197,3 +197,69 @@ def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, no
     for k in ("u", "v", "w"):
         fl[k][m] += noise * U_s * rng.uniform(-1.0, 1.0, int(m.sum()))
     return fl
+
+
+# phi = 1 burnt state (setup.cpp:2106-2116)
+T_BURNT_PHI1 = 2387.637
+X_BURNT_PHI1 = {"H2": 0.014565, "H": 0.00181, "O": 0.00060738, "O2": 0.0056094, "OH": 0.0072848, "H2O": 0.32437,
+                "HO2": 1.2571e-06, "H2O2": 1.3391e-07, "N2": 0.64575}
+S_L_PHI1 = 2.2  # m/s, laminar burning velocity of the phi = 1 H2-air flame used as the inflow speed
+
+
+def _molar_masses():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mech", "h2air.json")
+    with open(path) as f:
+        d = json.load(f)
+    return np.array([s["W"] for s in d["species"]])
+
+
+def smooth(left, right, x, center, alpha):
+    """math-util.cpp:89-91: left + (1 + tanh(alpha (x - center))) / 2 (right - left)."""
+    return left + 0.5 * (1.0 + np.tanh(alpha * (x - center))) * (right - left)
+
+
+def premixed_flame_1d(nx, ny=4, nz=4, u_in=None, noise=0.01, seed=20261204, midpoint=0.7, alpha=0.2, minim=1e-10):
+    """Quasi-1-D premixed H2-air flame (the FLAME_SPEED set-up of
+    setup.cpp:2014-2124 at phi = 1): periodic y/z; TYPE_I_C inflow ghost at
+    i = 0 holding the unburnt state (300 K, 1 atm, u_in), TYPE_O_C outflow
+    ghost at i = Nx-1 holding 1 atm; a tanh front between the unburnt and the
+    burnt phi = 1 states at `midpoint` Nx (smooth(), alpha = 0.2); seeded
+    +-`noise` relative white noise on the mass fractions, renormalised."""
+    rng = np.random.default_rng(seed)
+    fl = default_fields(nx, ny, nz)
+    t = fl["type"]
+    t[:, 0, :] = TYPE_P
+    t[:, -1, :] = TYPE_P
+    t[:, :, 0] = TYPE_P
+    t[:, :, -1] = TYPE_P
+    t[0, :, :] = TYPE_I_C
+    t[-1, :, :] = TYPE_O_C
+    shape = t.shape
+    Nx = shape[0]
+    if u_in is None:
+        u_in = S_L_PHI1 * UNITS_FLAME[1] / UNITS_FLAME[0]
+    i = np.arange(Nx, dtype=np.float64)
+    Xu, Xb = mole_vector(X_UNBURNT_PHI1, minim), mole_vector(X_BURNT_PHI1, minim)
+    X = smooth(Xu[:, None], Xb[:, None], i[None, :], midpoint * Nx, alpha)  # (NS, Nx)
+    T = smooth(300.0, T_BURNT_PHI1, i, midpoint * Nx, alpha)
+    X3 = np.broadcast_to(X[:, :, None, None], (len(H2AIR_SPECIES),) + shape).copy()
+    # noise on the mass fractions of the fluid cells
+    W = _molar_masses()
+    m = t == TYPE_F
+    Y = X3 * W[:, None, None, None]
+    Y /= Y.sum(axis=0, keepdims=True)
+    Y[:, m] *= 1.0 + noise * rng.uniform(-1.0, 1.0, (len(H2AIR_SPECIES), int(m.sum())))
+    Y /= Y.sum(axis=0, keepdims=True)
+    Xn = Y / W[:, None, None, None]
+    fl["X"] = Xn / Xn.sum(axis=0, keepdims=True)
+    fl["T"][...] = np.broadcast_to(T[:, None, None], shape) / LATTICE_K
+    # the inflow ghosts hold the fresh mixture exactly
+    fl["X"][:, 0] = Xu[:, None, None]
+    fl["T"][0] = 300.0 / LATTICE_K
+    fl["p"][...] = lattice_p(ONE_ATM)
+    fl["u"][...] = u_in
+    fl["v"][...] = 0.0
+    fl["w"][...] = 0.0
+    return fl

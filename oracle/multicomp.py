@@ -511,7 +511,8 @@ def _onlyone(a, b, c):
 class MultiStateBC(MultiState):
     """MultiState plus the multicomponent TMS_BC outflow branch with the NSCBC
     target state (lbm-bc.cpp:126-340, impose_nscbc.cpp:45-546) for TYPE_O_C
-    ghosts."""
+    ghosts and the inflow branch with the inline LODI target state
+    (lbm-bc.cpp:463-938) for TYPE_I_C ghosts."""
 
     SIGMA_OUT = 0.25  # NSCBC_SIGMA_OUT, defines.hpp:100-102
 
@@ -646,20 +647,82 @@ class MultiStateBC(MultiState):
         T_si = m.T_from_UV(np.float64(U_.si_energy_mass(U)), Y[None], np.array([T0_si]))[0]
         return np.float64(U_.temp(T_si))
 
-    def interface_cells(self):
-        """(cell, l_interface) of every fluid cell whose first outflow-ghost
-        direction is axis-aligned (lbm-bc.cpp:141-155)."""
+    def interface_cells(self, kinds=(TYPE_O, TYPE_O_C)):
+        """(cell, l_interface) of every fluid cell whose first ghost direction
+        of the given kinds (outflow by default, lbm-bc.cpp:141-155; inflow
+        with (TYPE_I, TYPE_I_C), lbm-bc.cpp:479-494) is axis-aligned."""
         out = []
         for c in np.nonzero(self.type == TYPE_F)[0]:
             i, j, k = self._ijk(c)
             for l in range(27):
                 tn = self.type[self._idx(i - CX[l], j - CY[l], k - CZ[l])]
                 tp = self.type[self._idx(i + CX[l], j + CY[l], k + CZ[l])]
-                if tn in (TYPE_O, TYPE_O_C) and tp == TYPE_F:
+                if tn in kinds and tp == TYPE_F:
                     if _onlyone(CX[l], CY[l], CZ[l]):
                         out.append((int(c), l))
                     break
         return out
+
+    def _inflow_lodi(self, c, l):
+        """TYPE_I_C inflow target (lbm-bc.cpp:519-677): characteristic
+        relaxation with sigma = 5 towards the ghost's state, first-order
+        one-sided normal derivatives; returns rho_in, rhoa_in, vel_in, T_in
+        after the setState_TP density update (lbm-bc.cpp:672-680)."""
+        U_, m = self.units, self.mech
+        sigma = 5.0
+        cvec = (CX[l], CY[l], CZ[l])
+        idir = int(np.nonzero(cvec)[0][0])
+        isign = int(cvec[idir])
+        st = self._strides()[idir]
+        c1, c2 = c + isign * st, c + 2 * isign * st
+        cb = c - isign * st
+        Nd = self.dims[idir]  # ghost-inclusive length (Q7)
+        a, gam, Yn, T_si = self._sound(c)
+        rho, p = self.rho[c], self.p[c]
+        vel = np.array([self.u[c], self.v[c], self.w[c]])
+        velb = np.array([self.u[cb], self.v[cb], self.w[cb]])
+
+        def fd_uw(s0, s1):  # FD.cpp:64-74 with vel = the interface direction
+            return (s0 - s1) / 1 if isign < 0 else (-s0 + s1) / 1
+
+        comp = [self.u, self.v, self.w][idir]
+        du = fd_uw(comp[c], comp[c1])
+        dp = fd_uw(self.p[c], self.p[c1])
+        dY = np.array([fd_uw(self.rho_a[b, c] / self.rho[c], self.rho_a[b, c1] / self.rho[c1]) for b in range(self.ns)])
+        un = vel[idir]
+        # tangential components as the reference pairs them: x -> (v, w), y -> (w, u), z -> (u, v)
+        t3, t4 = {0: (1, 2), 1: (2, 0), 2: (0, 1)}[idir]
+        Rmix = np.float64(U_.cp(m.R / m.W_mean(Yn)))
+        L1 = (un - a) * (dp - rho * a * du)
+        L2 = sigma * a / Nd * rho * Rmix * (self.T[c] - self.T[cb])
+        L3 = sigma * a / Nd * (vel[t3] - velb[t3])
+        L4 = sigma * a / Nd * (vel[t4] - velb[t4])
+        L5 = (un + a) * (dp + rho * a * du)
+        K = sigma * a * a / Nd * rho * (1.0 - (un / a) * (un / a))
+        # the z branch tests cy (lbm-bc.cpp:652, Q4): always the "right" form there
+        left = (CY[l] > 0) if idir == 2 else (isign > 0)
+        if left:
+            L5 = K * (un - velb[idir])
+        else:
+            L1 = K * (un - velb[idir])
+        d1 = 1.0 / (a * a) * (L2 + 0.5 * (L5 + L1))
+        d2 = 1.0 / (2.0 * rho * a) * (L5 - L1)
+        d5 = 0.5 * (L5 + L1)
+        rho_in = rho - 1 * d1
+        vel_in = vel.copy()
+        vel_in[idir] = vel[idir] - 1 * d2
+        vel_in[t3] = vel[t3] - 1 * L3
+        vel_in[t4] = vel[t4] - 1 * L4
+        p_in = p - 1 * d5
+        rhoa_in = (self.rho_a[:, c] / rho - 1 * un * dY) * rho_in
+        T_si_in, _ = self._state_DP(rho_in, p_in, rhoa_in / rho_in)
+        T_in = np.float64(U_.temp(T_si_in))
+        # setState_TP(T_in, p_in) at Y = rhoa_in / rho_in: the density follows
+        Y = m.normalize_Y(rhoa_in / rho_in)
+        T_si = np.float64(U_.si_temp(T_in))
+        p_si = np.float64(U_.si_p(p_in))
+        rho_in = np.float64(U_.rho(p_si * m.W_mean(Y) / (m.R * T_si)))
+        return rho_in, Y * rho_in, vel_in, T_in
 
     def tms_bc(self):
         """TMS_BC outflow sweep (lbm-bc.cpp:126-340) for TYPE_O_C ghosts."""
@@ -686,6 +749,45 @@ class MultiStateBC(MultiState):
             fa_loc, g_loc = self._feq_geq(rho_loc, rhoa_loc, vel_loc, vela_loc, T_loc)
             fnew = np.where(unknown[None, :], 2 * fa_out - fa_loc, fa_out + self.f[:, :, c] - fa_loc)
             gnew = np.where(unknown, 2 * g_out - g_loc, g_out + self.g[:, c] - g_loc)
+            updates.append((c, fnew, gnew))
+        for c, fnew, gnew in updates:
+            self.f[:, :, c] = fnew
+            self.g[:, c] = gnew
+        self._tms_inflow()
+
+    def _tms_inflow(self):
+        """TMS_BC inflow sweep (lbm-bc.cpp:463-938) for TYPE_I_C ghosts: the
+        unknown directions get f_tgt + f_in - f_loc, the known ones
+        f_tgt + f - f_loc, with f_in the equilibrium of the LODI inflow state,
+        f_loc that of the local moments (inflow populations substituted) and
+        f_tgt that of the local density and composition at the inflow
+        velocity and temperature."""
+        U_ = self.units
+        updates = []
+        for c, l in self.interface_cells((TYPE_I, TYPE_I_C)):
+            i, j, k = self._ijk(c)
+            if self.type[self._idx(i - CX[l], j - CY[l], k - CZ[l])] != TYPE_I_C:
+                raise ValueError("only TYPE_I_C inflow is handled")
+            rho_in, rhoa_in, vel_in, T_in = self._inflow_lodi(c, l)
+            fa_in, g_in = self._feq_geq(rho_in, rhoa_in, vel_in, np.repeat(vel_in[None], self.ns, 0), T_in)
+            unknown = np.array([self.type[self._idx(i - CX[q], j - CY[q], k - CZ[q])] in (TYPE_I, TYPE_I_C)
+                                for q in range(27)])
+            fsrc = np.where(unknown[None, :], fa_in, self.f[:, :, c])
+            gsrc = np.where(unknown, g_in, self.g[:, c])
+            rhoa_loc = fsrc.sum(axis=1)
+            mom = fsrc @ C3.T.astype(np.float64)
+            rhoe_loc = gsrc.sum()
+            rho_loc = rhoa_loc.sum()
+            vel_loc = mom.sum(axis=0) / rho_loc if rho_loc > 0 else np.zeros(3)
+            vela_loc = np.where(rhoa_loc[:, None] > 0, mom / np.where(rhoa_loc > 0, rhoa_loc, 1.0)[:, None], 0.0)
+            Uint = rhoe_loc / rho_loc
+            for a in range(self.ns):
+                Uint = Uint - 0.5 * rhoa_loc[a] / rho_loc * (vela_loc[a] @ vela_loc[a])
+            T_loc = self._calculate_temp(Uint, rho_loc, rhoa_loc, np.float64(U_.si_temp(self.T[c])))
+            fa_loc, g_loc = self._feq_geq(rho_loc, rhoa_loc, vel_loc, vela_loc, T_loc)
+            fa_tgt, g_tgt = self._feq_geq(rho_loc, rhoa_loc, vel_in, np.repeat(vel_in[None], self.ns, 0), T_in)
+            fnew = np.where(unknown[None, :], fa_tgt + fa_in - fa_loc, fa_tgt + self.f[:, :, c] - fa_loc)
+            gnew = np.where(unknown, g_tgt + g_in - g_loc, g_tgt + self.g[:, c] - g_loc)
             updates.append((c, fnew, gnew))
         for c, fnew, gnew in updates:
             self.f[:, :, c] = fnew

