@@ -865,7 +865,7 @@ int lbm_collide(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
-    if (multi(c)) return run_mc(c, McKernel::Collide, "mc_collide", 0, 1);
+    if (multi(c)) return run_mc(c, McKernel::Collide, "mc_collide_mixture", 0, 1);
     if (!fused(c)) return run_collide_explicit(c, 0, 1);
     int rc;
     if (c->kind == lbm_ctx::EXPLICIT_F) {

@@ -119,22 +119,31 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t* dZT, const uint16_
     }
 }
 
-// Sums the chunk partials in a fixed order (deterministic) and the bias
-// gradient db[n] = sum_m dZ[m][n].
-__global__ void k_reduce_grad(const float* part, int nchunk, int N, int K, float* dW, const uint16_t* dZT, int M,
-                              float* db)
+// Sums the chunk partials in a fixed order (deterministic).
+__global__ void k_reduce_grad(const float* part, int nchunk, int N, int K, float* dW)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N * K) {
-        float s = 0.f;
-        for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * N * K + i];
-        dW[i] = s;
+    if (i >= N * K) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * N * K + i];
+    dW[i] = s;
+}
+
+// Bias gradient db[n] = sum_m dZ[m][n]: one block per n, strided partial
+// sums reduced in LDS in a fixed order.
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* dZT, int M, float* db)
+{
+    __shared__ float red[256];
+    const int nrow = blockIdx.x;
+    float s = 0.f;
+    for (int m = threadIdx.x; m < M; m += 256) s += bf2f(dZT[(size_t)nrow * M + m]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
-    if (i < N) {
-        float s = 0.f;
-        for (int m = 0; m < M; ++m) s += bf2f(dZT[(size_t)i * M + m]);
-        db[i] = s;
-    }
+    if (threadIdx.x == 0) db[nrow] = red[0];
 }
 
 // Adam on FP32 master weights; refreshes the BF16 copies W and W^T.
@@ -412,8 +421,8 @@ void mlp_grad(const Mlp& net, const uint16_t* X, const float* T, int M, Work& w,
         const int nchunk = cdiv(M, w.chunk);
         hipLaunchKernelGGL(k_wgrad, dim3(N / 16, K / 16, nchunk), dim3(64), 0, s, w.dZT, w.XT, M, N, K, w.chunk,
                            w.part);
-        hipLaunchKernelGGL(k_reduce_grad, dim3(cdiv(N * K, 256)), dim3(256), 0, s, w.part, nchunk, N, K, w.gW[L],
-                           w.dZT, M, w.gb[L]);
+        hipLaunchKernelGGL(k_reduce_grad, dim3(cdiv(N * K, 256)), dim3(256), 0, s, w.part, nchunk, N, K, w.gW[L]);
+        hipLaunchKernelGGL(k_bias_grad, dim3(N), dim3(256), 0, s, w.dZT, M, w.gb[L]);
         if (L > 0)
             hipLaunchKernelGGL(k_layer_bwd_x, dim3(M / 16, K / 16), dim3(64), 0, s, w.dZ[L], N, net.WbT[L], K,
                                w.Z[L - 1], w.dZ[L - 1], M);
