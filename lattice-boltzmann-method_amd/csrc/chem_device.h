@@ -255,6 +255,24 @@ __host__ __device__ inline double binary_diff(int a, int b, double lnT, double p
     return exp(poly4(DIFF_FIT[a][b], lnT)) / p;
 }
 
+// The same fits evaluated in arithmetic type R (double reproduces the
+// functions above bit for bit; float is the FP32 path's binary32 transport).
+template <typename R>
+__host__ __device__ inline R poly4r(const double c[5], R x)
+{
+    return (R)c[0] + x * ((R)c[1] + x * ((R)c[2] + x * ((R)c[3] + x * (R)c[4])));
+}
+template <typename R>
+__host__ __device__ inline R species_viscosity_r(int k, R lnT)
+{
+    return exp(poly4r<R>(VISC_FIT[k], lnT));
+}
+template <typename R>
+__host__ __device__ inline R binary_diff_r(int a, int b, R lnT, R p)
+{
+    return exp(poly4r<R>(DIFF_FIT[a][b], lnT)) / p;
+}
+
 // Wilke mixture viscosity with the direct Phi_ab formula.
 __host__ __device__ inline double mix_viscosity(double lnT, const double X[NS])
 {
@@ -273,6 +291,39 @@ __host__ __device__ inline double mix_viscosity(double lnT, const double X[NS])
         tot += X[a] * mu[a] / den;
     }
     return tot;
+}
+
+template <typename R>
+__host__ __device__ inline R mix_viscosity_r(R lnT, const double X[NS])
+{
+    R mu[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) mu[k] = species_viscosity_r<R>(k, lnT);
+    R tot = 0;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        R den = 0;
+#pragma unroll
+        for (int b = 0; b < NS; ++b) {
+            const R r = R(1) + sqrt(mu[a] / mu[b]) * (R)sqrt(sqrt(W[b] / W[a]));
+            den += (R)X[b] * (r * r / (R)sqrt(8.0 * (1.0 + W[a] / W[b])));
+        }
+        tot += (R)X[a] * mu[a] / den;
+    }
+    return tot;
+}
+
+template <typename R>
+__host__ __device__ inline R mix_conductivity_r(R lnT, const double X[NS])
+{
+    R s1 = 0, s2 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const R l = exp(poly4r<R>(COND_FIT[k], lnT));
+        s1 += (R)X[k] * l;
+        s2 += (R)X[k] / l;
+    }
+    return R(0.5) * (s1 + R(1) / s2);
 }
 
 // Combination-averaged mixture conductivity 1/2 (sum X l + 1 / sum X / l).

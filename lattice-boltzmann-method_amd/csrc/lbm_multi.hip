@@ -13,6 +13,8 @@
 //   Collide          lbm-collision.cpp:191-332
 //   Streaming        lbm-streaming.cpp:3-119
 //   calculate_moment lbm-moment.cpp:193-377
+#include <type_traits>
+
 #include "chem_device.h"
 #include "lbm_multi.h"
 #include "surrogate_device.h"
@@ -453,30 +455,34 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
     normalize(Y0);
     const double Tsi = v.u.si_temp(T);
     const double rho_si = v.u.si_rho(rho);
-    // species viscosities and Wilke weights at (T, rho)
+    // species viscosities and Wilke weights at (T, rho); transport arithmetic
+    // R is binary64 on the plain path and binary32 on the FP32 path
+    using R = typename std::conditional<sizeof(P) == 4, float, double>::type;
     const double lnT = log(Tsi);
-    double mu[NS], den[NS];
+    const R lnTr = (R)lnT;
+    R mu[NS], den[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
-        mu[a] = species_viscosity(a, lnT);
-        den[a] = 0.0;
+        mu[a] = species_viscosity_r<R>(a, lnTr);
+        den[a] = 0;
     }
 #pragma unroll
     for (int a = 0; a < NS; ++a)
 #pragma unroll
         for (int b = a; b < NS; ++b) {
-            const double f1 = 1.0 + sqrt(mu[a] / mu[b]) * sqrt(sqrt(W[b] / W[a]));
-            const double pab = f1 * f1 / sqrt(8.0 * (1.0 + W[a] / W[b]));
-            den[a] += Xa[b] * pab;
-            if (b != a) den[b] += Xa[a] * (mu[b] / mu[a] * W[a] / W[b] * pab);
+            const R f1 = R(1) + sqrt(mu[a] / mu[b]) * (R)sqrt(sqrt(W[b] / W[a]));
+            const R pab = f1 * f1 / (R)sqrt(8.0 * (1.0 + W[a] / W[b]));
+            den[a] += (R)Xa[b] * pab;
+            if (b != a) den[b] += (R)Xa[a] * (mu[b] / mu[a] * (R)W[a] / (R)W[b] * pab);
         }
     double om[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
-        const double visc = Xa[a] * (double)v.u.mu(mu[a]) / den[a];
+        const double visc = Xa[a] * (double)v.u.mu(mu[a]) / (double)den[a];
         om[a] = 2 * Xa[a] * p / (Xa[a] * p + 2 * visc);
     }
     const double p_si = pressure(Tsi, rho_si, Y0);
+    const R p_sir = (R)p_si;
     const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
     const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
     const double* Tf = Fd[LBM_FIELD_T];
@@ -488,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
             const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
                          uaz = v.ua[(2 * NS + a) * n + c];
             for (int b = 0; b < NS; ++b) {
-                const double D = v.u.nu(binary_diff(a, b, lnT, p_si));
+                const double D = v.u.nu(binary_diff_r<R>(a, b, lnTr, p_sir));
                 const double k = -1.0 * p * Xa[a] * Xa[b] / D;
                 acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
                 acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
@@ -566,8 +572,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
         const double cp = v.u.cp(cp_mass(Tsi, Y0));
         const double Wm = mean_W(Y0);
         const double Rmix = v.u.cp(R_UNIV / Wm);
-        const double mu = v.u.mu(mix_viscosity(lnT, X0));
-        const double kap = v.u.thermal_conductivity(mix_conductivity(lnT, X0));
+        // transport arithmetic R: binary64 on the plain path, binary32 on the FP32 path
+        using R = typename std::conditional<sizeof(P) == 4, float, double>::type;
+        const double mu = v.u.mu(mix_viscosity_r<R>((R)lnT, X0));
+        const double kap = v.u.thermal_conductivity(mix_conductivity_r<R>((R)lnT, X0));
         const double theta = v.u.energy_mass(R_UNIV * Tsi / Wm);
         const double om1 = 2 * p * cp * 1 / (p * cp * 1 + 2 * kap);
         const double om = 2 * p * 1 / (p * 1 + 2 * mu);
