@@ -183,6 +183,7 @@ MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
     MultiView<P> v{};
     v.d = c->d;
     v.type = c->type;
+    v.nbflag = c->nbflag;
     v.f = static_cast<P*>(c->sf[src]);
     v.fo = static_cast<P*>(c->sf[dst]);
     v.g = static_cast<P*>(c->pg[src]);

@@ -10,6 +10,7 @@ template <typename P>
 struct MultiView {
     Dims d;
     const uint8_t* type;
+    const uint8_t* nbflag;  // NB_SIMPLE: every neighbour is fluid or a periodic ghost
     P* f;   // species populations of the source buffer, [NS][27][n]
     P* fo;  // destination buffer
     P* g;   // mixture energy populations, [27][n]

@@ -656,17 +656,31 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
 {
+    // one thread per (cell, population set): blockIdx.y = species 0..NS-1, NS = g
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int set = blockIdx.y;
     const int64_t n = v.d.n;
     const Ijk x = ijk(v.d, c);
     const int nx = v.d.nx, ny = v.d.ny, nz = v.d.nz;
+    const P* src_base = set < NS ? v.f + (int64_t)set * Q * n : v.g;
+    P* dst_base = set < NS ? v.fo + (int64_t)set * Q * n : v.go;
+    if (v.nbflag[c] & NB_SIMPLE) {  // fluid and periodic neighbours only: plain pull with the wrap
+        const int si[3] = {x.i + 1 == nx - 1 ? 1 : x.i + 1, x.i, x.i - 1 == 0 ? nx - 2 : x.i - 1};
+        const int sj[3] = {x.j + 1 == ny - 1 ? 1 : x.j + 1, x.j, x.j - 1 == 0 ? ny - 2 : x.j - 1};
+        const int sk[3] = {x.k + 1 == nz - 1 ? 1 : x.k + 1, x.k, x.k - 1 == 0 ? nz - 2 : x.k - 1};
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            dst_base[(int64_t)l * n + c] = src_base[(int64_t)l * n + cell(v.d, si[cx(l) + 1], sj[cy(l) + 1], sk[cz(l) + 1])];
+        return;
+    }
+#pragma unroll
     for (int l = 0; l < Q; ++l) {
         int in = x.i - cx(l), jn = x.j - cy(l), kn = x.k - cz(l);
         const uint8_t tn = v.type[cell(v.d, in, jn, kn)];
         if (tn == TYPE_S || tn == TYPE_A) {
-            for (int a = 0; a < NS; ++a) spop(v.fo, a, l, n, c) = spop(v.f, a, opp(l), n, c);
-            if (tn == TYPE_A) v.go[(int64_t)l * n + c] = v.g[(int64_t)opp(l) * n + c];
+            // bounce-back; the energy populations bounce only at adiabatic walls
+            if (set < NS || tn == TYPE_A) dst_base[(int64_t)l * n + c] = src_base[(int64_t)opp(l) * n + c];
         } else if (tn == TYPE_O || tn == TYPE_O_C || tn == TYPE_I || tn == TYPE_I_C || tn == TYPE_FS) {
             // inflow/outflow sources are left for TMS_BC; free-slip walls are
             // rejected for the multicomponent state at set-up
@@ -674,9 +688,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
             if (in == 0) in = nx - 2; else if (in == nx - 1) in = 1;
             if (jn == 0) jn = ny - 2; else if (jn == ny - 1) jn = 1;
             if (kn == 0) kn = nz - 2; else if (kn == nz - 1) kn = 1;
-            const int src = cell(v.d, in, jn, kn);
-            for (int a = 0; a < NS; ++a) spop(v.fo, a, l, n, c) = spop(v.f, a, l, n, src);
-            v.go[(int64_t)l * n + c] = v.g[(int64_t)l * n + src];
+            dst_base[(int64_t)l * n + c] = src_base[(int64_t)l * n + cell(v.d, in, jn, kn)];
         }
     }
 }
@@ -1292,7 +1304,9 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
         hipLaunchKernelGGL(k_mc_collide_species<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);
         break;
     case McKernel::Collide: hipLaunchKernelGGL(k_mc_collide<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
-    case McKernel::Stream: hipLaunchKernelGGL(k_mc_stream<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::Stream:
+        hipLaunchKernelGGL(k_mc_stream<P>, dim3((unsigned)((v.d.n + BLOCK - 1) / BLOCK), NS + 1), dim3(BLOCK), 0, s, v);
+        break;
     case McKernel::Moments: hipLaunchKernelGGL(k_mc_moments<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     }
 }
