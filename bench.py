@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="mixing256", choices=["mixing256", "turb256", "tgv64"])
+    ap.add_argument("--workload", default="mixing256", choices=["mixing256", "turb256", "tgv64", "copy1g"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -245,6 +245,21 @@ def main():
     from lbm_amd import cases
     from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED
 
+    if args.workload == "copy1g":
+        # calibration for the PMC byte counters: a plain 1 GiB device copy
+        import torch
+        n = 1 << 28
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        print(json.dumps({"workload": "copy1g", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                          "copies": args.steps, "GBs": 8 * n * args.steps / sec / 1e9}))
+        return
     if args.workload == "mixing256":
         rec = mixing_layer_bench(args)
         if not args.no_cpu_baseline:

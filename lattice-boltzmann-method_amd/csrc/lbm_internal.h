@@ -14,6 +14,22 @@ struct Dims {
     int64_t n;
 };
 
+// XCD-aware workgroup order.  The dispatcher hands consecutive workgroups to
+// the 8 XCDs round-robin, and each XCD has its own L2; remapping gives every
+// XCD one contiguous range of tiles, so the rows that neighbouring tiles pull
+// from meet in the same L2.  Launch with xcd_grid (a multiple of 8 workgroups).
+constexpr unsigned N_XCD = 8;
+__device__ inline unsigned xcd_block()
+{
+    const unsigned per = gridDim.x / N_XCD;
+    return (blockIdx.x % N_XCD) * per + blockIdx.x / N_XCD;
+}
+inline unsigned xcd_grid(int64_t n, int block)
+{
+    const int64_t nb = (n + block - 1) / block;
+    return (unsigned)((nb + N_XCD - 1) / N_XCD * N_XCD);
+}
+
 // nbflag bits, computed once per lbm_set_types.
 enum : uint8_t {
     NB_SIMPLE = 1,  // TYPE_F cell whose 26 neighbours are all TYPE_F or TYPE_P

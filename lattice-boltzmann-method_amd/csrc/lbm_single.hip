@@ -469,7 +469,7 @@ __device__ inline void pull_cell(const SingleView<P>& v, const Ijk& x, int c, P 
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_fused_collide(SingleView<P> v)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
     bool warn = false;
     if (c < v.d.n && v.type[c] == TYPE_F) {
         const Ijk x = ijk_of(v.d, c);
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_fused_collide(SingleView<P> v)
 template <typename P, bool Mat>
 __global__ __launch_bounds__(BLOCK) void k_fused_moments(SingleView<P> v)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
     const Ijk x = ijk_of(v.d, c);
     P fl[Q], gl[Q];
@@ -633,15 +633,15 @@ void launch_moments(const SingleView<P>& v, hipStream_t s)
 template <typename P>
 void launch_fused_collide(const SingleView<P>& v, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fused_collide<P>, grid_of(v.d.n), dim3(BLOCK), 0, s, v);
+    hipLaunchKernelGGL(k_fused_collide<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
 }
 template <typename P>
 void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t s)
 {
     if (materialize)
-        hipLaunchKernelGGL((k_fused_moments<P, true>), grid_of(v.d.n), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_fused_moments<P, true>), dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
     else
-        hipLaunchKernelGGL((k_fused_moments<P, false>), grid_of(v.d.n), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_fused_moments<P, false>), dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
 }
 
 int diag_partials(const Dims& d) { return (int)((d.n + BLOCK - 1) / BLOCK); }
