@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--chemistry", default="surrogate", choices=["surrogate", "plain"],
+                    help="mixing256: learned chemistry (default) or the plain integrator in the FP32 run "
+                         "(no capture or training; used for the PMC byte-counter passes)")
     return ap.parse_args()
 
 
@@ -170,31 +173,36 @@ def mixing_layer_bench(args):
     lat.load_case(fl)
     lat.Init()
     lat.calculate_moment()
-    lat.surrogate_capture(cap_steps * cells // cap_stride + 4096, cap_stride)
-    lat.step(cap_steps)
-    lat.surrogate_capture_stop()
-    lat.sync()
-    t0 = time.perf_counter()
-    losses = np.concatenate([lat.surrogate_train(train_iters * 3 // 4, batch=batch, lr=2e-3, seed=1),
-                             lat.surrogate_train(train_iters // 4, batch=batch, lr=3e-4, seed=2)])
-    train_s = time.perf_counter() - t0
-    lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
+    surrogate = args.chemistry == "surrogate"
+    if surrogate:
+        lat.surrogate_capture(cap_steps * cells // cap_stride + 4096, cap_stride)
+        lat.step(cap_steps)
+        lat.surrogate_capture_stop()
+        lat.sync()
+        t0 = time.perf_counter()
+        losses = np.concatenate([lat.surrogate_train(train_iters * 3 // 4, batch=batch, lr=2e-3, seed=1),
+                                 lat.surrogate_train(train_iters // 4, batch=batch, lr=3e-4, seed=2)])
+        train_s = time.perf_counter() - t0
+        lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
+    else:
+        lat.step(cap_steps)
     sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS)
     T32, Y32 = fields(lat)
-    sstats = lat.surrogate_stats()
+    sstats = lat.surrogate_stats() if surrogate else None
     diag = lat.diagnostics()
     dev_bytes = lat.device_bytes()
     lat.close()
     mlups = cells * steps / sec / 1e6
 
     extra = {"kernels": {n: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]} for n, v in kern.items()},
-             "surrogate": {"train_s": round(train_s, 2), "train_iters": train_iters, "batch": batch,
-                           "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-50:].mean()),
-                           "captured_states": None, **sstats},
              "device_bytes_fp32": dev_bytes, "nan_count": diag["nan_count"]}
-    extra["surrogate"]["captured_states"] = extra["surrogate"].pop("captured")
-    extra["fallback_rate"] = round((sstats["fallback"] + sstats["audit_failures"]) / max(sstats["predicted"] +
-                                                                                        sstats["fallback"], 1), 6)
+    if surrogate:
+        extra["surrogate"] = {"train_s": round(train_s, 2), "train_iters": train_iters, "batch": batch,
+                              "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-50:].mean()),
+                              "captured_states": None, **sstats}
+        extra["surrogate"]["captured_states"] = extra["surrogate"].pop("captured")
+        extra["fallback_rate"] = round((sstats["fallback"] + sstats["audit_failures"]) /
+                                       max(sstats["predicted"] + sstats["fallback"], 1), 6)
 
     # dominant kernel's roofline (FP32 storage)
     roofline = None
@@ -228,7 +236,7 @@ def mixing_layer_bench(args):
         "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(sec / steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "reacting_mixing_layer_256x128x128_9sp_fp32_shifted_surrogate", "cells": cells,
+        "config": {"workload": "reacting_mixing_layer_256x128x128_9sp_fp32_shifted_" + args.chemistry, "cells": cells,
                    "grid": [nx, ny, nz], "mode": "strict", "parallelism": "none"},
         "roofline": roofline, "cpu_baseline": None, "extra": extra, "host": platform.node(),
     }

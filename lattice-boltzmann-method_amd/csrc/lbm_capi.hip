@@ -75,6 +75,9 @@ struct lbm_ctx {
     int* tin_dir = nullptr;
     int tin_count = 0;
     std::string tms_unsupported;
+    // single-component TMS_BC interface cells (fluid cells next to TYPE_O/I/S)
+    int* sbc_cell = nullptr;
+    int sbc_count = 0;
     double* src = nullptr;  // [NS][n] chemistry mass sources
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
     double mfref[mech::NS * Q] = {}, mgref[Q] = {};
@@ -348,7 +351,7 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir};
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -579,6 +582,25 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                     }
                 }
     }
+    // single-component TMS_BC interface cells: some direction with a TYPE_O,
+    // TYPE_I or TYPE_S upstream neighbour and a fluid downstream neighbour
+    // (the interface tests of lbm-bc.cpp:357-368, 827-838, 1128-1137)
+    std::vector<int> scell;
+    if (!multi(c))
+        for (int i = 1; i < d.nx - 1; ++i)
+            for (int j = 1; j < d.ny - 1; ++j)
+                for (int k = 1; k < d.nz - 1; ++k) {
+                    const int64_t cidx = ((int64_t)i * d.ny + j) * d.nz + k;
+                    if (types[cidx] != TYPE_F) continue;
+                    for (int l = 0; l < Q; ++l) {
+                        const uint8_t tn = types[((int64_t)(i - cx(l)) * d.ny + (j - cy(l))) * d.nz + (k - cz(l))];
+                        const uint8_t tp = types[((int64_t)(i + cx(l)) * d.ny + (j + cy(l))) * d.nz + (k + cz(l))];
+                        if ((tn == TYPE_O || tn == TYPE_I || tn == TYPE_S) && tp == TYPE_F) {
+                            scell.push_back((int)cidx);
+                            break;
+                        }
+                    }
+                }
     if (fused(c) && (needs_tms || !fused_ok))
         return fail(c, LBM_E_UNSUPPORTED,
                     "fused mode handles periodic, adiabatic (TYPE_A) and free-slip (TYPE_FS) neighbours; "
@@ -610,6 +632,15 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
         }
         c->tms_unsupported = tms_unsupported;
         needs_tms = false;
+    }
+    if (!multi(c)) {
+        if (c->sbc_cell) (void)hipFree(c->sbc_cell);
+        c->sbc_cell = nullptr;
+        c->sbc_count = (int)scell.size();
+        if (c->sbc_count) {
+            HIP_TRY(c, hipMalloc((void**)&c->sbc_cell, sizeof(int) * scell.size()));
+            HIP_TRY(c, hipMemcpy(c->sbc_cell, scell.data(), sizeof(int) * scell.size(), hipMemcpyHostToDevice));
+        }
     }
     c->needs_tms = needs_tms;
     c->types_set = true;
@@ -908,10 +939,15 @@ int lbm_tms_bc(lbm_ctx* c)
             launch_mc_tms_in<P>(make_mview<P>(c, 0, 0), c->tin_cell, c->tin_dir, c->tin_count, c->stream);
         }));
     }
-    if (c->needs_tms)
-        return fail(c, LBM_E_UNSUPPORTED,
-                    "single-component TMS reconstruction next to TYPE_S/I/O cells is not provided by this build");
-    return LBM_OK;  // periodic, adiabatic and free-slip walls need no reconstruction
+    if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
+    // fused mode rejects TYPE_S/I/O neighbours at lbm_set_types; periodic,
+    // adiabatic and free-slip walls need no reconstruction
+    if (fused(c) || c->sbc_count == 0) return LBM_OK;
+    // outflow, inflow and isothermal-wall sweeps (lbm-bc.cpp:341, 809, 1111) on
+    // the streamed populations of buffer 0
+    return DISPATCH(c, launch(c, "tms", [&] {
+        launch_tms<P>(make_view<P>(c, 0, 0), c->sbc_cell, c->sbc_count, c->stream);
+    }));
 }
 
 int lbm_moments(lbm_ctx* c)

@@ -67,6 +67,9 @@ template <typename P> void launch_stream(const SingleView<P>& v, hipStream_t s);
 template <typename P> void launch_moments(const SingleView<P>& v, hipStream_t s);
 template <typename P> void launch_fused_collide(const SingleView<P>& v, hipStream_t s);
 template <typename P> void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t s);
+// TMS_BC on the interface cells `cells` (fluid cells with a TYPE_O, TYPE_I or
+// TYPE_S upstream neighbour whose downstream neighbour is fluid), in place on v.f/v.g.
+template <typename P> void launch_tms(const SingleView<P>& v, const int* cells, int count, hipStream_t s);
 // gamma: ratio of specific heats used for the Mach number |u| / sqrt(gamma p / rho)
 void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
                  double* out8, hipStream_t s);

@@ -924,7 +924,6 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     const int t = blockIdx.x * 64 + threadIdx.x;
     if (t >= count) return;
     const int c = cells[t], l = dirs[t];
-    const int64_t n = v.d.n;
     const Ijk x = ijk(v.d, c);
     const int strides[3] = {v.d.ny * v.d.nz, v.d.nz, 1};
     const int Ns[3] = {v.d.nx - 2, v.d.ny - 2, v.d.nz - 2};

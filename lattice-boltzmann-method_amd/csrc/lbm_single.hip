@@ -11,6 +11,8 @@
 //   Collide        lbm-collision.cpp:9-99
 //   Streaming      lbm-streaming.cpp:3-119 (+ dirSlip, lbm-bc.cpp:1352-1399)
 //   calculate_moment lbm-moment.cpp:11-85
+//   TMS_BC         lbm-bc.cpp:341-460, 809-938, 1111-1230 (+ calculate_feq_geq,
+//                  lbm-edf.cpp:153-164)
 //   OutputKeEns    output.cpp:212-276 (diagnostic reduction)
 #include <type_traits>
 
@@ -607,6 +609,153 @@ __global__ __launch_bounds__(BLOCK) void k_diag_final(const double* part, int nb
     }
 }
 
+// ----------------------------------------------------------------- TMS_BC
+// The three single-component sweeps of TMS_BC (lbm-bc.cpp:341-460 outflow,
+// 809-938 inflow, 1111-1230 isothermal wall).  Each sweep rewrites only the
+// populations of its own interface cell and reads the macroscopic fields of
+// neighbours, which TMS_BC never writes, so applying the three sweeps in
+// order to one cell at a time (one thread per interface cell) is the
+// reference's sweep-after-sweep result.  Arithmetic is FP64 for both storage
+// types.
+
+// calculate_feq_geq (lbm-edf.cpp:153-164) of one state, direction by direction.
+struct EqSet {
+    GeqAxes<double> G;
+    double rho, Ushift;
+};
+__device__ inline EqSet eq_set(double rho, const double u[3], double T, double R, double cv)
+{
+    EqSet e;
+    e.G = geq_axes<double>(R * T, u);
+    e.rho = rho;
+    e.Ushift = cv * T - 1.5 * (R * T);
+    return e;
+}
+__device__ inline double eq_f(const EqSet& e, int l)
+{
+    return e.rho * pick(e.G.P[0], cx(l)) * pick(e.G.P[1], cy(l)) * pick(e.G.P[2], cz(l));
+}
+__device__ inline double eq_g(const EqSet& e, int l) { return geq_dir(e.G, l, e.rho, e.Ushift); }
+
+// One reconstruction: the unknown directions `unk` take the boundary set `b`
+// in the local moments; then every direction becomes
+//   tgt + f - loc (known)   or   tgt + b - loc (unknown),
+// with tgt = b (outflow, wall) or the equilibrium of the local density at the
+// boundary velocity and temperature (inflow, `inflow` set).
+__device__ inline void tms_reconstruct(const EqSet& b, uint32_t unk, bool inflow, const double u_b[3], double T_b,
+                                       double R, double cv, double f[Q], double g[Q])
+{
+    double r = 0, mx = 0, my = 0, mz = 0, e = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const bool un = (unk >> l) & 1u;
+        const double a = un ? eq_f(b, l) : f[l];
+        const double q = un ? eq_g(b, l) : g[l];
+        r += a;
+        if (cx(l)) mx += a * cx(l);
+        if (cy(l)) my += a * cy(l);
+        if (cz(l)) mz += a * cz(l);
+        e += q;
+    }
+    const double ul[3] = {mx / r, my / r, mz / r};
+    const double T = (e / r - 0.5 * (ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2])) / cv;
+    const EqSet loc = eq_set(r, ul, T, R, cv);
+    const EqSet tgt = inflow ? eq_set(r, u_b, T_b, R, cv) : b;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const bool un = (unk >> l) & 1u;
+        const double fk = un ? eq_f(b, l) : f[l];
+        const double gk = un ? eq_g(b, l) : g[l];
+        f[l] = (eq_f(tgt, l) + fk) - eq_f(loc, l);
+        g[l] = (eq_g(tgt, l) + gk) - eq_g(loc, l);
+    }
+}
+
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_tms(SingleView<P> v, const int* cells, int count)
+{
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t];
+    const Ijk x = ijk_of(v.d, c);
+    double* const* F = v.fld;
+    const double R = v.R, cv = v.cv;
+    // unknown directions per boundary kind and the first interface direction
+    // (upstream of the kind, downstream fluid) of the outflow / inflow sweeps
+    uint32_t unkO = 0, unkI = 0, unkS = 0, wall = 0;
+    int lo = -1, li = -1;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const uint8_t tu = v.type[cell_of(v.d, x.i - cx(l), x.j - cy(l), x.k - cz(l))];
+        const bool dnF = v.type[cell_of(v.d, x.i + cx(l), x.j + cy(l), x.k + cz(l))] == TYPE_F;
+        if (tu == TYPE_O) {
+            unkO |= 1u << l;
+            if (dnF && lo < 0) lo = l;
+        } else if (tu == TYPE_I) {
+            unkI |= 1u << l;
+            if (dnF && li < 0) li = l;
+        } else if (tu == TYPE_S) {
+            unkS |= 1u << l;
+            if (dnF) wall |= 1u << l;
+        }
+    }
+    double f[Q], g[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        f[l] = double(ldp(v.f, l, v.d.n, c)) + v.fref_d(l);
+        g[l] = double(ldp(v.g, l, v.d.n, c)) + v.gref_d(l);
+    }
+    if (lo >= 0) {
+        // velocity and temperature downstream, pressure of the outflow ghost
+        const int dn = cell_of(v.d, x.i + cx(lo), x.j + cy(lo), x.k + cz(lo));
+        const int up = cell_of(v.d, x.i - cx(lo), x.j - cy(lo), x.k - cz(lo));
+        const double u[3] = {F[LBM_FIELD_U][dn], F[LBM_FIELD_V][dn], F[LBM_FIELD_W][dn]};
+        const double T = F[LBM_FIELD_T][dn];
+        const EqSet b = eq_set(F[LBM_FIELD_P][up] / (R * T), u, T, R, cv);
+        tms_reconstruct(b, unkO, false, u, T, R, cv, f, g);
+    }
+    if (li >= 0) {
+        // velocity and temperature of the inflow ghost, pressure downstream
+        const int dn = cell_of(v.d, x.i + cx(li), x.j + cy(li), x.k + cz(li));
+        const int up = cell_of(v.d, x.i - cx(li), x.j - cy(li), x.k - cz(li));
+        const double u[3] = {F[LBM_FIELD_U][up], F[LBM_FIELD_V][up], F[LBM_FIELD_W][up]};
+        const double T = F[LBM_FIELD_T][up];
+        const EqSet b = eq_set(F[LBM_FIELD_P][dn] / (R * T), u, T, R, cv);
+        tms_reconstruct(b, unkI, true, u, T, R, cv, f, g);
+    }
+    if (wall) {
+        // wall state at q = 1/2 between the cell downstream and the wall cell,
+        // averaged over the interface links; the cell's own density
+        double u[3] = {0, 0, 0}, T = 0, rbb = 0;
+        int nd = 0;
+        const double q = 0.5;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            rbb += f[l];
+            if (!((wall >> l) & 1u)) continue;
+            ++nd;
+            const int dn = cell_of(v.d, x.i + cx(l), x.j + cy(l), x.k + cz(l));
+            const int up = cell_of(v.d, x.i - cx(l), x.j - cy(l), x.k - cz(l));
+            u[0] += (q * F[LBM_FIELD_U][dn] + F[LBM_FIELD_U][up]) / (1.0 + q);
+            u[1] += (q * F[LBM_FIELD_V][dn] + F[LBM_FIELD_V][up]) / (1.0 + q);
+            u[2] += (q * F[LBM_FIELD_W][dn] + F[LBM_FIELD_W][up]) / (1.0 + q);
+            T += (q * F[LBM_FIELD_T][dn] + F[LBM_FIELD_T][up]) / (1.0 + q);
+        }
+        const double inv = 1.0 / nd;
+        u[0] = inv * u[0];
+        u[1] = inv * u[1];
+        u[2] = inv * u[2];
+        T = inv * T;
+        const EqSet b = eq_set(rbb, u, T, R, cv);
+        tms_reconstruct(b, unkS, false, u, T, R, cv, f, g);
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        stp(v.f, l, v.d.n, c, P(f[l] - v.fref_d(l)));
+        stp(v.g, l, v.d.n, c, P(g[l] - v.gref_d(l)));
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 static inline dim3 grid_of(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
@@ -644,6 +793,12 @@ void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t 
         hipLaunchKernelGGL((k_fused_moments<P, false>), dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
 }
 
+template <typename P>
+void launch_tms(const SingleView<P>& v, const int* cells, int count, hipStream_t s)
+{
+    if (count > 0) hipLaunchKernelGGL(k_tms<P>, grid_of(count), dim3(BLOCK), 0, s, v, cells, count);
+}
+
 int diag_partials(const Dims& d) { return (int)((d.n + BLOCK - 1) / BLOCK); }
 
 void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
@@ -661,7 +816,8 @@ void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double 
     template void launch_stream<P>(const SingleView<P>&, hipStream_t);                     \
     template void launch_moments<P>(const SingleView<P>&, hipStream_t);                    \
     template void launch_fused_collide<P>(const SingleView<P>&, hipStream_t);              \
-    template void launch_fused_moments<P>(const SingleView<P>&, bool, hipStream_t);
+    template void launch_fused_moments<P>(const SingleView<P>&, bool, hipStream_t);        \
+    template void launch_tms<P>(const SingleView<P>&, const int*, int, hipStream_t);
 LBM_INSTANTIATE(double)
 LBM_INSTANTIATE(float)
 

@@ -69,6 +69,49 @@ def taylor_green_3d(N, U0, p0, T0, **_unused):
     return fl
 
 
+def channel(nx, ny, nz, U, T0=1.0, p0=1.0 / 3.0, T_in=1.1, T_wall=0.9, p_out=None, R=1.0 / 3.0):
+    """Single-component channel with every TMS_BC boundary kind.
+
+    Fixed-state inflow ghost (TYPE_I: u = U, v = 0.05 U, T = T_in) at i = 0,
+    outflow ghost (TYPE_O: p = p_out) at i = nx+1, isothermal walls (TYPE_S,
+    T = T_wall) at j = 0 (at rest) and j = ny+1 (moving with u = -0.3 U),
+    periodic in z; the x faces are typed last.  Interior: u = U with
+    transverse waves v = 0.1 U sin(2 pi i/nx) sin(pi j/(ny+1)) and
+    w = 0.05 U cos(2 pi (k-1)/nz) sin(pi j/(ny+1)), T = T0, p = p0.
+    """
+    fl = default_fields(nx, ny, nz)
+    t = fl["type"]
+    t[:, :, 0] = TYPE_P
+    t[:, :, -1] = TYPE_P
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    t[0, :, :] = TYPE_I
+    t[-1, :, :] = TYPE_O
+    shape = t.shape
+    i, j, k = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), np.arange(shape[2]), indexing="ij")
+    for name in ("u", "v", "w"):
+        fl[name][:] = 0.0
+    fl["T"][:] = T0
+    fl["p"][:] = p0
+    F = t == TYPE_F
+    # libm's sin/cos (math module) element by element, as a C set-up evaluates them
+    msin, mcos = np.vectorize(math.sin), np.vectorize(math.cos)
+    sj = msin(math.pi * j / (ny + 1))
+    fl["u"][F] = U
+    fl["v"][F] = (0.1 * U * msin(2.0 * math.pi * i / nx) * sj)[F]
+    fl["w"][F] = (0.05 * U * mcos(2.0 * math.pi * (k - 1) / nz) * sj)[F]
+    I = t == TYPE_I
+    fl["u"][I] = U
+    fl["v"][I] = 0.05 * U
+    fl["T"][I] = T_in
+    fl["p"][t == TYPE_O] = p0 if p_out is None else p_out
+    S = t == TYPE_S
+    fl["T"][S] = T_wall
+    fl["u"][S & (j == shape[1] - 1)] = -0.3 * U
+    fl["rho"][:] = fl["p"] / (R * fl["T"])
+    return fl
+
+
 def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.0 / 3.0, T0=1.0, p0=1.0 / 3.0,
                         gamma=1.4, Pr=0.71, nu=None, **_unused):
     """Thermal compressible decaying turbulence on an N^3 periodic box.

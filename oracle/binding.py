@@ -31,7 +31,8 @@ def lib():
         L.orc_geq_moment.argtypes = [ctypes.c_int, ctypes.c_double, d3, d3, ctypes.c_double]
         L.orc_gstr.restype = ctypes.c_double
         L.orc_gstr.argtypes = [ctypes.c_int, ctypes.c_double, d3]
-        for fn in ("orc_init_single", "orc_collide_single", "orc_stream_single", "orc_moments_single"):
+        for fn in ("orc_init_single", "orc_collide_single", "orc_stream_single", "orc_moments_single",
+                   "orc_tms_single"):
             getattr(L, fn).restype = None
             getattr(L, fn).argtypes = [ctypes.c_void_p]
         L.orc_diag_single.restype = None
@@ -122,7 +123,7 @@ class SingleState:
         self._call("orc_stream_single")
 
     def tms_bc(self):
-        pass  # no single-component boundary cells are handled by the oracle yet
+        self._call("orc_tms_single")
 
     def moments(self):
         self._call("orc_moments_single")

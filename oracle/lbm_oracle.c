@@ -286,6 +286,131 @@ void orc_moments_single(orc_single* s)
     }
 }
 
+/* One TMS_BC reconstruction of cell c (lbm-bc.cpp:392-455, 869-935,
+ * 1164-1225): the unknown directions (upstream cell of type `kind`) are
+ * replaced by the boundary set {fb, gb} in the local moments, and every
+ * direction is corrected by the equilibrium difference
+ *   f <- f_tgt + f - f_loc (known),   f <- f_tgt + f_b - f_loc (unknown).
+ * The outflow and solid sweeps use f_tgt = f_b; the inflow sweep uses the
+ * equilibrium of the local density at the inflow velocity and temperature. */
+static void tms_correct(orc_single* s, int i, int j, int k, int kind, const double fb[27], const double gb[27],
+                        const double* ftgt, const double* gtgt)
+{
+    size_t N = ncell(s), c = cell(s, i, j, k);
+    double cv = s->R / (s->gamma - 1.0);
+    double rho = 0, mx = 0, my = 0, mz = 0, re = 0;
+    for (int l = 0; l < 27; ++l) {
+        int up = s->type[cell(s, i - CX[l], j - CY[l], k - CZ[l])] == kind;
+        double a = up ? fb[l] : s->f[l * N + c];
+        double b = up ? gb[l] : s->g[l * N + c];
+        rho += a;
+        mx += a * CX[l];
+        my += a * CY[l];
+        mz += a * CZ[l];
+        re += b;
+    }
+    double vel[3] = {mx / rho, my / rho, mz / rho}, corr[3] = {0, 0, 0};
+    double U = re / rho - 0.5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+    double T = U / cv, theta = s->R * T, Ucv = cv * T;
+    for (int l = 0; l < 27; ++l) {
+        double floc = orc_feq(l, rho, vel, theta, corr);
+        double gloc = orc_geq(l, rho, Ucv, theta, vel);
+        int up = s->type[cell(s, i - CX[l], j - CY[l], k - CZ[l])] == kind;
+        double ft = ftgt ? ftgt[l] : fb[l], gt = gtgt ? gtgt[l] : gb[l];
+        double fk = up ? fb[l] : s->f[l * N + c];
+        double gk = up ? gb[l] : s->g[l * N + c];
+        s->f[l * N + c] = ft + fk - floc;
+        s->g[l * N + c] = gt + gk - gloc;
+    }
+}
+
+/* calculate_feq_geq, lbm-edf.cpp:153-164. */
+static void feq_geq(const orc_single* s, double rho, const double vel[3], double T, double f[27], double g[27])
+{
+    double cv = s->R / (s->gamma - 1.0), corr[3] = {0, 0, 0};
+    for (int l = 0; l < 27; ++l) {
+        f[l] = orc_feq(l, rho, vel, s->R * T, corr);
+        g[l] = orc_geq(l, rho, cv * T, s->R * T, vel);
+    }
+}
+
+/* The first direction whose upstream cell has type `kind` and whose
+ * downstream cell is fluid (lbm-bc.cpp:357-368, 827-838), or -1. */
+static int tms_first(const orc_single* s, int i, int j, int k, int kind)
+{
+    for (int l = 0; l < 27; ++l)
+        if (s->type[cell(s, i - CX[l], j - CY[l], k - CZ[l])] == kind &&
+            s->type[cell(s, i + CX[l], j + CY[l], k + CZ[l])] == ORC_F)
+            return l;
+    return -1;
+}
+
+/* LBM::TMS_BC single-component (lbm-bc.cpp:341-460 outflow, 809-938 inflow,
+ * 1111-1230 isothermal wall), in the reference's sweep order. */
+void orc_tms_single(orc_single* s)
+{
+    double fb[27], gb[27], ft[27], gt[27];
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                if (s->type[cell(s, i, j, k)] != ORC_F) continue;
+                int l = tms_first(s, i, j, k, ORC_O);
+                if (l < 0) continue;
+                /* velocity and temperature downstream, pressure of the ghost */
+                size_t dn = cell(s, i + CX[l], j + CY[l], k + CZ[l]), up = cell(s, i - CX[l], j - CY[l], k - CZ[l]);
+                double vel[3] = {s->u[dn], s->v[dn], s->w[dn]}, T = s->T[dn];
+                double rho = s->p[up] / (s->R * T);
+                feq_geq(s, rho, vel, T, fb, gb);
+                tms_correct(s, i, j, k, ORC_O, fb, gb, NULL, NULL);
+            }
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                if (s->type[cell(s, i, j, k)] != ORC_F) continue;
+                int l = tms_first(s, i, j, k, ORC_I);
+                if (l < 0) continue;
+                /* velocity and temperature of the ghost, pressure downstream */
+                size_t dn = cell(s, i + CX[l], j + CY[l], k + CZ[l]), up = cell(s, i - CX[l], j - CY[l], k - CZ[l]);
+                double vel[3] = {s->u[up], s->v[up], s->w[up]}, T = s->T[up];
+                double rho = s->p[dn] / (s->R * T);
+                feq_geq(s, rho, vel, T, fb, gb);
+                /* target: the local density (the moments with f_in in the
+                 * unknown directions) at the inflow velocity and temperature */
+                size_t N = ncell(s), c = cell(s, i, j, k);
+                double rl = 0;
+                for (int m = 0; m < 27; ++m)
+                    rl += s->type[cell(s, i - CX[m], j - CY[m], k - CZ[m])] == ORC_I ? fb[m] : s->f[m * N + c];
+                feq_geq(s, rl, vel, T, ft, gt);
+                tms_correct(s, i, j, k, ORC_I, fb, gb, ft, gt);
+            }
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                size_t N = ncell(s), c = cell(s, i, j, k);
+                if (s->type[c] != ORC_F) continue;
+                int nd = 0;
+                double vel[3] = {0, 0, 0}, T = 0, q = 0.5;
+                for (int l = 0; l < 27; ++l) {
+                    size_t up = cell(s, i - CX[l], j - CY[l], k - CZ[l]), dn = cell(s, i + CX[l], j + CY[l], k + CZ[l]);
+                    if (s->type[up] != ORC_S || s->type[dn] != ORC_F) continue;
+                    ++nd;
+                    /* wall value interpolated at q = 1/2 between the cell
+                     * downstream and the wall cell (lbm-bc.cpp:1152-1158) */
+                    vel[0] += (q * s->u[dn] + s->u[up]) / (1.0 + q);
+                    vel[1] += (q * s->v[dn] + s->v[up]) / (1.0 + q);
+                    vel[2] += (q * s->w[dn] + s->w[up]) / (1.0 + q);
+                    T += (q * s->T[dn] + s->T[up]) / (1.0 + q);
+                }
+                if (nd == 0) continue;
+                double rbb = 0;
+                for (int l = 0; l < 27; ++l) rbb += s->f[l * N + c];
+                for (int d = 0; d < 3; ++d) vel[d] = 1.0 / nd * vel[d];
+                T = 1.0 / nd * T;
+                feq_geq(s, rbb, vel, T, fb, gb);
+                tms_correct(s, i, j, k, ORC_S, fb, gb, NULL, NULL);
+            }
+}
+
 /* Kinetic energy and enstrophy as output.cpp:212-276 defines them, plus the
  * conserved sums. */
 void orc_diag_single(const orc_single* s, double out[4])

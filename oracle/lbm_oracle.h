@@ -50,6 +50,8 @@ void orc_init_single(orc_single* s);
 void orc_collide_single(orc_single* s);
 void orc_stream_single(orc_single* s);
 void orc_moments_single(orc_single* s);
+/* TMS_BC: outflow (TYPE_O), inflow (TYPE_I), isothermal wall (TYPE_S) */
+void orc_tms_single(orc_single* s);
 /* out = {sum rho|u|^2, enstrophy, sum rho, sum rhoe} over TYPE_F cells */
 void orc_diag_single(const orc_single* s, double out[4]);
 

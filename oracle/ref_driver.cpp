@@ -16,11 +16,16 @@
 //                                     way setup.cpp:94-148 does (ghost-inclusive
 //                                     index, TYPE_P ghosts), Init(), then steps
 //   time <N> <nsteps> ...same params  time nsteps of the step loop (CPU baseline)
+//   channel <nx> <ny> <nz> <nsteps> <snaps> <outdir> <U> <nu> <R> <gamma> <Pr> <T0> <p0> <Tin> <Tw> <pout>
+//                                     inflow/outflow/wall channel (every single-
+//                                     component TMS_BC branch); a negative snap
+//                                     -s dumps step s before TMS_BC
 #include "lbm.hpp"
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <cstdint>
+#include <algorithm>
 #include <string>
 #include <vector>
 #include <chrono>
@@ -212,6 +217,88 @@ static int mode_tgv(int argc, char** argv)
     return 0;
 }
 
+// Channel with every single-component TMS_BC branch: TYPE_I inflow ghost at
+// i = 0 (u_in, T_in), TYPE_O outflow ghost at i = Nx-1 (p_out), TYPE_S
+// isothermal walls at j = 0 (at rest) and j = Ny-1 (moving with -0.3 u_in),
+// both at T_wall, periodic in z; the x faces are typed last.  The inflow
+// ghost carries a small transverse velocity 0.05 u_in.  Interior: u_in with
+// small transverse waves in v and w, T0, p0.
+struct ChannelParams { int nx, ny, nz; double U, nu, R, gamma, Pr, T0, p0, Tin, Tw, pout; };
+
+static void setup_channel(LBM& lb, const ChannelParams& P)
+{
+    lb.set_gasconst(P.R);
+    lb.set_gamma(P.gamma);
+    lb.set_prtl(P.Pr);
+    int Nx = lb.get_Nx(), Ny = lb.get_Ny(), Nz = lb.get_Nz();
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < Ny; ++j)
+            for (int k = 0; k < Nz; ++k) {
+                MIXTURE& m = lb.mixture[i][j][k];
+                if (k == 0 || k == Nz - 1) m.type = TYPE_P;
+                if (j == 0 || j == Ny - 1) m.type = TYPE_S;
+                if (i == 0) m.type = TYPE_I;
+                if (i == Nx - 1) m.type = TYPE_O;
+                m.u = 0.0; m.v = 0.0; m.w = 0.0;
+                m.temp = P.T0;
+                m.p = P.p0;
+                if (m.type == TYPE_F) {
+                    m.u = P.U;
+                    m.v = 0.1 * P.U * sin(2.0 * M_PI * i / (Nx - 2)) * sin(M_PI * j / (Ny - 1));
+                    m.w = 0.05 * P.U * cos(2.0 * M_PI * (k - 1) / (Nz - 2)) * sin(M_PI * j / (Ny - 1));
+                } else if (m.type == TYPE_I) {
+                    m.u = P.U;
+                    m.v = 0.05 * P.U;
+                    m.temp = P.Tin;
+                } else if (m.type == TYPE_O) {
+                    m.p = P.pout;
+                } else if (m.type == TYPE_S) {
+                    m.temp = P.Tw;
+                    if (j == Ny - 1) m.u = -0.3 * P.U;  // moving upper wall
+                }
+                m.rho = m.p / (P.R * m.temp);
+            }
+}
+
+// channel <nx> <ny> <nz> <nsteps> <snap csv> <outdir> <U> <nu> <R> <gamma> <Pr> <T0> <p0> <Tin> <Tw> <pout>
+static int mode_channel(int argc, char** argv)
+{
+    if (argc < 17) return 2;
+    ChannelParams P;
+    P.nx = atoi(argv[2]); P.ny = atoi(argv[3]); P.nz = atoi(argv[4]);
+    int nsteps = atoi(argv[5]);
+    std::vector<int> snaps;
+    {
+        std::string s(argv[6]);
+        size_t pos = 0;
+        while (pos < s.size()) {
+            size_t e = s.find(',', pos);
+            if (e == std::string::npos) e = s.size();
+            snaps.push_back(atoi(s.substr(pos, e - pos).c_str()));
+            pos = e + 1;
+        }
+    }
+    std::string outdir(argv[7]);
+    P.U = atof(argv[8]); P.nu = atof(argv[9]); P.R = atof(argv[10]); P.gamma = atof(argv[11]);
+    P.Pr = atof(argv[12]); P.T0 = atof(argv[13]); P.p0 = atof(argv[14]); P.Tin = atof(argv[15]);
+    P.Tw = atof(argv[16]); P.pout = argc > 17 ? atof(argv[17]) : P.p0;
+    LBM lb(P.nx, P.ny, P.nz, P.nu);
+    setup_channel(lb, P);
+    lb.Init();
+    for (int s : snaps) if (s == 0) dump_snapshot(lb, outdir + "/snap_0.bin");
+    for (int step = 1; step <= nsteps; ++step) {
+        lb.Collide();
+        lb.Streaming();
+        if (std::find(snaps.begin(), snaps.end(), -step) != snaps.end())  // state before TMS_BC
+            dump_snapshot(lb, outdir + "/pre_" + std::to_string(step) + ".bin");
+        lb.TMS_BC();
+        lb.calculate_moment();
+        for (int s : snaps)
+            if (s == step) dump_snapshot(lb, outdir + "/snap_" + std::to_string(step) + ".bin");
+    }
+    return 0;
+}
+
 static int mode_time(int argc, char** argv)
 {
     TgvParams P;
@@ -242,5 +329,6 @@ int main(int argc, char** argv)
     if (!strcmp(argv[1], "fn")) return mode_fn(argc, argv);
     if (!strcmp(argv[1], "tgv")) return mode_tgv(argc, argv);
     if (!strcmp(argv[1], "time")) return mode_time(argc, argv);
+    if (!strcmp(argv[1], "channel")) return mode_channel(argc, argv);
     return 2;
 }

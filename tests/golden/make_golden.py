@@ -19,6 +19,11 @@ Fixtures
   tgv64_series.npz   64^3 box (the 64^3 Re=1600 Ma=0.1 configuration): KE,
                      enstrophy, mass, energy every step up to 140 (the
                      reference diverges to NaN in this window)
+  channel.npz        10x5x3 channel with a fixed-state inflow ghost (TYPE_I),
+                     an outflow ghost (TYPE_O) and isothermal walls (TYPE_S,
+                     one moving), periodic in z: interior moments after
+                     steps 0, 1, 2, 10, 40, 100; populations after steps 1,
+                     2, 10; and the state of step 2 before TMS_BC
 """
 import math
 import os
@@ -60,6 +65,21 @@ def run_tgv(p, nsteps, snaps, env):
         return series, out
 
 
+CHANNEL = dict(nx=10, ny=5, nz=3, U=0.05, nu=0.02, R=R_GAS, gamma=GAMMA, Pr=PR, T0=1.0, p0=1.0 / 3.0,
+               Tin=1.1, Tw=0.9, pout=0.999 / 3.0)
+CHANNEL_KEYS = ("nx", "ny", "nz", "U", "nu", "R", "gamma", "Pr", "T0", "p0", "Tin", "Tw", "pout")
+
+
+def run_channel(p, nsteps, snaps, env):
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [DRIVER, "channel", str(p["nx"]), str(p["ny"]), str(p["nz"]), str(nsteps),
+               ",".join(map(str, snaps)), d] + [repr(p[k]) for k in CHANNEL_KEYS[3:]]
+        subprocess.run(cmd, check=True, env=env, stdout=subprocess.DEVNULL)
+        n = p["nx"] * p["ny"] * p["nz"]
+        return {s: np.fromfile(os.path.join(d, ("pre_%d" % -s if s < 0 else "snap_%d" % s) + ".bin")).reshape(n, 127)
+                for s in snaps}
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: oracle/build_ref.sh")
@@ -96,6 +116,16 @@ def main():
     series, _ = run_tgv(p64, 140, [], env)
     np.savez_compressed(os.path.join(HERE, "tgv64_series.npz"), series=series,
                         params=np.array([p64[k] for k in ("N", "U0", "nu", "R", "gamma", "Pr", "T0", "p0")]))
+    snaps = [0, 1, -2, 2, 10, 40, 100]
+    sn = run_channel(CHANNEL, 100, snaps, env)
+    arrays = {"params": np.array([CHANNEL[k] for k in CHANNEL_KEYS], np.float64)}
+    for s in snaps:
+        key = "pre2" if s < 0 else str(s)
+        arrays[f"mom_{key}"] = sn[s][:, :7]
+        if s in (1, -2, 2, 10):
+            arrays[f"f_{key}"] = sn[s][:, 19:46]
+            arrays[f"g_{key}"] = sn[s][:, 46:73]
+    np.savez_compressed(os.path.join(HERE, "channel.npz"), **arrays)
     print("golden fixtures written to", HERE)
 
 

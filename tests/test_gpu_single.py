@@ -27,6 +27,9 @@ from oracle import binding as ob
 pytestmark = pytest.mark.gpu
 
 SCAL = ["rho", "u", "v", "w", "p", "T", "rhoe"]
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1])
 
 
 def relerr(a, b):
@@ -329,3 +332,89 @@ def test_fp32_populations_round_trip():
     f64 = interior(ref.get_populations(POP_F))
     # deviations from the rest state are stored in FP32: absolute error ~1e-8 of f
     assert np.max(np.abs(f32 - f64)) < 1e-8
+
+
+# ------------------------------------------------------------------ TMS_BC
+CH_KEYS = ("nx", "ny", "nz", "U", "nu", "R", "gamma", "Pr", "T0", "p0", "Tin", "Tw", "pout")
+
+
+def _tms_channel(params, precision=LBM_FP64):
+    p = dict(zip(CH_KEYS, params))
+    nx, ny, nz = int(p["nx"]), int(p["ny"]), int(p["nz"])
+    fl = cases.channel(nx, ny, nz, p["U"], T0=p["T0"], p0=p["p0"], T_in=p["Tin"], T_wall=p["Tw"], p_out=p["pout"],
+                       R=p["R"])
+    pr = dict(R=p["R"], gamma=p["gamma"], Pr=p["Pr"], nu=p["nu"])
+    lat = Lattice(nx, ny, nz, pr["nu"], gas_const=pr["R"], gamma=pr["gamma"], prtl=pr["Pr"], mode=LBM_MODE_STRICT,
+                  precision=precision, ref_rho=1.0, ref_T=p["T0"])
+    lat.load_case(fl)
+    return lat, oracle_from(fl, pr)
+
+
+def _channel_params(golden_dir):
+    return np.load(os.path.join(golden_dir, "channel.npz"))["params"]
+
+
+def test_tms_call_matches_oracle(golden_dir):
+    """One TMS_BC call on the same post-streaming state: outflow, inflow and
+    wall sweeps against the oracle's."""
+    lat, s = _tms_channel(_channel_params(golden_dir))
+    lat.Init()
+    s.init()
+    for _ in range(3):
+        lat.step(1)
+        s.step()
+    lat.Collide()
+    lat.Streaming()
+    s.collide()
+    s.stream()
+    assert relerr(interior(lat.get_populations(POP_F)), interior(s.f)) < 1e-12
+    f_before = lat.get_populations(POP_F)
+    lat.TMS_BC()
+    s.tms_bc()
+    f, g = lat.get_populations(POP_F), lat.get_populations(POP_G)
+    assert relerr(interior(f), interior(s.f)) < 1e-12
+    assert relerr(interior(g), interior(s.g)) < 1e-12
+    # the fluid cells next to a TYPE_I/O/S ghost changed, no other cell did
+    changed = np.any(f != f_before, axis=0)
+    t = s.type
+    nb = np.zeros(t.shape, bool)
+    for l in range(27):
+        nb |= np.isin(np.roll(t, (CX[l], CY[l], CZ[l]), axis=(0, 1, 2)), (ob.S, ob.I, ob.O))
+    assert changed.any() and np.array_equal(changed, nb & (t == ob.F))
+
+
+def test_channel_matches_reference_snapshots(golden_dir):
+    """The reference's own channel run (inflow, outflow, isothermal walls) at
+    steps 1, 2, 10, 40, 100: interior moments and populations."""
+    gd = np.load(os.path.join(golden_dir, "channel.npz"))
+    lat, _ = _tms_channel(gd["params"])
+    lat.Init()
+    step = 0
+    for target in (1, 2, 10, 40, 100):
+        lat.step(target - step)
+        step = target
+        ref = gd[f"mom_{target}"]
+        uscale = np.max(np.abs(ref[:, 1:4]))
+        for col, name in enumerate(SCAL):
+            got = interior(lat.get_field(name)).reshape(-1)
+            scale = uscale if 1 <= col <= 3 else np.max(np.abs(ref[:, col]))
+            assert np.max(np.abs(got - ref[:, col])) / scale < 1e-10, (target, name)
+        if f"f_{target}" in gd:
+            f = interior(lat.get_populations(POP_F)).reshape(27, -1).T
+            g = interior(lat.get_populations(POP_G)).reshape(27, -1).T
+            assert relerr(f, gd[f"f_{target}"]) < 1e-10 and relerr(g, gd[f"g_{target}"]) < 1e-10
+
+
+def test_channel_fp32_tracks_fp64(golden_dir):
+    """FP32 shifted storage through the TMS boundaries (reconstruction in
+    FP64): 100 steps of the channel stay within 1e-5 of the FP64 run."""
+    params = _channel_params(golden_dir)
+    a, _ = _tms_channel(params, LBM_FP64)
+    b, _ = _tms_channel(params, LBM_FP32_SHIFTED)
+    for lat in (a, b):
+        lat.Init()
+        lat.step(100)
+    for name in ("rho", "u", "v", "T", "p"):
+        ref = interior(a.get_field(name))
+        scale = max(np.max(np.abs(interior(a.get_field("u")))), 1e-300) if name in ("u", "v") else np.max(np.abs(ref))
+        assert np.max(np.abs(interior(b.get_field(name)) - ref)) / scale < 1e-5, name
