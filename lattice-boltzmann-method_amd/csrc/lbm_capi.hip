@@ -79,6 +79,7 @@ struct lbm_ctx {
     int* sbc_cell = nullptr;
     int sbc_count = 0;
     double* src = nullptr;  // [NS][n] chemistry mass sources
+    void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
     double mfref[mech::NS * Q] = {}, mgref[Q] = {};
     double* mref_dev = nullptr;  // [NS*27 + 27]
@@ -203,6 +204,7 @@ MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
     v.reaction = c->cfg.reaction;
     v.tau_count = c->tau_count;
     v.src = c->src;
+    v.tsp = static_cast<P*>(c->tsp);
     v.tchem = c->tchem;
     v.step = c->step;
     if (c->capturing) {
@@ -351,7 +353,7 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell};
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -475,6 +477,9 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         const size_t srcb = sizeof(double) * (size_t)n * c->ns;
         if ((e = hipMalloc((void**)&c->src, srcb)) != hipSuccess) return bail("hipMalloc chemistry sources", e);
         (void)hipMemsetAsync(c->src, 0, srcb, c->stream);
+        const size_t tspb = c->esize * (size_t)n * c->ns * 4;
+        if ((e = hipMalloc(&c->tsp, tspb)) != hipSuccess) return bail("hipMalloc species transport", e);
+        c->bytes += tspb;
         if (fp32(c)) {
             if ((e = hipMalloc((void**)&c->mref_dev, sizeof(double) * (mech::NS + 1) * Q)) != hipSuccess)
                 return bail("hipMalloc", e);

@@ -26,6 +26,8 @@ struct MultiView {
     const double* fref;  // FP32 storage: rest-state shifts [NS][27] of the species populations
     const double* gref;  // and [27] of the energy populations (unused for FP64)
     double* src;    // [NS][n] species mass added by the chemistry over the step (lattice units)
+    P* tsp;         // [4][NS][n] Collide_Species transport of the step: omega_a, then the
+                    // Maxwell-Stefan friction impulse per axis
     double* tchem;  // [n] temperature after the chemistry [K] (surrogate audit only)
     // chemistry surrogate (all null when unused)
     unsigned step;                 // step counter for the sampling hashes

@@ -434,23 +434,26 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const un
 }
 
 // ------------------------------------------------------- Collide_Species
+// Two kernels.  The transport pass couples the species of a cell: species
+// viscosities, Wilke weights and omega_a (lbm-collision.cpp:405-429), binary
+// diffusivities and the Maxwell-Stefan friction impulse (:433-470); it stores
+// omega_a and the impulse per species in v.tsp.  The collide pass then runs
+// one thread per (cell, species) (:452-520) with a small register footprint;
+// its per-species inputs are the species' own planes.
 template <typename P>
-__global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
+__global__ __launch_bounds__(BLOCK) void k_mc_species_transport(MultiView<P> v)
 {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
     const int64_t n = v.d.n;
-    const Ijk x = ijk(v.d, c);
     double* const* Fd = v.fld;
     const double rho = Fd[LBM_FIELD_RHO][c], T = Fd[LBM_FIELD_T][c], p = Fd[LBM_FIELD_P][c];
-    const double umix[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
-    double ra[NS], Xa[NS], Y0[NS], src[NS];
+    double ra[NS], Xa[NS], Y0[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
         ra[a] = v.rhoa[a * n + c];
         Xa[a] = v.X[a * n + c];
         Y0[a] = ra[a] / rho;
-        src[a] = v.reaction ? v.src[a * n + c] : 0.0;
     }
     normalize(Y0);
     const double Tsi = v.u.si_temp(T);
@@ -475,75 +478,89 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
             den[a] += (R)Xa[b] * pab;
             if (b != a) den[b] += (R)Xa[a] * (mu[b] / mu[a] * (R)W[a] / (R)W[b] * pab);
         }
-    double om[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
         const double visc = Xa[a] * (double)v.u.mu(mu[a]) / (double)den[a];
-        om[a] = 2 * Xa[a] * p / (Xa[a] * p + 2 * visc);
+        v.tsp[a * n + c] = P(2 * Xa[a] * p / (Xa[a] * p + 2 * visc));
     }
     const double p_si = pressure(Tsi, rho_si, Y0);
     const R p_sir = (R)p_si;
+    for (int a = 0; a < NS; ++a) {
+        // Maxwell-Stefan friction on species a with the binary diffusivities of the state
+        double acc[3] = {0, 0, 0};
+        const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
+                     uaz = v.ua[(2 * NS + a) * n + c];
+        for (int b = 0; b < NS; ++b) {
+            const double D = v.u.nu(binary_diff_r<R>(a, b, lnTr, p_sir));
+            const double k = -1.0 * p * Xa[a] * Xa[b] / D;
+            acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
+            acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
+            acc[2] += k * (uaz - v.ua[(2 * NS + b) * n + c]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v.tsp[((1 + d) * NS + a) * n + c] = P(acc[d] * 1.0 / ra[a]);
+    }
+}
+
+// grid: (xcd_grid(n), NS); species a = blockIdx.y
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
+{
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    const int a = blockIdx.y;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    double* const* Fd = v.fld;
+    const double T = Fd[LBM_FIELD_T][c];
+    const double umix[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
+    const double ra = v.rhoa[a * n + c];
+    const double src = v.reaction ? v.src[a * n + c] : 0.0;
+    const double om = v.tsp[a * n + c];
+    const double Fa[3] = {v.tsp[(NS + a) * n + c], v.tsp[(2 * NS + a) * n + c], v.tsp[(3 * NS + a) * n + c]};
     const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
     const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
     const double* Tf = Fd[LBM_FIELD_T];
-    for (int a = 0; a < NS; ++a) {
-        // Maxwell-Stefan friction on species a with the binary diffusivities of the state
-        double Fa[3];
-        {
-            double acc[3] = {0, 0, 0};
-            const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
-                         uaz = v.ua[(2 * NS + a) * n + c];
-            for (int b = 0; b < NS; ++b) {
-                const double D = v.u.nu(binary_diff_r<R>(a, b, lnTr, p_sir));
-                const double k = -1.0 * p * Xa[a] * Xa[b] / D;
-                acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
-                acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
-                acc[2] += k * (uaz - v.ua[(2 * NS + b) * n + c]);
-            }
+    P fl[Q];
+    double mom[3] = {0, 0, 0};
 #pragma unroll
-            for (int d = 0; d < 3; ++d) Fa[d] = acc[d] * 1.0 / ra[a];
-        }
-        P fl[Q];
-        double mom[3] = {0, 0, 0};
+    for (int l = 0; l < Q; ++l) {
+        fl[l] = spop(v.f, a, l, n, c);
+        const double f = (double)fl[l] + fshift(v, a, l);
+        if (cx(l)) mom[0] += cx(l) * f;
+        if (cy(l)) mom[1] += cy(l) * f;
+        if (cz(l)) mom[2] += cz(l) * f;
+    }
+    const double uraw[3] = {mom[0] / ra, mom[1] / ra, mom[2] / ra};
+    const double udu[3] = {uraw[0] + Fa[0], uraw[1] + Fa[1], uraw[2] + Fa[2]};
+    const double Ra = v.u.cp(R_UNIV / W[a]);
+    const double theta = Ra * T;
+    double dQ[3];
 #pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            fl[l] = spop(v.f, a, l, n, c);
-            const double f = (double)fl[l] + fshift(v, a, l);
-            if (cx(l)) mom[0] += cx(l) * f;
-            if (cy(l)) mom[1] += cy(l) * f;
-            if (cz(l)) mom[2] += cz(l) * f;
-        }
-        const double uraw[3] = {mom[0] / ra[a], mom[1] / ra[a], mom[2] / ra[a]};
-        const double udu[3] = {uraw[0] + Fa[0], uraw[1] + Fa[1], uraw[2] + Fa[2]};
-        const double Ra = v.u.cp(R_UNIV / W[a]);
-        const double theta = Ra * T;
-        double dQ[3];
+    for (int d = 0; d < 3; ++d) {
+        auto Qf = [&](int m) {
+            const double r = v.rhoa[a * n + m], ud = v.ua[(d * NS + a) * n + m];
+            return r * ud * (1.0 - 3.0 * Ra * Tf[m]) - r * (ud * ud * ud);
+        };
+        dQ[d] = fd_central<double>(Qf(nm[d]), Qf(c), Qf(np[d]), v.type[nm[d]], v.type[np[d]]);
+    }
+    double corr[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            auto Qf = [&](int m) {
-                const double r = v.rhoa[a * n + m], ud = v.ua[(d * NS + a) * n + m];
-                return r * ud * (1.0 - 3.0 * Ra * Tf[m]) - r * (ud * ud * ud);
-            };
-            dQ[d] = fd_central<double>(Qf(nm[d]), Qf(c), Qf(np[d]), v.type[nm[d]], v.type[np[d]]);
-        }
-        double corr[3];
+    for (int d = 0; d < 3; ++d) corr[d] = 1 * (2.0 - om) / (2.0 * ra * om) * dQ[d];
+    const double zero[3] = {0, 0, 0};
+    Axis3<double> Fe[3], Fu[3], Fr[3];
+    feq_axes<double>(Fe, uraw, theta, corr);
+    feq_axes<double>(Fu, udu, theta, corr);
+    feq_axes<double>(Fr, umix, theta, zero);
+    const bool low = ra <= SPECIES_MIN;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) corr[d] = 1 * (2.0 - om[a]) / (2.0 * ra[a] * om[a]) * dQ[d];
-        const double zero[3] = {0, 0, 0};
-        Axis3<double> Fe[3], Fu[3], Fr[3];
-        feq_axes<double>(Fe, uraw, theta, corr);
-        feq_axes<double>(Fu, udu, theta, corr);
-        feq_axes<double>(Fr, umix, theta, zero);
-        const bool low = ra[a] <= SPECIES_MIN;
-#pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            const double fe = feq_dir<double>(Fe, l, ra[a]);
-            const double fdu = feq_dir<double>(Fu, l, ra[a]);
-            const double fr = feq_dir<double>(Fr, l, src[a]);
-            if (low && fr <= 0.0) continue;  // post-collision value left as it was
-            const double f = (double)fl[l] + fshift(v, a, l);
-            stf(v, v.fo, a, l, c, f + om[a] * (fe - f) + (fdu - fe) + fr);
-        }
+    for (int l = 0; l < Q; ++l) {
+        const double fe = feq_dir<double>(Fe, l, ra);
+        const double fdu = feq_dir<double>(Fu, l, ra);
+        const double fr = feq_dir<double>(Fr, l, src);
+        if (low && fr <= 0.0) continue;  // post-collision value left as it was
+        const double f = (double)fl[l] + fshift(v, a, l);
+        stf(v, v.fo, a, l, c, f + om * (fe - f) + (fdu - fe) + fr);
     }
 }
 
@@ -657,7 +674,7 @@ template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
 {
     // one thread per (cell, population set): blockIdx.y = species 0..NS-1, NS = g
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
     const int set = blockIdx.y;
     const int64_t n = v.d.n;
@@ -694,7 +711,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
 }
 
 // ------------------------------------------------------- calculate_moment
-// Dense symmetric positive definite solve A x = b (Cholesky, lower factor).
+// Dense symmetric positive definite solve A x = b (Cholesky, lower factor);
+// A is overwritten by the factor, B holds the three right-hand sides.
 __device__ inline void cholesky_solve3(double A[NS][NS], double B[NS][3])
 {
 #pragma unroll
@@ -731,8 +749,54 @@ __device__ inline void cholesky_solve3(double A[NS][NS], double B[NS][3])
     }
 }
 
+// The same solve with A held as its lower triangle, L[tri(i, j)] = A[i][j]
+// for j <= i (36 fewer live doubles); same operations in the same order.
+constexpr int NTRI = NS * (NS + 1) / 2;
+__host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+__device__ inline void cholesky_solve3(double L[NTRI], double B[NS][3])
+{
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double s = L[tri(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s -= L[tri(j, k)] * L[tri(j, k)];
+        const double d = sqrt(s);
+        L[tri(j, j)] = d;
+#pragma unroll
+        for (int i = j + 1; i < NS; ++i) {
+            double t = L[tri(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t -= L[tri(i, k)] * L[tri(j, k)];
+            L[tri(i, j)] = t / d;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            double t = B[i][r];
+#pragma unroll
+            for (int k = 0; k < i; ++k) t -= L[tri(i, k)] * B[k][r];
+            B[i][r] = t / L[tri(i, i)];
+        }
+#pragma unroll
+        for (int i = NS - 1; i >= 0; --i) {
+            double t = B[i][r];
+#pragma unroll
+            for (int k = i + 1; k < NS; ++k) t -= L[tri(k, i)] * B[k][r];
+            B[i][r] = t / L[tri(i, i)];
+        }
+    }
+}
+
+// The FP64 kernel holds more live doubles than fit two waves per SIMD: it
+// runs at one wave per SIMD with the accumulation registers and no scratch
+// spills (2.9x faster than the spilling default); the FP32 one keeps the
+// compiler's default (two waves per SIMD, some spills), which measured faster
+// than forcing either no spills or one wave.
 template <typename P>
-__global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof(P) == 8 ? 1 : 8))) void k_mc_moments(MultiView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
@@ -793,26 +857,50 @@ __global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
     }
     const double vel[3] = {m0 / rho, m1 / rho, m2 / rho};
     // Maxwell-Stefan velocities: A u = (rho u)_a with the old state's D, p, X
-    double A[NS][NS];
+    // (storage by type: measured, the FP64 kernel runs best with the packed
+    // triangle at one wave per SIMD, the FP32 one with the full matrix at two)
+    if constexpr (sizeof(P) == 8) {
+        double A[NTRI];  // lower triangle; A[a][b] = A[b][a] = A[tri(max, min)]
 #pragma unroll
-    for (int a = 0; a < NS; ++a)
+        for (int a = 0; a < NS; ++a)
 #pragma unroll
-        for (int b = 0; b < NS; ++b) {
-            if (b < a) continue;
-            if (b == a) {
-                A[a][a] = rr[a] == 0.0 ? 1.0 : rr[a];
-            } else {
-                const double D = v.u.nu(binary_diff(a, b, lnT_old, p_old_si));
-                A[a][b] = -1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D;
-                A[b][a] = A[a][b];
+            for (int b = 0; b < NS; ++b) {
+                if (b < a) continue;
+                if (b == a) {
+                    A[tri(a, a)] = rr[a] == 0.0 ? 1.0 : rr[a];
+                } else {
+                    const double D = v.u.nu(binary_diff(a, b, lnT_old, p_old_si));
+                    A[tri(b, a)] = -1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D;
+                }
             }
-        }
 #pragma unroll
-    for (int a = 0; a < NS; ++a)
+        for (int a = 0; a < NS; ++a)
 #pragma unroll
-        for (int b = 0; b < NS; ++b)
-            if (b != a) A[a][a] -= A[a][b];
-    cholesky_solve3(A, B);
+            for (int b = 0; b < NS; ++b)
+                if (b != a) A[tri(a, a)] -= A[b > a ? tri(b, a) : tri(a, b)];
+        cholesky_solve3(A, B);
+    } else {
+        double A[NS][NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a)
+#pragma unroll
+            for (int b = 0; b < NS; ++b) {
+                if (b < a) continue;
+                if (b == a) {
+                    A[a][a] = rr[a] == 0.0 ? 1.0 : rr[a];
+                } else {
+                    const double D = v.u.nu(binary_diff(a, b, lnT_old, p_old_si));
+                    A[a][b] = -1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D;
+                    A[b][a] = A[a][b];
+                }
+            }
+#pragma unroll
+        for (int a = 0; a < NS; ++a)
+#pragma unroll
+            for (int b = 0; b < NS; ++b)
+                if (b != a) A[a][a] -= A[a][b];
+        cholesky_solve3(A, B);
+    }
     // temperature from energy at the new composition
     double Y[NS];
 #pragma unroll
@@ -847,7 +935,6 @@ __global__ __launch_bounds__(BLOCK) void k_mc_moments(MultiView<P> v)
         for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = B[a][d];
     }
 }
-
 
 // ---------------------------------------------------------------- TMS_BC
 // Outflow sweep of TMS_BC (lbm-bc.cpp:126-340) for TYPE_O_C ghosts with the
@@ -1300,13 +1387,16 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
     case McKernel::Init: hipLaunchKernelGGL(k_mc_init<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     case McKernel::Chemistry: hipLaunchKernelGGL(k_mc_chemistry<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     case McKernel::CollideSpecies:
-        hipLaunchKernelGGL(k_mc_collide_species<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL(k_mc_collide_species<P>, dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
         break;
     case McKernel::Collide: hipLaunchKernelGGL(k_mc_collide<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     case McKernel::Stream:
-        hipLaunchKernelGGL(k_mc_stream<P>, dim3((unsigned)((v.d.n + BLOCK - 1) / BLOCK), NS + 1), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS + 1), dim3(BLOCK), 0, s, v);
         break;
-    case McKernel::Moments: hipLaunchKernelGGL(k_mc_moments<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::Moments:
+        hipLaunchKernelGGL(k_mc_moments<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        break;
     }
 }
 
