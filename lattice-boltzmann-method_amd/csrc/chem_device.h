@@ -18,6 +18,19 @@ namespace chem {
 using namespace mech;
 constexpr double SPECIES_MIN = 1e-8;  // defines.hpp:96
 
+// 1 / W_a, for products in place of divisions by the molar masses
+struct InvW {
+    double v[NS];
+};
+constexpr InvW make_inv_w()
+{
+    InvW r{};
+    for (int a = 0; a < NS; ++a) r.v[a] = 1.0 / W[a];
+    return r;
+}
+constexpr InvW INV_W_TABLE = make_inv_w();
+__host__ __device__ inline double inv_w(int a) { return INV_W_TABLE.v[a]; }
+
 // ---------------------------------------------------------------- units
 // Units_Conv after set_m_kg_s(dx, dt): m, kg, s, K stored as float; every
 // conversion rounds its argument to float and computes in float, or in

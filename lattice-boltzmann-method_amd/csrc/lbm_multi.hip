@@ -485,21 +485,35 @@ __global__ __launch_bounds__(BLOCK) void k_mc_species_transport(MultiView<P> v)
     }
     const double p_si = pressure(Tsi, rho_si, Y0);
     const R p_sir = (R)p_si;
-    for (int a = 0; a < NS; ++a) {
-        // Maxwell-Stefan friction on species a with the binary diffusivities of the state
-        double acc[3] = {0, 0, 0};
-        const double uax = v.ua[(0 * NS + a) * n + c], uay = v.ua[(1 * NS + a) * n + c],
-                     uaz = v.ua[(2 * NS + a) * n + c];
-        for (int b = 0; b < NS; ++b) {
-            const double D = v.u.nu(binary_diff_r<R>(a, b, lnTr, p_sir));
-            const double k = -1.0 * p * Xa[a] * Xa[b] / D;
-            acc[0] += k * (uax - v.ua[(0 * NS + b) * n + c]);
-            acc[1] += k * (uay - v.ua[(1 * NS + b) * n + c]);
-            acc[2] += k * (uaz - v.ua[(2 * NS + b) * n + c]);
+    // Maxwell-Stefan friction with the binary diffusivities of the state.  The
+    // fit table is symmetric, so D_ab = D_ba is evaluated once per pair; the
+    // b = a term, k (u_a - u_a), is zero and skipped.  Each acc[a] still sums
+    // its terms in the order b = 0, 1, ..., NS-1.
+    double ua[3][NS], acc[NS][3];
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ua[d][a] = v.ua[(d * NS + a) * n + c];
+            acc[a][d] = 0.0;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) v.tsp[((1 + d) * NS + a) * n + c] = P(acc[d] * 1.0 / ra[a]);
-    }
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int b = a + 1; b < NS; ++b) {
+            const double D = v.u.nu(binary_diff_r<R>(a, b, lnTr, p_sir));
+            const double kab = -1.0 * p * Xa[a] * Xa[b] / D;
+            const double kba = -1.0 * p * Xa[b] * Xa[a] / D;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                acc[a][d] += kab * (ua[d][a] - ua[d][b]);
+                acc[b][d] += kba * (ua[d][b] - ua[d][a]);
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v.tsp[((1 + d) * NS + a) * n + c] = P(acc[a][d] * 1.0 / ra[a]);
 }
 
 // grid: (xcd_grid(n), NS); species a = blockIdx.y
@@ -606,25 +620,37 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
         const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
         const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
         double qdiff[3] = {0, 0, 0}, qcorr[3] = {0, 0, 0};
+        // neighbour mass fractions through one reciprocal density per
+        // neighbour (a multiply instead of a division per species)
+        double irn[3], irp[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            irn[d] = 1.0 / Fd[LBM_FIELD_RHO][nm[d]];
+            irp[d] = 1.0 / Fd[LBM_FIELD_RHO][np[d]];
+        }
+        const double irho = 1.0 / rho;
         for (int a = 0; a < NS; ++a) {
-            const double ha = v.u.energy_mass(h_RT(a, Tsi) * R_UNIV * Tsi / W[a]);
+            const double ha = v.u.energy_mass(h_RT(a, Tsi) * R_UNIV * Tsi * inv_w(a));
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double uad = v.ua[(d * NS + a) * n + c];
                 qdiff[d] += om1 / (-om + om1) * ha * ra[a] * (uad - vel[d]);
-                const double Yn = v.rhoa[a * n + nm[d]] / Fd[LBM_FIELD_RHO][nm[d]];
-                const double Yp = v.rhoa[a * n + np[d]] / Fd[LBM_FIELD_RHO][np[d]];
-                const double dY = fd_fuw<double>(Yn, ra[a] / rho, Yp, uad - vel[d], v.type[nm[d]], v.type[np[d]]);
+                const double Yn = v.rhoa[a * n + nm[d]] * irn[d];
+                const double Yp = v.rhoa[a * n + np[d]] * irp[d];
+                const double dY = fd_fuw<double>(Yn, ra[a] * irho, Yp, uad - vel[d], v.type[nm[d]], v.type[np[d]]);
                 qcorr[d] += 0.5 * (2.0 - om1) / (-om + om1) * 1 * p * ha * dY;
             }
         }
+        // mixture gas constant of a neighbour, R_univ sum(Y/W) / sum(Y): the
+        // ratio is the same with rho_a in place of Y_a
         auto Rn = [&](int m) -> double {
             if (!fluidlike(v.type[m])) return 0.0;
             double sYW = 0, sY = 0;
+#pragma unroll
             for (int a = 0; a < NS; ++a) {
-                const double Ya = v.rhoa[a * n + m] / Fd[LBM_FIELD_RHO][m];
-                sYW += Ya / W[a];
-                sY += Ya;
+                const double ra_m = v.rhoa[a * n + m];
+                sYW += ra_m * inv_w(a);
+                sY += ra_m;
             }
             return v.u.cp(R_UNIV * sYW / sY);
         };
