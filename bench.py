@@ -69,7 +69,7 @@ def make_lattice(fl, pr, precision, mode):
 
 SINGLE_KERNELS = ("fused_collide_stream", "fused_moments", "collide_explicit")
 MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_fallback", "mc_collide_species",
-                 "mc_collide_mixture", "mc_stream", "mc_tms_outflow", "mc_tms_inflow", "mc_moments")
+                 "mc_collide_mixture", "mc_stream", "mc_tms_outflow", "mc_tms_inflow", "mc_tms_wall", "mc_moments")
 
 
 def timed(lat, steps, warmup, names=SINGLE_KERNELS):

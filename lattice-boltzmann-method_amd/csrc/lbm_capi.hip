@@ -71,9 +71,11 @@ struct lbm_ctx {
     int* tms_cell = nullptr;
     int* tms_dir = nullptr;
     int tms_count = 0;
-    int* tin_cell = nullptr;  // TMS_BC inflow interface cells (TYPE_I_C)
+    int* tin_cell = nullptr;  // TMS_BC inflow interface cells (TYPE_I, TYPE_I_C)
     int* tin_dir = nullptr;
     int tin_count = 0;
+    int* twall_cell = nullptr;  // TMS_BC isothermal-wall interface cells (TYPE_S)
+    int twall_count = 0;
     std::string tms_unsupported;
     // single-component TMS_BC interface cells (fluid cells next to TYPE_O/I/S)
     int* sbc_cell = nullptr;
@@ -353,7 +355,7 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp};
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -540,7 +542,7 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                 if (keep) needs_tms = true;
                 flag[cidx] = (simple ? NB_SIMPLE : 0) | (keep ? NB_KEEP : 0);
             }
-    std::vector<int> tcell, tdir, icell, idir;
+    std::vector<int> tcell, tdir, icell, idir, wcell;
     std::string tms_unsupported;
     if (multi(c)) {
         for (int64_t i = 0; i < d.n; ++i)
@@ -554,19 +556,14 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
             for (int j = 1; j < d.ny - 1; ++j)
                 for (int k = 1; k < d.nz - 1; ++k) {
                     if (at(i, j, k) != TYPE_F) continue;
-                    for (int l = 0; l < Q; ++l) {
-                        const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
-                        if (tn == TYPE_I || tn == TYPE_S)
-                            tms_unsupported = "fixed-state inflow (TYPE_I) and isothermal-wall (TYPE_S) TMS "
-                                              "reconstruction is not provided for the multicomponent state";
-                    }
+                    const int cidx = (int)(((int64_t)i * d.ny + j) * d.nz + k);
                     // the inflow sweep's interface detection (lbm-bc.cpp:479-494)
                     for (int l = 0; l < Q; ++l) {
                         const uint8_t tn = at(i - cx(l), j - cy(l), k - cz(l));
                         const uint8_t tp = at(i + cx(l), j + cy(l), k + cz(l));
                         if ((tn == TYPE_I || tn == TYPE_I_C) && tp == TYPE_F) {
-                            if (c2(l) == 1 && tn == TYPE_I_C) {
-                                icell.push_back((int)(((int64_t)i * d.ny + j) * d.nz + k));
+                            if (c2(l) == 1) {
+                                icell.push_back(cidx);
                                 idir.push_back(l);
                             }
                             break;
@@ -577,14 +574,18 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                         const uint8_t tp = at(i + cx(l), j + cy(l), k + cz(l));
                         if ((tn == TYPE_O || tn == TYPE_O_C) && tp == TYPE_F) {
                             if (c2(l) == 1) {
-                                if (tn == TYPE_O)
-                                    tms_unsupported = "TYPE_O (zero-gradient) outflow is not provided; use TYPE_O_C";
-                                tcell.push_back((int)(((int64_t)i * d.ny + j) * d.nz + k));
+                                tcell.push_back(cidx);
                                 tdir.push_back(l);
                             }
                             break;
                         }
                     }
+                    // the wall sweep's interface links (lbm-bc.cpp:957-967)
+                    for (int l = 0; l < Q; ++l)
+                        if (at(i - cx(l), j - cy(l), k - cz(l)) == TYPE_S && at(i + cx(l), j + cy(l), k + cz(l)) == TYPE_F) {
+                            wcell.push_back(cidx);
+                            break;
+                        }
                 }
     }
     // single-component TMS_BC interface cells: some direction with a TYPE_O,
@@ -634,6 +635,13 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
             HIP_TRY(c, hipMalloc((void**)&c->tin_dir, sizeof(int) * idir.size()));
             HIP_TRY(c, hipMemcpy(c->tin_cell, icell.data(), sizeof(int) * icell.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMemcpy(c->tin_dir, idir.data(), sizeof(int) * idir.size(), hipMemcpyHostToDevice));
+        }
+        if (c->twall_cell) (void)hipFree(c->twall_cell);
+        c->twall_cell = nullptr;
+        c->twall_count = (int)wcell.size();
+        if (c->twall_count) {
+            HIP_TRY(c, hipMalloc((void**)&c->twall_cell, sizeof(int) * wcell.size()));
+            HIP_TRY(c, hipMemcpy(c->twall_cell, wcell.data(), sizeof(int) * wcell.size(), hipMemcpyHostToDevice));
         }
         c->tms_unsupported = tms_unsupported;
         needs_tms = false;
@@ -935,13 +943,17 @@ int lbm_tms_bc(lbm_ctx* c)
     if (multi(c)) {
         if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
         if (!c->tms_unsupported.empty()) return fail(c, LBM_E_UNSUPPORTED, c->tms_unsupported);
-        // outflow sweep, then inflow sweep (lbm-bc.cpp:126, 463)
+        // outflow, inflow and isothermal-wall sweeps (lbm-bc.cpp:126, 463, 941)
         if (int rc = DISPATCH(c, launch(c, "mc_tms_outflow", [&] {
                 launch_mc_tms<P>(make_mview<P>(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
             })))
             return rc;
-        return DISPATCH(c, launch(c, "mc_tms_inflow", [&] {
-            launch_mc_tms_in<P>(make_mview<P>(c, 0, 0), c->tin_cell, c->tin_dir, c->tin_count, c->stream);
+        if (int rc = DISPATCH(c, launch(c, "mc_tms_inflow", [&] {
+                launch_mc_tms_in<P>(make_mview<P>(c, 0, 0), c->tin_cell, c->tin_dir, c->tin_count, c->stream);
+            })))
+            return rc;
+        return DISPATCH(c, launch(c, "mc_tms_wall", [&] {
+            launch_mc_tms_wall<P>(make_mview<P>(c, 0, 0), c->twall_cell, c->twall_count, c->stream);
         }));
     }
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");

@@ -64,6 +64,9 @@ void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int
 // after the outflow sweep).
 template <typename P>
 void launch_mc_tms_in(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
+// TMS_BC isothermal-wall sweep (TYPE_S) on the interface cells `cells`
+template <typename P>
+void launch_mc_tms_wall(const MultiView<P>& v, const int* cells, int count, hipStream_t s);
 
 // Surrogate chemistry of every fluid cell: features (BF16 [rows][16] in X,
 // envelope flags), network (BF16 MFMA, FP32 outputs [rows][16] in Y; rows >=

@@ -962,6 +962,99 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof
     }
 }
 
+// calculate_moment_point (lbm-moment.cpp:379-537) of cell c, writing its
+// fields.  Unlike calculate_moment: no SPECIES_MIN clamp; the binary
+// diffusivities at the T of the cell's fields and the new density and
+// composition; the Maxwell-Stefan matrix from the fields' p and X, not yet
+// updated; internal energy rho e / rho - (sum 1/2 rho_a |u_a|^2) / rho.  The
+// reference solves by partial-pivoting LU, this by Cholesky (the matrix is
+// symmetric positive definite).
+template <typename P>
+__device__ __noinline__ void mc_moment_point(const MultiView<P>& v, int c)
+{
+    const int64_t n = v.d.n;
+    double* const* Fd = v.fld;
+    double re = 0, qx = 0, qy = 0, qz = 0;
+    for (int l = 0; l < Q; ++l) {
+        const double g = ldg(v, v.g, l, c);
+        re += g;
+        if (cx(l)) qx += cx(l) * g;
+        if (cy(l)) qy += cy(l) * g;
+        if (cz(l)) qz += cz(l) * g;
+    }
+    Fd[LBM_FIELD_RHOE][c] = re;
+    Fd[LBM_FIELD_QX][c] = qx;
+    Fd[LBM_FIELD_QY][c] = qy;
+    Fd[LBM_FIELD_QZ][c] = qz;
+    double Pt[6] = {0, 0, 0, 0, 0, 0};
+    double ra[NS], B[NS][3];
+    double rho = 0, m0 = 0, m1 = 0, m2 = 0;
+    for (int a = 0; a < NS; ++a) {
+        double r = 0, mx = 0, my = 0, mz = 0;
+        for (int l = 0; l < Q; ++l) {
+            const double f = ldf(v, v.f, a, l, c);
+            r += f;
+            if (cx(l)) mx += cx(l) * f;
+            if (cy(l)) my += cy(l) * f;
+            if (cz(l)) mz += cz(l) * f;
+            if (cx(l)) Pt[0] += f;
+            if (cy(l)) Pt[1] += f;
+            if (cz(l)) Pt[2] += f;
+            if (cx(l) * cy(l)) Pt[3] += (cx(l) * cy(l)) * f;
+            if (cx(l) * cz(l)) Pt[4] += (cx(l) * cz(l)) * f;
+            if (cy(l) * cz(l)) Pt[5] += (cy(l) * cz(l)) * f;
+        }
+        ra[a] = r;
+        B[a][0] = mx;
+        B[a][1] = my;
+        B[a][2] = mz;
+        rho += r;
+        m0 += mx;
+        m1 += my;
+        m2 += mz;
+    }
+    for (int a = 0; a < 6; ++a) Fd[LBM_FIELD_PXX + a][c] = Pt[a];
+    Fd[LBM_FIELD_RHO][c] = rho;
+    Fd[LBM_FIELD_U][c] = m0 / rho;
+    Fd[LBM_FIELD_V][c] = m1 / rho;
+    Fd[LBM_FIELD_W][c] = m2 / rho;
+    double Y[NS];
+    for (int a = 0; a < NS; ++a) Y[a] = ra[a] / rho;
+    normalize(Y);
+    const double Tsi_old = v.u.si_temp(Fd[LBM_FIELD_T][c]);
+    const double rho_si = v.u.si_rho(rho);
+    const double p_si = pressure(Tsi_old, rho_si, Y);
+    const double lnT = log(Tsi_old);
+    const double p_fld = Fd[LBM_FIELD_P][c];
+    double A[NS][NS];
+    for (int a = 0; a < NS; ++a)
+        for (int b = 0; b < NS; ++b) {
+            if (b != a) {
+                const double D = v.u.nu(binary_diff(a, b, lnT, p_si));
+                A[a][b] = -1.0 * 1 / 2.0 * p_fld * v.X[a * n + c] * v.X[b * n + c] / D;
+            } else {
+                A[a][a] = ra[a] == 0.0 ? 1.0 : ra[a];
+            }
+        }
+    for (int a = 0; a < NS; ++a)
+        for (int b = 0; b < NS; ++b)
+            if (b != a) A[a][a] -= A[a][b];
+    cholesky_solve3(A, B);
+    double ke = 0;
+    for (int a = 0; a < NS; ++a) ke += 0.5 * ra[a] * (B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2]);
+    const double Uint = re / rho - ke / rho;
+    const double Tsi = T_from_UV(v.u.si_energy_mass(Uint), Y, Tsi_old);
+    double X[NS];
+    X_from_Y(Y, X);
+    for (int a = 0; a < NS; ++a) {
+        v.rhoa[a * n + c] = ra[a];
+        v.X[a * n + c] = X[a];
+        for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = B[a][d];
+    }
+    Fd[LBM_FIELD_T][c] = v.u.temp(Tsi);
+    Fd[LBM_FIELD_P][c] = v.u.p(pressure(Tsi, rho_si, Y));
+}
+
 // ---------------------------------------------------------------- TMS_BC
 // Outflow sweep of TMS_BC (lbm-bc.cpp:126-340) for TYPE_O_C ghosts with the
 // NSCBC target state of impose_nscbc.cpp:45-546.  One thread per interface
@@ -1045,6 +1138,27 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     const int idir = cl[0] ? 0 : (cl[1] ? 1 : 2);
     const int isign = cl[idir];
     double* const* Fd = v.fld;
+    const int64_t n = v.d.n;
+    const int cg = c - isign * strides[idir];  // the ghost
+    double rho_out, vout[3], ra_out[NS], Yo[NS], T_out;
+    if (v.type[cg] == TYPE_O) {
+        // TYPE_O target (lbm-bc.cpp:168-198): the moments of the cell
+        // downstream, recomputed by calculate_moment_point, the ghost's
+        // pressure, T from setState_DP
+        const int cd = c + isign * strides[idir];
+        mc_moment_point(v, cd);
+        rho_out = Fd[LBM_FIELD_RHO][cd];
+        vout[0] = Fd[LBM_FIELD_U][cd];
+        vout[1] = Fd[LBM_FIELD_V][cd];
+        vout[2] = Fd[LBM_FIELD_W][cd];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            ra_out[a] = v.rhoa[a * n + cd];
+            Yo[a] = ra_out[a] / rho_out;
+        }
+        double Tsi_o;
+        T_out = T_from_DP(v.u, rho_out, Fd[LBM_FIELD_P][cg], Yo, &Tsi_o);
+    } else {
     // ---- impose_NSCBC: LODI waves at the boundary cell
     double V0[NV];
     cell_vals(v, c, V0);
@@ -1100,19 +1214,18 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     }
     // update_bc_cells, dt = 1
     const double drho = (Ls + 0.5 * (L1 + L5)) / (a_s * a_s);
-    double vout[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) vout[d] = vel[d] - 1 * (d == idir ? (L5 - L1) / (2.0 * a_s * rho) : un * Dn[1 + d]);
-    const double rho_out = rho - 1 * drho;
+    rho_out = rho - 1 * drho;
     const double p_out = p - 1 * (0.5 * (L1 + L5));
-    double ra_out[NS], Yo[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
         ra_out[a] = (Yc[a] - 1 * (un * Dn[5 + a])) * rho_out;
         Yo[a] = ra_out[a] / rho_out;
     }
     double Tsi_out;
-    const double T_out = T_from_DP(v.u, rho_out, p_out, Yo, &Tsi_out);
+    T_out = T_from_DP(v.u, rho_out, p_out, Yo, &Tsi_out);
+    }
     // ---- equilibria of the target state (calculate_feq_geq, lbm-edf.cpp:198-209)
     const double Tso = v.u.si_temp(T_out);
     const double U_out = v.u.energy_mass(u_mass(Tso, Yo));
@@ -1223,6 +1336,25 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
     const int st = strides[idir];
     const int c1 = c + isign * st, cb = c - isign * st;
     double* const* Fd = v.fld;
+    double rho_in, vin[3], p_in, ra_in[NS], Yi[NS], T_in;
+    if (v.type[cb] == TYPE_I) {
+        // TYPE_I target (lbm-bc.cpp:506-518): velocity, temperature and
+        // composition of the ghost; the pressure of the cell downstream after
+        // calculate_moment_point
+        mc_moment_point(v, c1);
+        vin[0] = Fd[LBM_FIELD_U][cb];
+        vin[1] = Fd[LBM_FIELD_V][cb];
+        vin[2] = Fd[LBM_FIELD_W][cb];
+        T_in = Fd[LBM_FIELD_T][cb];
+        rho_in = Fd[LBM_FIELD_RHO][cb];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            ra_in[a] = v.rhoa[a * n + cb];
+            Yi[a] = ra_in[a] / rho_in;
+        }
+        normalize(Yi);
+        p_in = Fd[LBM_FIELD_P][c1];
+    } else {
     const double sigma = 5.0;
     // ---- inline LODI target
     const double rho = Fd[LBM_FIELD_RHO][c], p = Fd[LBM_FIELD_P][c];
@@ -1260,13 +1392,13 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
     const double d1 = 1.0 / (a_s * a_s) * (L2 + 0.5 * (L5 + L1));
     const double d2 = 1.0 / (2.0 * rho * a_s) * (L5 - L1);
     const double d5 = 0.5 * (L5 + L1);
-    double rho_in = rho - 1 * d1;
-    double vin[3] = {vel[0], vel[1], vel[2]};
+    rho_in = rho - 1 * d1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vin[d] = vel[d];
     vin[idir] = vel[idir] - 1 * d2;
     vin[t3] = vel[t3] - 1 * L3;
     vin[t4] = vel[t4] - 1 * L4;
-    const double p_in = p - 1 * d5;
-    double ra_in[NS], Yi[NS];
+    p_in = p - 1 * d5;
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
         const double dY = fd_uw(Yc[a], v.rhoa[a * n + c1] / rho1);
@@ -1274,7 +1406,8 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
         Yi[a] = ra_in[a] / rho_in;
     }
     double Tsi_in;
-    const double T_in = T_from_DP(v.u, rho_in, p_in, Yi, &Tsi_in);  // normalises Yi
+    T_in = T_from_DP(v.u, rho_in, p_in, Yi, &Tsi_in);  // normalises Yi
+    }
     // setState_TP(T_in, p_in): the inflow density follows
     const double Tsi = v.u.si_temp(T_in);
     rho_in = v.u.rho(v.u.si_p(p_in) * mean_W(Yi) / (R_UNIV * Tsi));
@@ -1369,6 +1502,153 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
     }
 }
 
+// ------------------------------------------------- TMS_BC wall sweep
+// Isothermal-wall sweep of TMS_BC (lbm-bc.cpp:941-1109): every link with a
+// TYPE_S cell upstream and fluid downstream is an interface link.  Target:
+// the cell's own species densities (sum of its populations), the mixture and
+// species velocities and the temperature interpolated at q = 1/2 between the
+// cell downstream and the wall cell, averaged over the links (species: the
+// downstream species velocity with the wall's mixture velocity); unknown
+// directions 2 f_tgt - f_loc, known f_tgt + f - f_loc.
+template <typename P>
+__global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* cells, int count)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t];
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    double* const* Fd = v.fld;
+    uint32_t unknown = 0, links = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint8_t tu = v.type[cell(v.d, x.i - cx(q), x.j - cy(q), x.k - cz(q))];
+        const uint8_t td = v.type[cell(v.d, x.i + cx(q), x.j + cy(q), x.k + cz(q))];
+        if (tu == TYPE_S) {
+            unknown |= 1u << q;
+            if (td == TYPE_F) links |= 1u << q;
+        }
+    }
+    // ---- target state
+    double ra_bb[NS], rho_bb = 0;
+    for (int a = 0; a < NS; ++a) {
+        double r = 0;
+        for (int q = 0; q < Q; ++q) r += ldf(v, v.f, a, q, c);
+        ra_bb[a] = r;
+        rho_bb += r;
+    }
+    double vt[3] = {0, 0, 0}, vat[NS][3], Tt = 0;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) vat[a][0] = vat[a][1] = vat[a][2] = 0.0;
+    int nd = 0;
+    const double qw = 0.5;
+    for (int q = 0; q < Q; ++q) {
+        if (!((links >> q) & 1u)) continue;
+        ++nd;
+        const int dn = cell(v.d, x.i + cx(q), x.j + cy(q), x.k + cz(q));
+        const int up = cell(v.d, x.i - cx(q), x.j - cy(q), x.k - cz(q));
+        double uw[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            uw[d] = Fd[LBM_FIELD_U + d][up];
+            vt[d] += (qw * Fd[LBM_FIELD_U + d][dn] + uw[d]) / (1.0 + qw);
+        }
+        Tt += (qw * Fd[LBM_FIELD_T][dn] + Fd[LBM_FIELD_T][up]) / (1.0 + qw);
+#pragma unroll
+        for (int a = 0; a < NS; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) vat[a][d] += (qw * v.ua[(d * NS + a) * n + dn] + uw[d]) / (1.0 + qw);
+    }
+    const double inv = 1.0 / nd;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vt[d] = inv * vt[d];
+    Tt = inv * Tt;
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) vat[a][d] = inv * vat[a][d];
+    // equilibria of the target (calculate_feq_geq, lbm-edf.cpp:168-196)
+    double Yt[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yt[a] = ra_bb[a] / rho_bb;
+    normalize(Yt);
+    const double Tst = v.u.si_temp(Tt);
+    const double U_t = v.u.energy_mass(u_mass(Tst, Yt));
+    const double th_t = v.u.energy_mass(R_UNIV * Tst / mean_W(Yt));
+    const double zero[3] = {0, 0, 0};
+    // ---- local moments with the target populations in the unknown directions
+    double ra_loc[NS], ma[NS][3], re_loc = 0;
+    double rho_loc = 0, mloc[3] = {0, 0, 0};
+    for (int a = 0; a < NS; ++a) {
+        const double th_a = v.u.energy_mass(R_UNIV * Tst / W[a]);
+        Axis3<double> Ft[3];
+        feq_axes<double>(Ft, vat[a], th_a, zero);
+        double r = 0, m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Ft, q, ra_bb[a]) : ldf(v, v.f, a, q, c);
+            r += f;
+            if (cx(q)) m0 += cx(q) * f;
+            if (cy(q)) m1 += cy(q) * f;
+            if (cz(q)) m2 += cz(q) * f;
+        }
+        ra_loc[a] = r;
+        ma[a][0] = m0;
+        ma[a][1] = m1;
+        ma[a][2] = m2;
+    }
+    const GeqAxes<double> Gt = geq_axes<double>(th_t, vt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        re_loc += (unknown >> q & 1u) ? geq_dir<double>(Gt, q, rho_bb, U_t - 1.5 * th_t) : ldg(v, v.g, q, c);
+    for (int a = 0; a < NS; ++a) {
+        rho_loc += ra_loc[a];
+        mloc[0] += ma[a][0];
+        mloc[1] += ma[a][1];
+        mloc[2] += ma[a][2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ma[a][d] = ra_loc[a] > 0 ? ma[a][d] / ra_loc[a] : 0.0;
+    }
+    double vloc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vloc[d] = rho_loc > 0 ? mloc[d] / rho_loc : 0.0;
+    double Uloc = re_loc / rho_loc;
+    for (int a = 0; a < NS; ++a)
+        Uloc = Uloc - 0.5 * ra_loc[a] / rho_loc * (ma[a][0] * ma[a][0] + ma[a][1] * ma[a][1] + ma[a][2] * ma[a][2]);
+    // calculate_temp (lbm-moment.cpp:540-553)
+    double Yl[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) Yl[a] = ra_loc[a] / rho_loc;
+    normalize(Yl);
+    const double T_loc = v.u.temp(T_from_UV(v.u.si_energy_mass(Uloc), Yl, v.u.si_temp(Fd[LBM_FIELD_T][c])));
+    const double Tsl = v.u.si_temp(T_loc);
+    const double U_loc = v.u.energy_mass(u_mass(Tsl, Yl));
+    const double th_loc = v.u.energy_mass(R_UNIV * Tsl / mean_W(Yl));
+    // ---- reconstruction
+    for (int a = 0; a < NS; ++a) {
+        const double tht = v.u.energy_mass(R_UNIV * Tst / W[a]);
+        const double thl = v.u.energy_mass(R_UNIV * Tsl / W[a]);
+        Axis3<double> Ft[3], Fl[3];
+        feq_axes<double>(Ft, vat[a], tht, zero);
+        feq_axes<double>(Fl, ma[a], thl, zero);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double ft = feq_dir<double>(Ft, q, ra_bb[a]);
+            const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
+            const double fin = ldf(v, v.f, a, q, c);
+            stf(v, v.f, a, q, c, (unknown >> q & 1u) ? 2 * ft - flc : ft + fin - flc);
+        }
+    }
+    const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double gt = geq_dir<double>(Gt, q, rho_bb, U_t - 1.5 * th_t);
+        const double gl = geq_dir<double>(Gl, q, rho_loc, U_loc - 1.5 * th_loc);
+        const double gin = ldg(v, v.g, q, c);
+        stg(v, v.g, q, c, (unknown >> q & 1u) ? 2 * gt - gl : gt + gin - gl);
+    }
+}
+
 // ------------------------------------------- rest state of FP32 storage
 // Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
 // the mean state the FP32 deviations are taken from.
@@ -1438,6 +1718,12 @@ void launch_mc_tms_in(const MultiView<P>& v, const int* cells, const int* dirs, 
     if (count > 0) hipLaunchKernelGGL(k_mc_tms_in<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
 }
 
+template <typename P>
+void launch_mc_tms_wall(const MultiView<P>& v, const int* cells, int count, hipStream_t s)
+{
+    if (count > 0) hipLaunchKernelGGL(k_mc_tms_wall<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, count);
+}
+
 static int cu_count()
 {
     int dev = 0, cus = 256;
@@ -1474,6 +1760,8 @@ template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const su
 template void launch_mc_chem_list<double>(const MultiView<double>&, const unsigned*, const unsigned*, hipStream_t);
 template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
 template void launch_mc_tms_in<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
+template void launch_mc_tms_wall<double>(const MultiView<double>&, const int*, int, hipStream_t);
+template void launch_mc_tms_wall<float>(const MultiView<float>&, const int*, int, hipStream_t);
 template void launch_mc_tms_in<float>(const MultiView<float>&, const int*, const int*, int, hipStream_t);
 
 }  // namespace lbm

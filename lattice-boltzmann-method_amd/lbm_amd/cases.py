@@ -206,6 +206,52 @@ def _all_species(d, minim=1e-6):
     return out
 
 
+def reacting_channel(nx, ny, nz, T_K=900.0, U=0.01, T_in_K=950.0, T_wall_K=800.0, X=None, X_in=None,
+                     p_out_factor=1.0):
+    """Multicomponent channel with the fixed-state boundary kinds of TMS_BC.
+
+    Inflow ghost (TYPE_I: u = U, v = 0.05 U, T_in, composition X_in) at
+    i = 0, outflow ghost (TYPE_O: p = p_out_factor atm) at i = nx+1,
+    isothermal walls (TYPE_S at T_wall) at j = 0 (at rest) and j = ny+1
+    (moving with u = -0.3 U), periodic in z; the x faces are typed last.
+    Interior: u = U with transverse waves, T_K, 1 atm, composition X (every
+    species present, so that the SPECIES_MIN clamp stays out of the way).
+    Temperatures in K; 'X' holds mole fractions (NS, ...)."""
+    fl = default_fields(nx, ny, nz)
+    t = fl["type"]
+    t[:, :, 0] = TYPE_P
+    t[:, :, -1] = TYPE_P
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    t[0, :, :] = TYPE_I
+    t[-1, :, :] = TYPE_O
+    shape = t.shape
+    i, j, k = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    Xv = mole_vector(_all_species(X or X_UNBURNT_PHI1))
+    Xi = mole_vector(_all_species(X_in or X_AIR))
+    fl["X"] = np.zeros((len(H2AIR_SPECIES),) + shape)
+    fl["X"][:] = Xv[:, None, None, None]
+    fl["T"][:] = T_K / LATTICE_K
+    fl["p"][:] = lattice_p(ONE_ATM)
+    for name in ("u", "v", "w"):
+        fl[name][:] = 0.0
+    F = t == TYPE_F
+    sj = np.sin(np.pi * j / (ny + 1))
+    fl["u"][F] = U
+    fl["v"][F] = (0.1 * U * np.sin(2 * np.pi * i / nx) * sj)[F]
+    fl["w"][F] = (0.05 * U * np.cos(2 * np.pi * (k - 1) / nz) * sj)[F]
+    I = t == TYPE_I
+    fl["u"][I] = U
+    fl["v"][I] = 0.05 * U
+    fl["T"][I] = T_in_K / LATTICE_K
+    fl["X"][:, I] = Xi[:, None]
+    fl["p"][t == TYPE_O] = lattice_p(ONE_ATM * p_out_factor)
+    S = t == TYPE_S
+    fl["T"][S] = T_wall_K / LATTICE_K
+    fl["u"][S & (j == shape[1] - 1)] = -0.3 * U
+    return fl
+
+
 def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, noise=0.01, seed=20261206,
                  X_fuel=None, X_ox=None, minim=1e-6):
     """Temporal H2-air / hot-air mixing layer: periodic x and z, NSCBC outflow

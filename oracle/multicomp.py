@@ -509,10 +509,12 @@ def _onlyone(a, b, c):
 
 
 class MultiStateBC(MultiState):
-    """MultiState plus the multicomponent TMS_BC outflow branch with the NSCBC
-    target state (lbm-bc.cpp:126-340, impose_nscbc.cpp:45-546) for TYPE_O_C
-    ghosts and the inflow branch with the inline LODI target state
-    (lbm-bc.cpp:463-938) for TYPE_I_C ghosts."""
+    """MultiState plus the multicomponent TMS_BC: the outflow sweep
+    (lbm-bc.cpp:126-340) with the NSCBC target (impose_nscbc.cpp:45-546) for
+    TYPE_O_C ghosts and the downstream-state target for TYPE_O ghosts; the
+    inflow sweep (lbm-bc.cpp:463-938) with the inline LODI target for TYPE_I_C
+    and the ghost-state target for TYPE_I; the isothermal-wall sweep
+    (lbm-bc.cpp:941-1109) for TYPE_S."""
 
     SIGMA_OUT = 0.25  # NSCBC_SIGMA_OUT, defines.hpp:100-102
 
@@ -724,13 +726,172 @@ class MultiStateBC(MultiState):
         rho_in = np.float64(U_.rho(p_si * m.W_mean(Y) / (m.R * T_si)))
         return rho_in, Y * rho_in, vel_in, T_in
 
+    def _moment_point(self, c):
+        """calculate_moment_point (lbm-moment.cpp:379-537) of cell c, writing
+        its fields.  Unlike calculate_moment: no SPECIES_MIN clamp; the binary
+        diffusivities at (T of the cell's fields, the new density and
+        composition); the Maxwell-Stefan matrix from the fields' p and X (not
+        yet updated); the solve by partial-pivoting LU."""
+        U_, m = self.units, self.mech
+        g = self.g[:, c]
+        self.rhoe[c] = g.sum()
+        self.eflux[:, c] = C3 @ g
+        fa = self.f[:, :, c]  # (ns, 27)
+        ra = fa.sum(axis=1)
+        mom = fa @ C3.T.astype(np.float64)  # (ns, 3)
+        P = np.zeros((3, 3))
+        for p_ in range(3):
+            for q in range(p_, 3):
+                P[p_, q] = np.sum(fa * (C3[p_] * C3[q]).astype(np.float64)[None, :])
+        P[1, 0], P[2, 0], P[2, 1] = P[0, 1], P[0, 2], P[1, 2]
+        self.ptens[:, :, c] = P
+        rho = ra.sum()
+        self.rho_a[:, c] = ra
+        self.rho[c] = rho
+        self.u[c], self.v[c], self.w[c] = mom.sum(axis=0) / rho
+        Y = m.normalize_Y(ra / rho)
+        T_old_si = np.float64(U_.si_temp(self.T[c]))
+        rho_si = np.float64(U_.si_rho(rho))
+        p_si = m.pressure(T_old_si, rho_si, Y)
+        Dab = U_.nu(m.binary_diff(T_old_si, p_si)).astype(np.float64)
+        A = np.zeros((self.ns, self.ns))
+        for a in range(self.ns):
+            for b in range(self.ns):
+                if b != a:
+                    A[a, b] = -1.0 * 1 / 2.0 * self.p[c] * self.X[a, c] * self.X[b, c] / Dab[a, b]
+                else:
+                    A[a, a] = 1.0 if ra[a] == 0.0 else ra[a]
+        for a in range(self.ns):
+            for b in range(self.ns):
+                if b != a:
+                    A[a, a] -= A[a, b]
+        ua = np.linalg.solve(A, mom)  # (ns, 3)
+        self.u_a[:, :, c] = ua.T
+        ke = 0.0
+        for a in range(self.ns):
+            ke += 0.5 * ra[a] * (ua[a] @ ua[a])
+        Uint = self.rhoe[c] / rho - ke / rho
+        T_si = m.T_from_UV(np.float64(U_.si_energy_mass(Uint)), Y[None], np.array([T_old_si]))[0]
+        self.X[:, c] = m.X_from_Y(Y)
+        self.T[c] = np.float64(U_.temp(T_si))
+        self.p[c] = np.float64(U_.p(m.pressure(T_si, rho_si, Y)))
+
+    def _reconstruct(self, c, kinds, fa_b, g_b, vel_tgt=None, T_tgt=None, inflow=False):
+        """The local part shared by the sweeps: unknown directions (upstream
+        ghost of one of `kinds`) take the boundary set (fa_b, g_b) in the local
+        moments; T_loc by calculate_temp; then unknown 2 f_b - f_loc and known
+        f_b + f - f_loc, or, for the inflow sweep, f_tgt + f_b - f_loc and
+        f_tgt + f - f_loc with f_tgt the equilibrium of the local density and
+        composition at (vel_tgt, T_tgt)."""
+        U_ = self.units
+        i, j, k = self._ijk(c)
+        unknown = np.array([self.type[self._idx(i - CX[q], j - CY[q], k - CZ[q])] in kinds for q in range(27)])
+        fsrc = np.where(unknown[None, :], fa_b, self.f[:, :, c])
+        gsrc = np.where(unknown, g_b, self.g[:, c])
+        rhoa_loc = fsrc.sum(axis=1)
+        mom = fsrc @ C3.T.astype(np.float64)
+        rhoe_loc = gsrc.sum()
+        rho_loc = rhoa_loc.sum()
+        vel_loc = mom.sum(axis=0) / rho_loc if rho_loc > 0 else np.zeros(3)
+        vela_loc = np.where(rhoa_loc[:, None] > 0, mom / np.where(rhoa_loc > 0, rhoa_loc, 1.0)[:, None], 0.0)
+        Uint = rhoe_loc / rho_loc
+        for a in range(self.ns):
+            Uint = Uint - 0.5 * rhoa_loc[a] / rho_loc * (vela_loc[a] @ vela_loc[a])
+        T_loc = self._calculate_temp(Uint, rho_loc, rhoa_loc, np.float64(U_.si_temp(self.T[c])))
+        fa_loc, g_loc = self._feq_geq(rho_loc, rhoa_loc, vel_loc, vela_loc, T_loc)
+        if inflow:
+            fa_t, g_t = self._feq_geq(rho_loc, rhoa_loc, vel_tgt, np.repeat(vel_tgt[None], self.ns, 0), T_tgt)
+            fnew = np.where(unknown[None, :], fa_t + fa_b - fa_loc, fa_t + self.f[:, :, c] - fa_loc)
+            gnew = np.where(unknown, g_t + g_b - g_loc, g_t + self.g[:, c] - g_loc)
+        else:
+            fnew = np.where(unknown[None, :], 2 * fa_b - fa_loc, fa_b + self.f[:, :, c] - fa_loc)
+            gnew = np.where(unknown, 2 * g_b - g_loc, g_b + self.g[:, c] - g_loc)
+        self.f[:, :, c] = fnew
+        self.g[:, c] = gnew
+
+    def _outflow_plain(self, c, l):
+        """TYPE_O outflow target (lbm-bc.cpp:168-198): moments of the cell
+        downstream (recomputed by calculate_moment_point), the pressure of
+        the ghost, T from (rho, p, Y) by setState_DP."""
+        U_ = self.units
+        d = self._idx(*(np.array(self._ijk(c)) + np.array([CX[l], CY[l], CZ[l]])))
+        g_ = self._idx(*(np.array(self._ijk(c)) - np.array([CX[l], CY[l], CZ[l]])))
+        self._moment_point(d)
+        vel_out = np.array([self.u[d], self.v[d], self.w[d]])
+        rho_out = self.rho[d]
+        rhoa_out = self.rho_a[:, d].copy()
+        T_si, _ = self._state_DP(rho_out, self.p[g_], rhoa_out / rho_out)
+        return rho_out, rhoa_out, vel_out, np.float64(U_.temp(T_si))
+
+    def _inflow_plain(self, c, l):
+        """TYPE_I inflow target (lbm-bc.cpp:506-518, 672-680): velocity,
+        temperature and composition of the ghost, the pressure of the cell
+        downstream after calculate_moment_point, the density from
+        setState_TP."""
+        U_, m = self.units, self.mech
+        d = self._idx(*(np.array(self._ijk(c)) + np.array([CX[l], CY[l], CZ[l]])))
+        g_ = self._idx(*(np.array(self._ijk(c)) - np.array([CX[l], CY[l], CZ[l]])))
+        self._moment_point(d)
+        vel_in = np.array([self.u[g_], self.v[g_], self.w[g_]])
+        T_in = self.T[g_]
+        Y = m.normalize_Y(self.rho_a[:, g_] / self.rho[g_])
+        T_si = np.float64(U_.si_temp(T_in))
+        p_si = np.float64(U_.si_p(self.p[d]))
+        rho_in = np.float64(U_.rho(p_si * m.W_mean(Y) / (m.R * T_si)))
+        return rho_in, Y * rho_in, vel_in, T_in
+
+    def _tms_wall(self):
+        """TMS_BC isothermal-wall sweep, MULTICOMP (lbm-bc.cpp:941-1109):
+        every link with a TYPE_S cell upstream and fluid downstream is an
+        interface link; the target takes the cell's own species densities,
+        and velocity (mixture and per species) and temperature interpolated at
+        q = 1/2 between the cell downstream and the wall, averaged over the
+        links; unknown directions 2 f_tgt - f_loc, known f_tgt + f - f_loc."""
+        q = 0.5
+        for c in np.nonzero(self.type == TYPE_F)[0]:
+            i, j, k = self._ijk(c)
+            links = []
+            for l in range(27):
+                up = self._idx(i - CX[l], j - CY[l], k - CZ[l])
+                dn = self._idx(i + CX[l], j + CY[l], k + CZ[l])
+                if self.type[up] == TYPE_S and self.type[dn] == TYPE_F:
+                    links.append((up, dn))
+            if not links:
+                continue
+            rhoa_bb = self.f[:, :, c].sum(axis=1)
+            rho_bb = 0.0
+            for a in range(self.ns):
+                rho_bb += rhoa_bb[a]
+            vel = np.zeros(3)
+            vela = np.zeros((self.ns, 3))
+            T = 0.0
+            for up, dn in links:
+                mu_up = np.array([self.u[up], self.v[up], self.w[up]])
+                mu_dn = np.array([self.u[dn], self.v[dn], self.w[dn]])
+                vel += (q * mu_dn + mu_up) / (1.0 + q)
+                T += (q * self.T[dn] + self.T[up]) / (1.0 + q)
+                vela += (q * self.u_a[:, :, dn].T + mu_up[None, :]) / (1.0 + q)
+            nd = len(links)
+            vel = 1.0 / nd * vel
+            T = 1.0 / nd * T
+            vela = 1.0 / nd * vela
+            fa_t, g_t = self._feq_geq(rho_bb, rhoa_bb, vel, vela, T)
+            self._reconstruct(c, (TYPE_S,), fa_t, g_t)
+
     def tms_bc(self):
-        """TMS_BC outflow sweep (lbm-bc.cpp:126-340) for TYPE_O_C ghosts."""
+        """TMS_BC: the outflow sweep (lbm-bc.cpp:126-340; TYPE_O_C ghosts with
+        the NSCBC target, TYPE_O ghosts with the downstream state), then the
+        inflow sweep and the isothermal-wall sweep."""
         m, U_ = self.mech, self.units
         updates = []
         for c, l in self.interface_cells():
             i, j, k = self._ijk(c)
-            rho_out, rhoa_out, vel_out, T_out = self._nscbc(c, l)
+            if self.type[self._idx(i - CX[l], j - CY[l], k - CZ[l])] == TYPE_O:
+                # the downstream cell's moments are rewritten in place as the
+                # sweep reaches this cell, as calculate_moment_point does
+                rho_out, rhoa_out, vel_out, T_out = self._outflow_plain(c, l)
+            else:
+                rho_out, rhoa_out, vel_out, T_out = self._nscbc(c, l)
             fa_out, g_out = self._feq_geq(rho_out, rhoa_out, vel_out, np.repeat(vel_out[None], self.ns, 0), T_out)
             unknown = np.array([self.type[self._idx(i - CX[q], j - CY[q], k - CZ[q])] in (TYPE_O, TYPE_O_C)
                                 for q in range(27)])
@@ -754,6 +915,7 @@ class MultiStateBC(MultiState):
             self.f[:, :, c] = fnew
             self.g[:, c] = gnew
         self._tms_inflow()
+        self._tms_wall()
 
     def _tms_inflow(self):
         """TMS_BC inflow sweep (lbm-bc.cpp:463-938) for TYPE_I_C ghosts: the
@@ -766,9 +928,10 @@ class MultiStateBC(MultiState):
         updates = []
         for c, l in self.interface_cells((TYPE_I, TYPE_I_C)):
             i, j, k = self._ijk(c)
-            if self.type[self._idx(i - CX[l], j - CY[l], k - CZ[l])] != TYPE_I_C:
-                raise ValueError("only TYPE_I_C inflow is handled")
-            rho_in, rhoa_in, vel_in, T_in = self._inflow_lodi(c, l)
+            if self.type[self._idx(i - CX[l], j - CY[l], k - CZ[l])] == TYPE_I:
+                rho_in, rhoa_in, vel_in, T_in = self._inflow_plain(c, l)
+            else:
+                rho_in, rhoa_in, vel_in, T_in = self._inflow_lodi(c, l)
             fa_in, g_in = self._feq_geq(rho_in, rhoa_in, vel_in, np.repeat(vel_in[None], self.ns, 0), T_in)
             unknown = np.array([self.type[self._idx(i - CX[q], j - CY[q], k - CZ[q])] in (TYPE_I, TYPE_I_C)
                                 for q in range(27)])

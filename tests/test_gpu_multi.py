@@ -196,16 +196,51 @@ def test_mc_mixing_layer_outflow_calls_match_oracle(reaction):
 
 
 def test_mc_unsupported_boundaries_fail_loudly():
+    """Free-slip walls are not provided with species: rejected at set-up."""
     fl = cases.mixing_layer(4, 6, 4)
-    fl["type"][:, 0, :] = cases.TYPE_I
-    lat, _ = pair(fl)
-    lat.Init()
-    lat.Collide_Species()
-    lat.Collide()
-    lat.Streaming()
+    fl["type"][:, 0, :] = cases.TYPE_FS
     from lbm_amd.capi import LbmError
-    with pytest.raises(LbmError, match="inflow"):
+    with pytest.raises(LbmError, match="free-slip"):
+        pair(fl)
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_mc_channel_fixed_state_boundaries_match_oracle(reaction):
+    """TYPE_I inflow (ghost state, the downstream pressure after
+    calculate_moment_point), TYPE_O outflow (downstream state, ghost
+    pressure) and TYPE_S isothermal walls, one moving: per-call parity with
+    the device re-seeded from the oracle before each call."""
+    fl = cases.reacting_channel(8, 5, 3, p_out_factor=0.995)
+    lat, s = pair(fl, reaction=reaction)
+    lat.Init()
+    s.init()
+    F = fluid(s)
+    lat.calculate_moment()
+    s.moments()
+    for step in range(1, 4):
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide_Species()
+        s.collide_species(reaction)
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide()
+        s.collide()
+        seed_gpu_from_oracle(lat, s)
+        lat.Streaming()
+        s.stream()
+        seed_gpu_from_oracle(lat, s)
         lat.TMS_BC()
+        s.tms_bc()
+        e = relerr(lat.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], s.f[:, :, F])
+        assert e < TOL, f"step {step} species f after TMS {e:.3e}"
+        e = relerr(lat.get_populations(POP_G).reshape(27, -1)[:, F], s.g[:, F])
+        assert e < TOL, f"step {step} g after TMS {e:.3e}"
+        # calculate_moment_point rewrote the fields of the cells downstream
+        # of the inflow and outflow interfaces
+        check_state(lat, s, TOL, f"step {step} after TMS")
+        seed_gpu_from_oracle(lat, s)
+        lat.calculate_moment()
+        s.moments()
+        check_state(lat, s, TOL, f"step {step}")
 
 
 def _run_precision(fl, precision, steps, reaction=True):

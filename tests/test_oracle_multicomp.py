@@ -17,7 +17,7 @@ import pytest
 
 from lbm_amd import cases
 from oracle.chem import Mechanism
-from oracle.multicomp import C3, MultiState, MultiStateBC, Units
+from oracle.multicomp import C3, CX, CY, CZ, MultiState, MultiStateBC, Units
 
 
 def make_state(fl, units=None):
@@ -242,3 +242,117 @@ def test_inflow_tms_keeps_a_matching_rest_state():
     F = s.type == 0
     assert np.max(np.abs(s.f[:, :, F] - f0[:, :, F]) / np.abs(f0[:, :, F])) < 1e-5
     assert np.max(np.abs(s.g[:, F] - g0[:, F])) < 1e-5 * np.max(np.abs(g0[:, F]))
+
+
+# ----------------------------------------- fixed-state TMS kinds (I, O, S)
+def _channel(**kw):
+    fl = cases.reacting_channel(6, 4, 3, **kw)
+    s = make_bc_state(fl)
+    s.init()
+    s.moments()
+    return s, fl
+
+
+def test_moment_point_matches_calculate_moment_at_a_settled_state():
+    """calculate_moment_point and calculate_moment differ only in the state
+    their Maxwell-Stefan matrix and diffusivities are taken at and in the
+    clamp; on a state that calculate_moment has just produced (every species
+    above SPECIES_MIN), the point version must return the same moments."""
+    s, _ = _channel()
+    s.moments()
+    c = int(np.nonzero(s.type == 0)[0][5])
+    ref = {k: getattr(s, k)[..., c].copy() for k in ("rho", "u", "v", "w", "T", "p", "rhoe", "rho_a", "X", "u_a")}
+    s._moment_point(c)
+    for k, v in ref.items():
+        np.testing.assert_allclose(getattr(s, k)[..., c], v, rtol=1e-9, atol=1e-18, err_msg=k)
+
+
+def test_fixed_state_sweeps_keep_a_matching_rest_state():
+    """Interior at rest in equilibrium, inflow ghost at the interior's
+    velocity, temperature and composition, walls at rest at the interior
+    temperature, outflow ghost at the interior pressure: every target equals
+    the local state, so the three sweeps leave the populations unchanged up
+    to the binary32 unit crossings."""
+    s, fl = _channel(U=0.0, T_in_K=900.0, T_wall_K=900.0, X_in=cases.X_UNBURNT_PHI1)
+    s.u[:] = 0.0
+    s.v[:] = 0.0
+    s.w[:] = 0.0
+    s.init()
+    s.moments()
+    f0, g0 = s.f.copy(), s.g.copy()
+    s.tms_bc()
+    F = s.type == 0
+    assert np.max(np.abs(s.f[:, :, F] - f0[:, :, F]) / np.abs(f0[:, :, F])) < 1e-5
+    assert np.max(np.abs(s.g[:, F] - g0[:, F])) < 1e-5 * np.max(np.abs(g0[:, F]))
+
+
+def test_wall_sweep_sets_species_density_and_velocity_targets():
+    """For each species, the populations the wall sweep writes have zeroth
+    moment rho_a,bb (the cell's own) and first moment rho_a,bb u_a,tgt,
+    u_a,tgt the q = 1/2 interpolation between the species velocity
+    downstream and the wall's velocity, averaged over the interface links:
+    unknown 2 f_t - f_loc plus known f_t + f - f_loc sum to the target's
+    moments because f_loc matches the local moments."""
+    s, _ = _channel()
+    # distinct species velocities in the fields, so that a mixture-velocity
+    # target would fail
+    rng = np.random.default_rng(5)
+    F = np.nonzero(s.type == 0)[0]
+    s.u_a[:, :, F] += 1e-3 * rng.standard_normal((3, s.ns, len(F)))
+    c = s._idx(3, 1, 1)  # first layer above the lower wall
+    i, j, k = s._ijk(c)
+    links = []
+    for l in range(27):
+        up = s._idx(i - CX[l], j - CY[l], k - CZ[l])
+        dn = s._idx(i + CX[l], j + CY[l], k + CZ[l])
+        if s.type[up] == 1 and s.type[dn] == 0:
+            links.append((up, dn))
+    assert len(links) == 6  # the cz = -1 links end in the periodic ghost layer (k = 0)
+    want_ra = s.f[:, :, c].sum(axis=1)
+    want_va = np.zeros((s.ns, 3))
+    for up, dn in links:
+        uw = np.array([s.u[up], s.v[up], s.w[up]])
+        want_va += (0.5 * s.u_a[:, :, dn].T + uw) / 1.5
+    want_va /= len(links)
+    s._tms_wall()
+    got_ra = s.f[:, :, c].sum(axis=1)
+    got_m = s.f[:, :, c] @ C3.T.astype(np.float64)
+    np.testing.assert_allclose(got_ra, want_ra, rtol=1e-12)
+    np.testing.assert_allclose(got_m / got_ra[:, None], want_va, rtol=1e-9, atol=1e-14)
+
+
+def test_plain_outflow_takes_the_downstream_densities():
+    """TYPE_O target: the species densities of the cell downstream,
+    recomputed by calculate_moment_point from its populations (no clamp):
+    after the sweep the interface cell's zeroth moments equal them."""
+    s, _ = _channel(p_out_factor=0.99)
+    s.collide_species(False)
+    s.collide()
+    s.stream()
+    cells = [(c, l) for c, l in s.interface_cells() if s.type[c - s._strides()[0] * CX[l]] == 6]
+    assert cells and all(l == 2 for _, l in cells)  # direction -x, ghost at i = nx+1
+    c, l = cells[0]
+    d = c + s._strides()[0] * CX[l]
+    want = s.f[:, :, d].sum(axis=1)  # populations of d are not touched by the sweep
+    s.tms_bc()
+    np.testing.assert_allclose(s.rho_a[:, d], want, rtol=1e-14)  # moment_point wrote d's fields
+    np.testing.assert_allclose(s.f[:, :, c].sum(axis=1), want, rtol=1e-12)
+
+
+def test_plain_inflow_imposes_the_ghost_velocity():
+    """TYPE_I: the target f_tgt is the equilibrium of the local density and
+    composition at the ghost's velocity, so the interface cell's momentum
+    after the sweep is rho_loc u_ghost."""
+    s, _ = _channel()
+    s.collide_species(False)
+    s.collide()
+    s.stream()
+    cells = s.interface_cells((5, 7))
+    assert cells and all(l == 1 for _, l in cells)
+    c, l = [(c, l) for c, l in cells if s._ijk(c)[1] == 2][0]  # away from the walls (their sweep runs last)
+    g_ = c - s._strides()[0]
+    ug = np.array([s.u[g_], s.v[g_], s.w[g_]])
+    s.tms_bc()
+    fa = s.f[:, :, c]
+    m = (fa @ C3.T.astype(np.float64)).sum(axis=0)
+    np.testing.assert_allclose(m / fa.sum(), ug, rtol=1e-9, atol=1e-15)
