@@ -57,6 +57,27 @@ def parse():
     return ap.parse_args()
 
 
+def pmc_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summaries (profiles/pmc_traffic_*.md, FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy's readings), or None when no summary
+    covers it.  Each summary carries one line 'machine-readable: {json}'
+    keyed by workload and kernel label."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "pmc_traffic_*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("machine-readable:"):
+                    try:
+                        rec = json.loads(line.split(":", 1)[1])
+                    except ValueError:
+                        continue
+                    hit = rec.get(workload, {}).get(kernel)
+                    if hit:
+                        return int(hit["traffic_bytes"])
+    return None
+
+
 def make_lattice(fl, pr, precision, mode):
     from lbm_amd.capi import Lattice
     N = pr["N"]
@@ -211,8 +232,8 @@ def mixing_layer_bench(args):
         bpl = MC_POP_TRAFFIC[dom] * 4
         achieved = bpl * cells / (kern[dom]["avg_ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                    "bytes_per_launch": bpl * cells, "avg_ms": round(kern[dom]["avg_ms"], 4)}
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("mixing256", dom),
+                    "kernel": dom, "bytes_per_launch": bpl * cells, "avg_ms": round(kern[dom]["avg_ms"], 4)}
 
     if not args.no_compare:
         ref = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP64)
@@ -300,7 +321,7 @@ def main():
     if k:
         achieved = bpl * cells / (k["avg_ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, "fused_collide_stream"),
                     "kernel": "k_fused_collide (pull-stream + collide)", "bytes_per_launch": bpl * cells,
                     "avg_ms": round(k["avg_ms"], 4)}
     extra = {"kernels": {n: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]} for n, v in kern.items()},
