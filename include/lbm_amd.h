@@ -143,6 +143,26 @@ void* lbm_get_stream(lbm_ctx* ctx);
 int lbm_profile_enable(lbm_ctx* ctx, int on);
 int lbm_profile_read(lbm_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
 
+/* ---------------------------------------------------------------- files
+ * OutputVTK (output.cpp:9-210): the current state as a VTK RectilinearGrid
+ * (.vtr, raw appended Float32 arrays in (k, j, i) order): Density,
+ * Velocity, CellType, Temperature, Pressure, and with species HRR, one mole
+ * (or mass) fraction and one relative x velocity per species.  flags:
+ * LBM_VTK_SI converts through Units_Conv (OUTPUT_SI), LBM_VTK_MASS_FRACTIONS
+ * writes mass fractions (OUTPUT_MASSFRAC); the reference selects both at
+ * compile time. */
+enum { LBM_VTK_SI = 1, LBM_VTK_MASS_FRACTIONS = 2 };
+int lbm_write_vtk(lbm_ctx* ctx, const char* path, int flags);
+/* write_restart (restart_file.cpp:274-350): the reference's restart file
+ * (int header dx dy dz Nx Ny Nz dt_sim step; the gas constants, or the
+ * species names and permeability; one MIXTURE record per cell, followed by
+ * its SPECIES records, in the x86-64 layout of lbm.hpp:56-102). */
+int lbm_write_restart(lbm_ctx* ctx, const char* path, int step);
+/* read_restart (restart_file.cpp:352-440) into this context: the grid,
+ * species and gas constants must match; restores cell types, fields and
+ * populations and returns the step in *step. */
+int lbm_read_restart(lbm_ctx* ctx, const char* path, int* step);
+
 /* Bytes of device memory the context holds. */
 int64_t lbm_device_bytes(const lbm_ctx* ctx);
 

@@ -1420,3 +1420,395 @@ int lbm_surrogate_gradient(lbm_ctx* c, const float* X, const float* T, int64_t m
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ output files
+// OutputVTK (output.cpp:9-210), write_restart (restart_file.cpp:274-350) and
+// read_restart (restart_file.cpp:352-440) over the device state: the fields
+// and populations come down through the same copies as lbm_get_* (FP32
+// storage converted back to FP64 with its rest-state shifts), the files keep
+// the reference's byte layout.
+namespace {
+
+const char* const kSpeciesNames[mech::NS] = {"H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"};
+
+// Units_Conv of the context; the single-component reference keeps the
+// default m = kg = s = K = 1 (units.hpp:10).
+chem::Units io_units(const lbm_ctx* c) { return multi(c) ? c->units : chem::Units{1.f, 1.f, 1.f, 1.f}; }
+
+// si_u and si_HRR of units.hpp:119,129 (float arguments; sq/cb in double)
+float si_u(const chem::Units& u, double x) { return (float)x * u.m / u.s; }
+float si_hrr(const chem::Units& u, double x)
+{
+    const double m = u.m, s = u.s;
+    return (float)((double)(float)x * ((double)u.kg * (m * m) / (s * s * s)));
+}
+
+struct Download {
+    std::vector<uint8_t> type;
+    std::vector<double> fld[LBM_FIELD_COUNT];
+    std::vector<double> sp[5];  // X, rho, u, v, w: [NS][n]
+};
+
+int download(lbm_ctx* c, Download& h, bool species)
+{
+    const int64_t n = c->d.n;
+    h.type.resize(n);
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(h.type.data(), c->type, n, hipMemcpyDeviceToHost));
+    for (int f = 0; f < LBM_FIELD_COUNT; ++f) {
+        h.fld[f].resize(n);
+        if (int rc = lbm_get_field(c, f, h.fld[f].data())) return rc;
+    }
+    if (species)
+        for (int f = 0; f < 5; ++f) {
+            h.sp[f].resize((size_t)n * c->ns);
+            if (int rc = lbm_get_species_field(c, f, h.sp[f].data())) return rc;
+        }
+    return LBM_OK;
+}
+
+struct File {
+    FILE* fp = nullptr;
+    ~File()
+    {
+        if (fp) fclose(fp);
+    }
+};
+
+template <typename T>
+void put(std::vector<char>& b, const T& x)
+{
+    const char* p = reinterpret_cast<const char*>(&x);
+    b.insert(b.end(), p, p + sizeof(T));
+}
+
+// MIXTURE / SPECIES records (lbm.hpp:56-102): byte offsets of the x86-64
+// layout, one record per cell in (i, j, k) order, k fastest.
+constexpr size_t MIX1_SIZE = 1024;  // single component: type, f, fpc, p_tensor, rho u v w p, g, gpc, rhoe, q, temp
+constexpr size_t MIXM_SIZE = 600;   // multicomponent:   type, p_tensor, rho u v w p, g, gpc, rhoe, q, temp, HRR
+constexpr size_t SPEC_SIZE = 472;   // X, rho, f, fpc, u, v, w
+
+}  // namespace
+
+int lbm_write_vtk(lbm_ctx* c, const char* path, int flags)
+{
+    if (!c || !path) return fail(c, LBM_E_ARG, "null argument");
+    if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede output");
+    const bool mc = multi(c);
+    Download h;
+    if (int rc = download(c, h, mc)) return rc;
+    const chem::Units U = io_units(c);
+    const bool si = flags & LBM_VTK_SI, massfrac = flags & LBM_VTK_MASS_FRACTIONS;
+    const int Nx = c->d.nx, Ny = c->d.ny, Nz = c->d.nz, dx = 1;
+    const int64_t n = c->d.n;
+    File f;
+    f.fp = fopen(path, "wb");
+    if (!f.fp) return fail(c, LBM_E_ARG, std::string("cannot open ") + path);
+    FILE* fp = f.fp;
+    const long N = (long)Nx * Ny * Nz;
+    long offset = 0;
+    fprintf(fp, "<?xml version=\"1.0\"?>\n");
+    fprintf(fp, "<VTKFile type=\"RectilinearGrid\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt32\">\n");
+    fprintf(fp, "  <RectilinearGrid WholeExtent=\"0 %d 0 %d 0 %d\">\n", Nx, Ny, Nz);
+    fprintf(fp, "  <Piece Extent=\"0 %d 0 %d 0 %d\">\n", Nx, Ny, Nz);
+    fprintf(fp, "    <PointData>\n");
+    fprintf(fp, "    </PointData>\n");
+    fprintf(fp, "    <CellData>\n");
+    auto arr = [&](const char* name, int ncomp) {
+        if (ncomp == 1)
+            fprintf(fp, "      <DataArray type=\"Float32\" Name=\"%s\" format=\"appended\" offset=\"%ld\" />\n", name, offset);
+        else
+            fprintf(fp,
+                    "      <DataArray type=\"Float32\" Name=\"%s\" NumberOfComponents=\"%d\" format=\"appended\" "
+                    "offset=\"%ld\" />\n",
+                    name, ncomp, offset);
+        offset += 4 + (long)ncomp * 4 * N;
+    };
+    arr("Density", 1);
+    arr("Velocity", 3);
+    arr("CellType", 1);
+    arr("Temperature", 1);
+    arr("Pressure", 1);
+    if (mc) {
+        arr("HRR", 1);
+        for (int a = 0; a < c->ns; ++a) arr((std::string("Mole Fraction ") + kSpeciesNames[a]).c_str(), 1);
+        for (int a = 0; a < c->ns; ++a) arr((std::string("Species Velocity ") + kSpeciesNames[a]).c_str(), 1);
+    }
+    fprintf(fp, "    </CellData>\n");
+    fprintf(fp, "    <Coordinates>\n");
+    fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CoordinateX\" format=\"appended\" offset=\"%ld\" />\n", offset);
+    offset += 4 + 4L * (Nx + 1);
+    fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CoordinateY\" format=\"appended\" offset=\"%ld\" />\n", offset);
+    offset += 4 + 4L * (Ny + 1);
+    fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CoordinateZ\" format=\"appended\" offset=\"%ld\" />\n", offset);
+    offset += 4 + 4L * (Nz + 1);
+    fprintf(fp, "    </Coordinates>\n");
+    fprintf(fp, "  </Piece>\n");
+    fprintf(fp, "  </RectilinearGrid>\n");
+    fprintf(fp, "  <AppendedData encoding=\"raw\">");
+    fprintf(fp, "_");
+    // arrays in (k, j, i) order, i fastest
+    std::vector<float> buf;
+    auto block = [&](int ncomp, auto value) {
+        const uint32_t size = (uint32_t)(ncomp * 4 * N);
+        fwrite(&size, sizeof(uint32_t), 1, fp);
+        buf.clear();
+        for (int k = 0; k < Nz; ++k)
+            for (int j = 0; j < Ny; ++j)
+                for (int i = 0; i < Nx; ++i) {
+                    const int64_t cc = ((int64_t)i * Ny + j) * Nz + k;
+                    for (int q = 0; q < ncomp; ++q) buf.push_back(value(cc, q));
+                }
+        fwrite(buf.data(), sizeof(float), buf.size(), fp);
+    };
+    const auto& F = h.fld;
+    block(1, [&](int64_t cc, int) { return si ? U.si_rho(F[LBM_FIELD_RHO][cc]) : (float)F[LBM_FIELD_RHO][cc]; });
+    block(3, [&](int64_t cc, int q) {
+        const double x = F[LBM_FIELD_U + q][cc];
+        return si ? si_u(U, x) : (float)x;
+    });
+    // the cell type as a float in a 4-byte slot (output.cpp:106-115)
+    block(1, [&](int64_t cc, int) { return (float)(int)h.type[cc]; });
+    block(1, [&](int64_t cc, int) { return si ? U.si_temp(F[LBM_FIELD_T][cc]) : (float)F[LBM_FIELD_T][cc]; });
+    block(1, [&](int64_t cc, int) { return si ? U.si_p(F[LBM_FIELD_P][cc]) : (float)F[LBM_FIELD_P][cc]; });
+    if (mc) {
+        block(1, [&](int64_t cc, int) { return si ? si_hrr(U, F[LBM_FIELD_HRR][cc]) : (float)F[LBM_FIELD_HRR][cc]; });
+        for (int a = 0; a < c->ns; ++a)
+            block(1, [&](int64_t cc, int) {
+                return massfrac ? (float)(h.sp[LBM_SP_RHO][a * n + cc] / F[LBM_FIELD_RHO][cc])
+                                : (float)h.sp[LBM_SP_X][a * n + cc];
+            });
+        // x component of the species velocity relative to the mixture (output.cpp:174-189)
+        for (int a = 0; a < c->ns; ++a)
+            block(1, [&](int64_t cc, int) {
+                const double ua = h.sp[LBM_SP_U][a * n + cc], um = F[LBM_FIELD_U][cc];
+                return si ? si_u(U, (float)(ua - um)) : (float)((double)(float)ua - um);
+            });
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+        const int L = ax == 0 ? Nx : (ax == 1 ? Ny : Nz);
+        const uint32_t size = (uint32_t)(4 * (L + 1));
+        fwrite(&size, sizeof(uint32_t), 1, fp);
+        for (int i = 0; i < L + 1; ++i) {
+            const float x = (float)(i * dx);
+            fwrite(&x, sizeof(float), 1, fp);
+        }
+    }
+    fprintf(fp, "  </AppendedData>\n");
+    fprintf(fp, "</VTKFile>\n");
+    if (ferror(fp)) return fail(c, LBM_E_ARG, std::string("write error on ") + path);
+    return LBM_OK;
+}
+
+int lbm_write_restart(lbm_ctx* c, const char* path, int step)
+{
+    if (!c || !path) return fail(c, LBM_E_ARG, "null argument");
+    if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede output");
+    const bool mc = multi(c);
+    Download h;
+    if (int rc = download(c, h, mc)) return rc;
+    const int64_t n = c->d.n;
+    const int ns = c->ns;
+    std::vector<double> f((size_t)n * Q * (mc ? ns : 1)), fpc(f.size()), g((size_t)n * Q), gpc(g.size());
+    if (int rc = lbm_get_populations(c, LBM_POP_F, f.data())) return rc;
+    if (int rc = lbm_get_populations(c, LBM_POP_FPC, fpc.data())) return rc;
+    if (int rc = lbm_get_populations(c, LBM_POP_G, g.data())) return rc;
+    if (int rc = lbm_get_populations(c, LBM_POP_GPC, gpc.data())) return rc;
+    File fl;
+    fl.fp = fopen(path, "wb");
+    if (!fl.fp) return fail(c, LBM_E_ARG, std::string("cannot open ") + path);
+    std::vector<char> b;
+    const int one = 1;
+    put(b, one);  // dx
+    put(b, one);  // dy
+    put(b, one);  // dz
+    put(b, c->d.nx);
+    put(b, c->d.ny);
+    put(b, c->d.nz);
+    put(b, one);  // dt_sim
+    put(b, step);
+    if (!mc) {
+        const double Ra = 1.0;  // LBM::Ra default; not used on this path
+        put(b, c->cfg.nu);
+        put(b, c->cfg.gas_const);
+        put(b, c->cfg.gamma);
+        put(b, Ra);
+        put(b, c->cfg.prtl);
+    } else {
+        const size_t nsp = (size_t)ns;
+        put(b, nsp);
+        for (int a = 0; a < ns; ++a) {
+            const size_t len = strlen(kSpeciesNames[a]);
+            put(b, len);
+            b.insert(b.end(), kSpeciesNames[a], kSpeciesNames[a] + len);
+        }
+        const double permeability = 999999;  // LBM::permeability default
+        put(b, permeability);
+    }
+    fwrite(b.data(), 1, b.size(), fl.fp);
+    const auto& F = h.fld;
+    std::vector<char> rec(mc ? MIXM_SIZE + ns * SPEC_SIZE : MIX1_SIZE);
+    auto wr = [&](size_t off, double x) { memcpy(rec.data() + off, &x, 8); };
+    for (int64_t cc = 0; cc < n; ++cc) {
+        std::fill(rec.begin(), rec.end(), 0);
+        const short t = (short)h.type[cc];
+        memcpy(rec.data(), &t, sizeof(short));
+        size_t o = 8;
+        if (!mc) {
+            for (int l = 0; l < Q; ++l) wr(o + 8 * l, f[(size_t)l * n + cc]);
+            o += 8 * Q;
+            for (int l = 0; l < Q; ++l) wr(o + 8 * l, fpc[(size_t)l * n + cc]);
+            o += 8 * Q;
+        }
+        // p_tensor[3][3], symmetric
+        const int pidx[9] = {LBM_FIELD_PXX, LBM_FIELD_PXY, LBM_FIELD_PXZ, LBM_FIELD_PXY, LBM_FIELD_PYY,
+                             LBM_FIELD_PYZ, LBM_FIELD_PXZ, LBM_FIELD_PYZ, LBM_FIELD_PZZ};
+        for (int q = 0; q < 9; ++q) wr(o + 8 * q, F[pidx[q]][cc]);
+        o += 72;
+        const int mfld[5] = {LBM_FIELD_RHO, LBM_FIELD_U, LBM_FIELD_V, LBM_FIELD_W, LBM_FIELD_P};
+        for (int q = 0; q < 5; ++q) wr(o + 8 * q, F[mfld[q]][cc]);
+        o += 40;
+        for (int l = 0; l < Q; ++l) wr(o + 8 * l, g[(size_t)l * n + cc]);
+        o += 8 * Q;
+        for (int l = 0; l < Q; ++l) wr(o + 8 * l, gpc[(size_t)l * n + cc]);
+        o += 8 * Q;
+        wr(o, F[LBM_FIELD_RHOE][cc]);
+        wr(o + 8, F[LBM_FIELD_QX][cc]);
+        wr(o + 16, F[LBM_FIELD_QY][cc]);
+        wr(o + 24, F[LBM_FIELD_QZ][cc]);
+        wr(o + 32, F[LBM_FIELD_T][cc]);
+        o += 40;
+        if (mc) {
+            wr(o, F[LBM_FIELD_HRR][cc]);
+            o += 8;
+            for (int a = 0; a < ns; ++a) {
+                const size_t an = (size_t)a * n + cc;
+                wr(o, h.sp[LBM_SP_X][an]);
+                wr(o + 8, h.sp[LBM_SP_RHO][an]);
+                for (int l = 0; l < Q; ++l) wr(o + 16 + 8 * l, f[((size_t)a * Q + l) * n + cc]);
+                for (int l = 0; l < Q; ++l) wr(o + 16 + 8 * Q + 8 * l, fpc[((size_t)a * Q + l) * n + cc]);
+                wr(o + 16 + 16 * Q, h.sp[LBM_SP_U][an]);
+                wr(o + 24 + 16 * Q, h.sp[LBM_SP_V][an]);
+                wr(o + 32 + 16 * Q, h.sp[LBM_SP_W][an]);
+                o += SPEC_SIZE;
+            }
+        }
+        fwrite(rec.data(), 1, rec.size(), fl.fp);
+    }
+    if (ferror(fl.fp)) return fail(c, LBM_E_ARG, std::string("write error on ") + path);
+    return LBM_OK;
+}
+
+int lbm_read_restart(lbm_ctx* c, const char* path, int* step_out)
+{
+    if (!c || !path) return fail(c, LBM_E_ARG, "null argument");
+    const bool mc = multi(c);
+    File fl;
+    fl.fp = fopen(path, "rb");
+    if (!fl.fp) return fail(c, LBM_E_ARG, std::string("cannot open ") + path);
+    auto rd = [&](void* p, size_t sz) { return fread(p, 1, sz, fl.fp) == sz; };
+    int hdr[8];
+    if (!rd(hdr, sizeof(hdr))) return fail(c, LBM_E_ARG, "truncated restart header");
+    if (hdr[3] != c->d.nx || hdr[4] != c->d.ny || hdr[5] != c->d.nz)
+        return fail(c, LBM_E_ARG, "restart grid " + std::to_string(hdr[3]) + "x" + std::to_string(hdr[4]) + "x" +
+                                      std::to_string(hdr[5]) + " differs from the context's (ghosts included)");
+    const int64_t n = c->d.n;
+    const int ns = c->ns;
+    if (!mc) {
+        double k[5];
+        if (!rd(k, sizeof(k))) return fail(c, LBM_E_ARG, "truncated restart header");
+        if (k[0] != c->cfg.nu || k[1] != c->cfg.gas_const || k[2] != c->cfg.gamma || k[4] != c->cfg.prtl)
+            return fail(c, LBM_E_ARG, "restart gas constants (nu, R, gamma, Pr) differ from the context's");
+    } else {
+        size_t nsp = 0;
+        if (!rd(&nsp, sizeof(nsp)) || nsp != (size_t)ns) return fail(c, LBM_E_ARG, "restart species count differs");
+        for (int a = 0; a < ns; ++a) {
+            size_t len = 0;
+            if (!rd(&len, sizeof(len)) || len > 64) return fail(c, LBM_E_ARG, "bad species name in restart");
+            std::string name(len, '\0');
+            if (!rd(&name[0], len) || name != kSpeciesNames[a])
+                return fail(c, LBM_E_ARG, "restart species " + name + " differs from " + kSpeciesNames[a]);
+        }
+        double perm;
+        if (!rd(&perm, sizeof(perm))) return fail(c, LBM_E_ARG, "truncated restart header");
+    }
+    std::vector<uint8_t> type(n);
+    std::vector<double> fld[LBM_FIELD_COUNT];
+    for (auto& v : fld) v.assign(n, 0.0);
+    std::vector<double> f((size_t)n * Q * (mc ? ns : 1)), fpc(f.size()), g((size_t)n * Q), gpc(g.size());
+    std::vector<double> sp[5];
+    if (mc)
+        for (auto& v : sp) v.assign((size_t)n * ns, 0.0);
+    std::vector<char> rec(mc ? MIXM_SIZE + ns * SPEC_SIZE : MIX1_SIZE);
+    auto get = [&](size_t off) {
+        double x;
+        memcpy(&x, rec.data() + off, 8);
+        return x;
+    };
+    for (int64_t cc = 0; cc < n; ++cc) {
+        if (!rd(rec.data(), rec.size())) return fail(c, LBM_E_ARG, "truncated restart records");
+        short t;
+        memcpy(&t, rec.data(), sizeof(short));
+        if (t < 0 || t > TYPE_Q) return fail(c, LBM_E_ARG, "cell type out of range in restart");
+        type[cc] = (uint8_t)t;
+        size_t o = 8;
+        if (!mc) {
+            for (int l = 0; l < Q; ++l) f[(size_t)l * n + cc] = get(o + 8 * l);
+            o += 8 * Q;
+            for (int l = 0; l < Q; ++l) fpc[(size_t)l * n + cc] = get(o + 8 * l);
+            o += 8 * Q;
+        }
+        fld[LBM_FIELD_PXX][cc] = get(o);
+        fld[LBM_FIELD_PXY][cc] = get(o + 8);
+        fld[LBM_FIELD_PXZ][cc] = get(o + 16);
+        fld[LBM_FIELD_PYY][cc] = get(o + 32);
+        fld[LBM_FIELD_PYZ][cc] = get(o + 40);
+        fld[LBM_FIELD_PZZ][cc] = get(o + 64);
+        o += 72;
+        const int mfld[5] = {LBM_FIELD_RHO, LBM_FIELD_U, LBM_FIELD_V, LBM_FIELD_W, LBM_FIELD_P};
+        for (int q = 0; q < 5; ++q) fld[mfld[q]][cc] = get(o + 8 * q);
+        o += 40;
+        for (int l = 0; l < Q; ++l) g[(size_t)l * n + cc] = get(o + 8 * l);
+        o += 8 * Q;
+        for (int l = 0; l < Q; ++l) gpc[(size_t)l * n + cc] = get(o + 8 * l);
+        o += 8 * Q;
+        fld[LBM_FIELD_RHOE][cc] = get(o);
+        fld[LBM_FIELD_QX][cc] = get(o + 8);
+        fld[LBM_FIELD_QY][cc] = get(o + 16);
+        fld[LBM_FIELD_QZ][cc] = get(o + 24);
+        fld[LBM_FIELD_T][cc] = get(o + 32);
+        o += 40;
+        if (mc) {
+            fld[LBM_FIELD_HRR][cc] = get(o);
+            o += 8;
+            for (int a = 0; a < ns; ++a) {
+                const size_t an = (size_t)a * n + cc;
+                sp[LBM_SP_X][an] = get(o);
+                sp[LBM_SP_RHO][an] = get(o + 8);
+                for (int l = 0; l < Q; ++l) f[((size_t)a * Q + l) * n + cc] = get(o + 16 + 8 * l);
+                for (int l = 0; l < Q; ++l) fpc[((size_t)a * Q + l) * n + cc] = get(o + 16 + 8 * Q + 8 * l);
+                sp[LBM_SP_U][an] = get(o + 16 + 16 * Q);
+                sp[LBM_SP_V][an] = get(o + 24 + 16 * Q);
+                sp[LBM_SP_W][an] = get(o + 32 + 16 * Q);
+                o += SPEC_SIZE;
+            }
+        }
+    }
+    if (int rc = lbm_set_types(c, type.data())) return rc;
+    for (int q = 0; q < LBM_FIELD_COUNT; ++q)
+        if (int rc = lbm_set_field(c, q, fld[q].data())) return rc;
+    if (mc)
+        for (int q = 0; q < 5; ++q)
+            if (int rc = lbm_set_species_field(c, q, sp[q].data())) return rc;
+    // the post-collision sets matter to the strict calls (Streaming reads
+    // them); fused mode resumes from f, g and the moments
+    if (!fused(c) || mc) {
+        if (int rc = lbm_set_populations(c, LBM_POP_FPC, fpc.data())) return rc;
+        if (int rc = lbm_set_populations(c, LBM_POP_GPC, gpc.data())) return rc;
+    }
+    if (int rc = lbm_set_populations(c, LBM_POP_F, f.data())) return rc;
+    if (int rc = lbm_set_populations(c, LBM_POP_G, g.data())) return rc;
+    if (step_out) *step_out = hdr[7];
+    return LBM_OK;
+}

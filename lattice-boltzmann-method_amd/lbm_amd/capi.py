@@ -31,8 +31,10 @@ EXPORTS = [
     "lbm_get_species_field", "lbm_surrogate_param_count", "lbm_surrogate_init", "lbm_surrogate_capture",
     "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
     "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
-    "lbm_surrogate_forward", "lbm_surrogate_gradient",
+    "lbm_surrogate_forward", "lbm_surrogate_gradient", "lbm_write_vtk", "lbm_write_restart", "lbm_read_restart",
 ]
+
+LBM_VTK_SI, LBM_VTK_MASS_FRACTIONS = 1, 2
 
 # surrogate network (include/lbm_amd.h): layer widths, real inputs / outputs
 SUR_DIMS = (16, 128, 128, 128, 16)
@@ -118,6 +120,9 @@ def load_library(path=LIB_PATH):
     for fn in ("lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm",
                "lbm_surrogate_set_norm"):
         getattr(L, fn).argtypes = [vp, fp]
+    L.lbm_write_vtk.argtypes = [vp, ctypes.c_char_p, i32]
+    L.lbm_write_restart.argtypes = [vp, ctypes.c_char_p, i32]
+    L.lbm_read_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(i32)]
     L.lbm_surrogate_forward.argtypes = [vp, fp, i64, fp]
     L.lbm_surrogate_gradient.argtypes = [vp, fp, fp, i64, fp, fp]
     _LIB = L
@@ -315,6 +320,27 @@ class Lattice:
 
     def device_bytes(self):
         return self._L.lbm_device_bytes(self._h)
+
+    # -- output files (output.cpp, restart_file.cpp)
+    def OutputVTK(self, nout, directory=".", si=False, mass_fractions=False):
+        """Writes field%06d.vtr as OutputVTK does (output.cpp:9-210); returns its path."""
+        path = os.path.join(directory, "field%06d.vtr" % nout)
+        flags = (LBM_VTK_SI if si else 0) | (LBM_VTK_MASS_FRACTIONS if mass_fractions else 0)
+        self._check(self._L.lbm_write_vtk(self._h, path.encode(), flags), "write_vtk")
+        return path
+
+    def write_restart(self, step, directory="."):
+        """Writes restart%06d.dat in the reference's layout (restart_file.cpp:274-350); returns its path."""
+        path = os.path.join(directory, "restart%06d.dat" % step)
+        self._check(self._L.lbm_write_restart(self._h, path.encode(), int(step)), "write_restart")
+        return path
+
+    def read_restart(self, path):
+        """Restores cell types, fields and populations from a restart file
+        (restart_file.cpp:352-440); returns its step."""
+        step = ctypes.c_int()
+        self._check(self._L.lbm_read_restart(self._h, str(path).encode(), ctypes.byref(step)), "read_restart")
+        return step.value
 
     # -- chemistry surrogate network (the network calls work on any context)
     def surrogate_init(self, seed):

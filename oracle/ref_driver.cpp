@@ -20,7 +20,13 @@
 //                                     inflow/outflow/wall channel (every single-
 //                                     component TMS_BC branch); a negative snap
 //                                     -s dumps step s before TMS_BC
+//   io <nx> <ny> <nz> <nsteps> <outdir> ...channel params
+//                                     the channel's VTK and restart files written
+//                                     by the reference's OutputVTK/write_restart
 #include "lbm.hpp"
+#include "output.hpp"
+#include "restart_file.hpp"
+#include <unistd.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -299,6 +305,36 @@ static int mode_channel(int argc, char** argv)
     return 0;
 }
 
+// io <nx> <ny> <nz> <nsteps> <outdir> <U> <nu> <R> <gamma> <Pr> <T0> <p0> <Tin> <Tw> <pout>
+// The channel after nsteps steps, written by the reference's own writers
+// (OutputVTK: field%06d.vtr; write_restart: restart%06d.dat) into outdir.
+static int mode_io(int argc, char** argv)
+{
+    if (argc < 16) return 2;
+    ChannelParams P;
+    P.nx = atoi(argv[2]); P.ny = atoi(argv[3]); P.nz = atoi(argv[4]);
+    int nsteps = atoi(argv[5]);
+    std::string outdir(argv[6]);
+    P.U = atof(argv[7]); P.nu = atof(argv[8]); P.R = atof(argv[9]); P.gamma = atof(argv[10]);
+    P.Pr = atof(argv[11]); P.T0 = atof(argv[12]); P.p0 = atof(argv[13]); P.Tin = atof(argv[14]);
+    P.Tw = atof(argv[15]); P.pout = argc > 16 ? atof(argv[16]) : P.p0;
+    LBM lb(P.nx, P.ny, P.nz, P.nu);
+    setup_channel(lb, P);
+    lb.Init();
+    for (int step = 1; step <= nsteps; ++step) {
+        lb.Collide();
+        lb.Streaming();
+        lb.TMS_BC();
+        lb.calculate_moment();
+    }
+    if (chdir(outdir.c_str()) != 0) { perror(outdir.c_str()); return 3; }
+    int nout = nsteps, step = nsteps;
+    OutputVTK(nout, &lb);
+    write_restart(step, &lb);
+    dump_snapshot(lb, "snap.bin");
+    return 0;
+}
+
 static int mode_time(int argc, char** argv)
 {
     TgvParams P;
@@ -330,5 +366,6 @@ int main(int argc, char** argv)
     if (!strcmp(argv[1], "tgv")) return mode_tgv(argc, argv);
     if (!strcmp(argv[1], "time")) return mode_time(argc, argv);
     if (!strcmp(argv[1], "channel")) return mode_channel(argc, argv);
+    if (!strcmp(argv[1], "io")) return mode_io(argc, argv);
     return 2;
 }

@@ -24,6 +24,9 @@ Fixtures
                      one moving), periodic in z: interior moments after
                      steps 0, 1, 2, 10, 40, 100; populations after steps 1,
                      2, 10; and the state of step 2 before TMS_BC
+  channel_io.npz     6x3x3 channel after 2 steps, written by the reference's
+                     OutputVTK (field000002.vtr) and write_restart
+                     (restart000002.dat): the two files' bytes
 """
 import math
 import os
@@ -126,6 +129,14 @@ def main():
             arrays[f"f_{key}"] = sn[s][:, 19:46]
             arrays[f"g_{key}"] = sn[s][:, 46:73]
     np.savez_compressed(os.path.join(HERE, "channel.npz"), **arrays)
+    io = dict(CHANNEL, nx=6, ny=3, nz=3)
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [DRIVER, "io", str(io["nx"]), str(io["ny"]), str(io["nz"]), "2", d] + [repr(io[k]) for k in CHANNEL_KEYS[3:]]
+        subprocess.run(cmd, check=True, env=env, stdout=subprocess.DEVNULL)
+        vtk = np.frombuffer(open(os.path.join(d, "field000002.vtr"), "rb").read(), np.uint8)
+        rst = np.frombuffer(open(os.path.join(d, "restart000002.dat"), "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "channel_io.npz"), vtk=vtk, restart=rst, steps=np.array(2),
+                        params=np.array([io[k] for k in CHANNEL_KEYS], np.float64))
     print("golden fixtures written to", HERE)
 
 
