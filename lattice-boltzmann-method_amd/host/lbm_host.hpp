@@ -100,6 +100,26 @@ public:
     void TMS_BC() { check(lbm_tms_bc(ctx_), "TMS_BC"); }
     void calculate_moment() { check(lbm_moments(ctx_), "calculate_moment"); }
 
+    // OutputVTK (output.cpp:9-210) and write_restart / read_restart
+    // (restart_file.cpp:274-440) from the device state, same file names and formats
+    void OutputVTK(int nout, int flags = 0)
+    {
+        char name[64];
+        snprintf(name, sizeof(name), "./field%06d.vtr", nout);
+        check(lbm_write_vtk(ctx_, name, flags), "OutputVTK");
+    }
+    void write_restart(int s)
+    {
+        char name[64];
+        snprintf(name, sizeof(name), "./restart%06d.dat", s);
+        check(lbm_write_restart(ctx_, name, s), "write_restart");
+    }
+    void read_restart(const std::string& file)
+    {
+        ensure_ctx();
+        check(lbm_read_restart(ctx_, file.c_str(), &step), "read_restart");
+    }
+
     // LBM::run (lbm-main.cpp:126-147): Init, output at t = 0, loop.
     void run(int nstep, int tout, const std::function<void(int, LBM&)>& output)
     {
