@@ -70,7 +70,9 @@ enum lbm_species_field { LBM_SP_X = 0, LBM_SP_RHO = 1, LBM_SP_U = 2, LBM_SP_V = 
 
 typedef struct {
     int nx, ny, nz;        /* interior cells; ghosts are added as in LBM::LBM (lbm-main.cpp:19-21) */
-    int n_species;         /* 0: single-component path; 9: the H2-air mixture (H2 H O O2 OH H2O HO2 H2O2 N2) */
+    int n_species;         /* 0: single-component path; else the species count of the build's
+                              mechanism (lbm_mechanism): 9 for H2-air (H2 H O O2 OH H2O HO2
+                              H2O2 N2), 6 for the two-step methane and propane mechanisms */
     int precision;         /* enum lbm_precision */
     int mode;              /* enum lbm_mode */
     int device;            /* HIP device ordinal */
@@ -103,6 +105,17 @@ void lbm_destroy(lbm_ctx* ctx);
 const char* lbm_last_error(const lbm_ctx* ctx);
 /* Text of the last lbm_create failure in this thread. */
 const char* lbm_create_error(void);
+
+/* The chemical mechanism this build of the library is compiled for, in place
+ * of the reference's compile-time Cantera::newSolution choice
+ * (lbm-main.cpp:83-88): its name ("h2air", "ch4_2s" or "c3h8"), with the
+ * species count in *n_species when that pointer is not null.  Each mechanism
+ * is its own library (liblbm_amd.so, liblbm_amd_ch4_2s.so, liblbm_amd_c3h8.so);
+ * the single-component path is the same in all of them. */
+const char* lbm_mechanism(int* n_species);
+/* Name of species k in the order of every per-species array of this API
+ * (gas->speciesName(k), setup.cpp); null when k is out of range. */
+const char* lbm_species_name(int k);
 
 /* Cell types, uint8 per ghost-inclusive cell (MIXTURE::type, defines.hpp:136-145). */
 int lbm_set_types(lbm_ctx* ctx, const uint8_t* types);

@@ -10,7 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
-#include "mech_h2air.h"
+#include "mech.h"
 
 namespace lbm {
 namespace chem {
@@ -199,6 +199,19 @@ __host__ __device__ inline double pressure(double T, double rho, const double Y[
 }
 
 // ------------------------------------------------------------ kinetics
+// Concentration factor C^o of a rate of progress: plain products for the
+// exponents 0-3 (any sign of C), otherwise (fractional reaction orders) C^o
+// when C > 0, else 0 (Cantera's rule for non-integer exponents).  The
+// exponents are table constants, so the branches fold at compile time.
+__host__ __device__ inline double conc_power(double c, double o)
+{
+    if (o == 0.0) return 1.0;
+    if (o == 1.0) return c;
+    if (o == 2.0) return c * c;
+    if (o == 3.0) return c * c * c;
+    return c > 0.0 ? pow(c, o) : 0.0;
+}
+
 // Net molar production rates [kmol/m^3/s] at (T, rho, Y), SI units.
 __host__ __device__ inline void net_production(double T, double rho, const double Y[NS], double wdot[NS])
 {
@@ -233,24 +246,20 @@ __host__ __device__ inline void net_production(double T, double rho, const doubl
                 kf = kf * Pr / (1.0 + Pr) * exp10(lFc / (1.0 + f1 * f1));
             }
         }
-        double dG = 0, cf = 1, cr = 1;
-        int dn = 0;
+        double dG = 0, dn = 0, cf = 1, cr = 1;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const int nf = NU_F[r][k], nr = NU_R[r][k];
-            dG += (nr - nf) * gRT[k];
-            dn += nr - nf;
-            if (nf == 1) cf *= C[k];
-            if (nf == 2) cf *= C[k] * C[k];
-            if (nr == 1) cr *= C[k];
-            if (nr == 2) cr *= C[k] * C[k];
+            dG += (NU_R[r][k] - NU_F[r][k]) * gRT[k];
+            dn += NU_R[r][k] - NU_F[r][k];
+            cf *= conc_power(C[k], ORD_F[r][k]);
+            cr *= conc_power(C[k], NU_R[r][k]);
         }
-        // k_r = k_f / K_c, K_c = exp(-dG) (p_ref / RT)^dn
-        const double kr = kf * exp(dG - dn * ln_pref_RT);
+        // k_r = k_f / K_c, K_c = exp(-dG) (p_ref / RT)^dn; 0 for irreversible steps
+        const double kr = REVERSIBLE[r] ? kf * exp(dG - dn * ln_pref_RT) : 0.0;
         const double q = kf * cf - kr * cr;
 #pragma unroll
         for (int k = 0; k < NS; ++k)
-            if (NU_R[r][k] - NU_F[r][k]) wdot[k] += (NU_R[r][k] - NU_F[r][k]) * q;
+            if (NU_R[r][k] - NU_F[r][k] != 0.0) wdot[k] += (NU_R[r][k] - NU_F[r][k]) * q;
     }
 }
 

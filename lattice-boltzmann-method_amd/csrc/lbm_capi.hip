@@ -374,6 +374,14 @@ int check_field(lbm_ctx* c, int field)
 extern "C" {
 
 const char* lbm_create_error(void) { return g_create_error.c_str(); }
+
+const char* lbm_mechanism(int* n_species)
+{
+    if (n_species) *n_species = mech::NS;
+    return mech::MECH_NAME;
+}
+
+const char* lbm_species_name(int k) { return (k >= 0 && k < mech::NS) ? mech::SPECIES_NAMES[k] : nullptr; }
 const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int lbm_create(const lbm_config* cfg, lbm_ctx** out)
@@ -389,7 +397,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return LBM_E_ARG;
     }
     if (cfg->n_species != 0 && cfg->n_species != mech::NS) {
-        g_create_error = "n_species must be 0 (single component) or 9 (the H2-air mechanism)";
+        g_create_error = std::string("n_species must be 0 (single component) or ") + std::to_string(mech::NS) +
+                         " (the " + mech::MECH_NAME + " mechanism of this library)";
         return LBM_E_UNSUPPORTED;
     }
     if (cfg->n_species == mech::NS) {
@@ -1429,7 +1438,7 @@ int lbm_surrogate_gradient(lbm_ctx* c, const float* X, const float* T, int64_t m
 // the reference's byte layout.
 namespace {
 
-const char* const kSpeciesNames[mech::NS] = {"H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"};
+const char* const* const kSpeciesNames = mech::SPECIES_NAMES;
 
 // Units_Conv of the context; the single-component reference keeps the
 // default m = kg = s = K = 1 (units.hpp:10).

@@ -260,16 +260,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chemistry(MultiView<P> v)
             row[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
 #pragma unroll
             for (int a = 0; a < NS; ++a) row[2 + a] = (float)s.Y0[a];
-            row[11] = v.u.s;
+            row[sur::NIN - 1] = v.u.s;
 #pragma unroll
-            for (int q = 12; q < 16; ++q) row[q] = 0.f;
+            for (int q = sur::NIN; q < 16; ++q) row[q] = 0.f;
             // dT from the exact temperature of the initial state (Tsi carries
             // the binary32 rounding of the lattice-unit crossing)
             row[16] = (float)(Tcur - T_from_UV(s.usi, s.Y0, s.Tsi));
 #pragma unroll
             for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / s.rho);
 #pragma unroll
-            for (int q = 26; q < 32; ++q) row[q] = 0.f;
+            for (int q = 16 + sur::NOUT; q < 32; ++q) row[q] = 0.f;
         }
     }
 }
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_sur_features(MultiView<P> v, uint1
         raw[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
 #pragma unroll
         for (int a = 0; a < NS; ++a) raw[2 + a] = (float)s.Y0[a];
-        raw[11] = v.u.s;
+        raw[NIN - 1] = v.u.s;
     }
     bool outside = false;
     uint32_t packed[8];
@@ -338,6 +338,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_sur_decode(MultiView<P> v, const f
         chem_state(v, c, false, s);
         const float4* pr4 = reinterpret_cast<const float4*>(Y + (int64_t)c * 16);
         const float4 y0 = pr4[0], y1 = pr4[1], y2 = pr4[2];
+        static_assert(NOUT <= 12, "three float4 loads cover the real outputs");
         const float pr[12] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w, y2.x, y2.y, y2.z, y2.w};
         // the imbalance of the predicted sources is removed in proportion to each
         // species' target scale (an inert species, scale 0, keeps its mean)

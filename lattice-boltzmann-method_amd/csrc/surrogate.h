@@ -3,13 +3,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mech.h"
+
 namespace lbm {
 namespace sur {
 
 constexpr int NL = 4;                              // weight layers
 constexpr int DIMS[NL + 1] = {16, 128, 128, 128, 16};
-constexpr int NIN = 12;                            // T, p, Y_k (9), dt
-constexpr int NOUT = 10;                           // dT, dY_k (9)
+constexpr int NIN = mech::NS + 3;                  // T, p, Y_k, dt
+constexpr int NOUT = mech::NS + 1;                 // dT, dY_k
+static_assert(NIN <= 16 && NOUT <= 12, "features and targets must fit the 16-wide layers");
 
 // FP32 master weights W[L] [DIMS[L+1]][DIMS[L]], biases, Adam moments and the
 // BF16 copies (row-major and transposed) the MFMA kernels read.
@@ -31,8 +34,8 @@ struct Work {
     float* loss_rows = nullptr;
 };
 
-// Capture rows [rows][32]: raw features T [K], p [Pa], Y_k (9), dt [s], 0 x4 |
-// targets dT [K], dY_k (9) over one step, 0 x6.
+// Capture rows [rows][32]: raw features T [K], p [Pa], Y_k (NS), dt [s], zero
+// padding to 16 | targets dT [K], dY_k (NS) over one step, zero padding.
 constexpr int CAP_STRIDE = 32;
 // Normalisation and envelope record [96]: feature mean, std, target mean,
 // std, feature min, max (16 each).

@@ -2,8 +2,8 @@
 // trained and evaluated on the GPU with BF16 matrix-core (MFMA) GEMMs and
 // FP32 accumulation, Adam on FP32 master weights.
 //
-// Layers: 16 (12 features, zero-padded) -> 128 -> 128 -> 128 -> 16 (10
-// outputs, zero-padded), GELU (tanh form) on the hidden layers.  Every training GEMM is
+// Layers: 16 (NIN = NS + 3 features, zero-padded) -> 128 -> 128 -> 128 -> 16
+// (NOUT = NS + 1 outputs, zero-padded), GELU (tanh form) on the hidden layers.  Every training GEMM is
 // C[m][n] = sum_k A[m][k] B[n][k] (+ bias) on 16x16 output tiles, one
 // wavefront per tile, v_mfma_f32_16x16x16_bf16 with 4 BF16 of A and of B per
 // lane per K-step of 16 (operand layout in surrogate_device.h).

@@ -1,4 +1,9 @@
-"""ctypes binding of liblbm_amd.so and a host-side mirror of the reference's LBM class.
+"""ctypes binding of the HIP library and a host-side mirror of the reference's LBM class.
+
+One library per chemical mechanism (the reference binds its Cantera
+mechanism at compile time, lbm-main.cpp:83-88): liblbm_amd.so (H2-air, 9
+species), liblbm_amd_ch4_2s.so (two-step methane), liblbm_amd_c3h8.so
+(two-step propane); the single-component path is the same in each.
 
 `Lattice` keeps the reference's member names and step-call structure
 (lbm.hpp:104-274, lbm-main.cpp:126-196): Init, Collide_Species, Collide,
@@ -13,7 +18,9 @@ import os
 import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "liblbm_amd.so")
+LIB_NAMES = {"h2air": "liblbm_amd.so", "ch4_2s": "liblbm_amd_ch4_2s.so", "c3h8": "liblbm_amd_c3h8.so"}
+MECHANISMS = tuple(LIB_NAMES)
+LIB_PATH = os.path.join(_PKG_DIR, LIB_NAMES["h2air"])
 
 LBM_FP64, LBM_FP32_SHIFTED = 0, 1
 LBM_MODE_FUSED, LBM_MODE_STRICT = 0, 1
@@ -32,15 +39,28 @@ EXPORTS = [
     "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
     "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
     "lbm_surrogate_forward", "lbm_surrogate_gradient", "lbm_write_vtk", "lbm_write_restart", "lbm_read_restart",
+    "lbm_mechanism", "lbm_species_name",
 ]
 
 LBM_VTK_SI, LBM_VTK_MASS_FRACTIONS = 1, 2
 
-# surrogate network (include/lbm_amd.h): layer widths, real inputs / outputs
+# surrogate network (include/lbm_amd.h): layer widths; real inputs T, p, Y_k, dt
+# and outputs dT, dY_k, for the species of the library's mechanism
 SUR_DIMS = (16, 128, 128, 128, 16)
-SUR_NIN, SUR_NOUT = 12, 10
-SUR_FEATURES = ["T", "p"] + [f"Y_{s}" for s in ("H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2")] + ["dt"]
-SUR_TARGETS = ["dT"] + [f"dY_{s}" for s in ("H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2")]
+H2AIR_SPECIES = ("H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2")
+
+
+def surrogate_features(species):
+    return ["T", "p"] + [f"Y_{s}" for s in species] + ["dt"]
+
+
+def surrogate_targets(species):
+    return ["dT"] + [f"dY_{s}" for s in species]
+
+
+SUR_FEATURES = surrogate_features(H2AIR_SPECIES)
+SUR_TARGETS = surrogate_targets(H2AIR_SPECIES)
+SUR_NIN, SUR_NOUT = len(SUR_FEATURES), len(SUR_TARGETS)
 
 
 class LbmError(RuntimeError):
@@ -73,16 +93,24 @@ class Diag(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-_LIB = None
+_LIBS = {}
 
 
-def load_library(path=LIB_PATH):
-    """Loads liblbm_amd.so; raises if it is not built."""
-    global _LIB
-    if _LIB is not None:
-        return _LIB
+def library_path(mechanism="h2air"):
+    if mechanism not in LIB_NAMES:
+        raise LbmError(f"unknown mechanism {mechanism!r}: one of {', '.join(MECHANISMS)}")
+    return os.path.join(_PKG_DIR, LIB_NAMES[mechanism])
+
+
+def load_library(path=None, mechanism="h2air"):
+    """Loads the library of `mechanism` (or the one at `path`); raises if it
+    is not built."""
+    path = path or library_path(mechanism)
+    if path in _LIBS:
+        return _LIBS[path]
     if not os.path.exists(path):
-        raise LbmError(f"{path} is not built: run __graft_entry__.build() or `make -C lattice-boltzmann-method_amd`")
+        raise LbmError(f"{path} is not built: run __graft_entry__.build() or `make -C lattice-boltzmann-method_amd"
+                       f"{'' if mechanism == 'h2air' else ' MECH=' + mechanism}`")
     L = ctypes.CDLL(path)
     vp, i32, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
     L.lbm_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
@@ -125,8 +153,19 @@ def load_library(path=LIB_PATH):
     L.lbm_read_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(i32)]
     L.lbm_surrogate_forward.argtypes = [vp, fp, i64, fp]
     L.lbm_surrogate_gradient.argtypes = [vp, fp, fp, i64, fp, fp]
-    _LIB = L
+    L.lbm_mechanism.argtypes = [ctypes.POINTER(i32)]
+    L.lbm_mechanism.restype = ctypes.c_char_p
+    L.lbm_species_name.argtypes = [i32]
+    L.lbm_species_name.restype = ctypes.c_char_p
+    _LIBS[path] = L
     return L
+
+
+def mechanism_info(lib):
+    """(name, species) of the mechanism a loaded library is built for."""
+    n = ctypes.c_int()
+    name = lib.lbm_mechanism(ctypes.byref(n)).decode()
+    return name, [lib.lbm_species_name(k).decode() for k in range(n.value)]
 
 
 def _dp(a):
@@ -389,14 +428,15 @@ class Lattice:
 
 
 class MixtureLattice(Lattice):
-    """Multicomponent (9-species H2-air) lattice, mirroring
-    LBM(Nx, Ny, Nz, species) of lbm-main.cpp:38-118 with units.set_m_kg_s(dx, dt)."""
-
-    SPECIES = ["H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"]
+    """Multicomponent lattice, mirroring LBM(Nx, Ny, Nz, species) of
+    lbm-main.cpp:38-118 with units.set_m_kg_s(dx, dt), on the library of
+    `mechanism` ("h2air": 9 species, "ch4_2s" and "c3h8": 6).  `SPECIES` lists
+    the species in the order of every per-species array."""
 
     def __init__(self, nx, ny, nz, dx_si, dt_si, reaction=True, mode=LBM_MODE_STRICT, precision=LBM_FP64, device=0,
-                 gamma=1.4):
-        self._L = load_library()
+                 gamma=1.4, mechanism="h2air"):
+        self._L = load_library(mechanism=mechanism)
+        self.mechanism, self.SPECIES = mechanism_info(self._L)
         cfg = Config(nx, ny, nz, len(self.SPECIES), precision, mode, device, 0.0, 0.0, gamma, 0.0, 1.0, 1.0,
                      dx_si, dt_si, 1 if reaction else 0)
         self._create(cfg)

@@ -155,16 +155,22 @@ def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.
 
 # ----------------------------------------------------------- multicomponent
 H2AIR_SPECIES = ["H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"]
+# species order of the two-step mechanisms (the libraries' lbm_species_name)
+MECH_SPECIES = {"h2air": H2AIR_SPECIES, "ch4_2s": ["O2", "H2O", "CH4", "CO", "CO2", "N2"],
+                "c3h8": ["CO", "C3H8", "CO2", "H2O", "N2", "O2"]}
 # phi = 1 H2-air, unburnt mole fractions (setup.cpp:2108-2116)
 X_UNBURNT_PHI1 = {"H2": 0.29586, "O2": 0.14793, "N2": 0.55621}
 X_AIR = {"O2": 0.21, "N2": 0.79}
+# stoichiometric fuel-air mixtures of the two-step mechanisms (air = O2 + 3.76 N2)
+X_CH4_AIR_PHI1 = {"CH4": 1.0 / 10.52, "O2": 2.0 / 10.52, "N2": 7.52 / 10.52}
+X_C3H8_AIR_PHI1 = {"C3H8": 1.0 / 24.8, "O2": 5.0 / 24.8, "N2": 18.8 / 24.8}
 UNITS_FLAME = (5e-6, 1e-9)  # units.set_m_kg_s(dx, dt) of the flame cases (setup.cpp:2019)
 LATTICE_K = 12000.0          # temperature unit of set_m_kg_s(dx, dt) (units.hpp:36)
 ONE_ATM = 101325.0
 
 
-def mole_vector(d, minim=1e-10):
-    x = np.array([d.get(n, minim) for n in H2AIR_SPECIES], dtype=np.float64)
+def mole_vector(d, minim=1e-10, species=None):
+    x = np.array([d.get(n, minim) for n in (species or H2AIR_SPECIES)], dtype=np.float64)
     return x / x.sum()
 
 
@@ -176,13 +182,15 @@ def lattice_p(p_si, dx=UNITS_FLAME[0], dt=UNITS_FLAME[1]):
     return float(np.float32(p_si * (float(m) * float(s) ** 2) / float(kg)))
 
 
-def reacting_box(nx, ny, nz, T_K=1400.0, X=None, amp_T=0.0, amp_u=0.0, seed=20261206):
+def reacting_box(nx, ny, nz, T_K=1400.0, X=None, amp_T=0.0, amp_u=0.0, seed=20261206, species=None):
     """Periodic box of premixed gas (the PERFECTLY_STIRRED_REACTOR_3D shape,
     setup.cpp:950-999, with optional smooth temperature and velocity
-    perturbations).  Returns fields with species mole fractions 'X' (NS, ...)."""
+    perturbations).  Returns fields with species mole fractions 'X' (NS, ...)
+    in the order of `species` (H2-air by default)."""
+    species = species or H2AIR_SPECIES
     fl = default_fields(nx, ny, nz)
     periodic_ghosts(fl)
-    X = mole_vector(X or X_UNBURNT_PHI1)
+    X = mole_vector(X or X_UNBURNT_PHI1, species=species)
     shape = fl["type"].shape
     i, j, k = np.meshgrid(*(np.arange(s, dtype=np.float64) for s in shape), indexing="ij")
     m = fl["type"] == TYPE_F
@@ -192,7 +200,7 @@ def reacting_box(nx, ny, nz, T_K=1400.0, X=None, amp_T=0.0, amp_u=0.0, seed=2026
     fl["u"][m] = (amp_u * np.sin(2 * np.pi * j / ny))[m]
     fl["v"][m] = (amp_u * np.cos(2 * np.pi * i / nx))[m]
     fl["w"][m] = (0.5 * amp_u * np.sin(2 * np.pi * k / nz))[m]
-    fl["X"] = np.zeros((len(H2AIR_SPECIES),) + shape)
+    fl["X"] = np.zeros((len(species),) + shape)
     fl["X"][:, m] = X[:, None]
     return fl
 

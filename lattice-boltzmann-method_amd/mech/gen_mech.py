@@ -1,23 +1,34 @@
-"""Generates the H2-air mechanism tables of the multicomponent path.
+"""Generates the mechanism tables of the multicomponent path.
 
-    python lattice-boltzmann-method_amd/mech/gen_mech.py [path/to/nh3.yaml]
+    python lattice-boltzmann-method_amd/mech/gen_mech.py [h2air|ch4_2s|c3h8 ...] [--yaml-dir DIR]
 
-Source: the H2/O2 kinetics module (reactions 1-23) of the reference's
-src/reaction-mech/nh3.yaml with its NASA7 thermo and Lennard-Jones transport
-data for H2, H, O, O2, OH, H2O, HO2, H2O2 and N2.  AR and HE appear in that
-module only as third-body efficiencies; they are not species here and their
-efficiencies are dropped.
+Mechanisms (the reference selects one at compile time, lbm-main.cpp:83-88;
+here each gets its own table header and its own build of the library):
 
-Outputs (committed):
-  h2air.json             data for the CPU oracle and the tests
-  ../csrc/mech_h2air.h   constexpr tables for the HIP kernels
+  h2air   the H2/O2 kinetics module (reactions 1-23) of the reference's
+          src/reaction-mech/nh3.yaml with its NASA7 thermo and Lennard-Jones
+          transport data for H2, H, O, O2, OH, H2O, HO2, H2O2 and N2.  AR and HE
+          appear in that module only as third-body efficiencies; they are not
+          species here and their efficiencies are dropped.
+  ch4_2s  src/reaction-mech/CH4_2S.yaml (2-step CH4 BFER: O2, H2O, CH4, CO,
+          CO2, N2; an irreversible step with fractional reaction orders and a
+          reversible CO oxidation with a half-integer O2 coefficient).
+  c3h8    src/reaction-mech/propane_mech.yaml (2-step propane: CO, C3H8, CO2,
+          H2O, N2, O2).
 
-Units: SI with kmol (as Cantera): A in (m^3/kmol)^(order-1)/s, Ea/R in K,
-W in kg/kmol.  Transport properties are this generator's own kinetic-theory
+Outputs per mechanism (committed):
+  <name>.json             data for the CPU oracle and the tests
+  ../csrc/mech_<name>.h   constexpr tables for the HIP kernels
+
+Units: SI with kmol (as Cantera): A in (m^3/kmol)^(order-1)/s with order the
+sum of the forward concentration exponents (explicit `orders` where the
+mechanism gives them, else the reactant coefficients), Ea/R in K, W in
+kg/kmol.  Transport properties are this generator's own kinetic-theory
 evaluation (Chapman-Enskog with Neufeld collision integrals, Brokaw's
 polar correction, the polar/non-polar combining rule and the
 Warnatz/Kee thermal-conductivity model), fitted as ln(prop) = poly4(ln T)
-over 250-3500 K, the CHEMKIN way.
+over 250-3500 K, the CHEMKIN way (the mixture-averaged model for every
+mechanism).
 """
 import json
 import math
@@ -29,9 +40,16 @@ import numpy as np
 import yaml
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_YAML = "/root/reference/src/reaction-mech/nh3.yaml"
-SPECIES = ["H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"]
-N_REACTIONS = 23  # the \H2_O2 kinetics module
+DEFAULT_YAML_DIR = "/root/reference/src/reaction-mech"
+MECHS = {
+    "h2air": {"yaml": "nh3.yaml", "species": ["H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"],
+              "reactions": 23,  # the \H2_O2 kinetics module
+              "source": "nh3.yaml H2/O2 module, reactions 1-23"},
+    "ch4_2s": {"yaml": "CH4_2S.yaml", "species": ["O2", "H2O", "CH4", "CO", "CO2", "N2"], "reactions": None,
+               "source": "CH4_2S.yaml (CH4 BFER two-step)"},
+    "c3h8": {"yaml": "propane_mech.yaml", "species": ["CO", "C3H8", "CO2", "H2O", "N2", "O2"], "reactions": None,
+             "source": "propane_mech.yaml (propane two-step)"},
+}
 
 R_UNIV = 8314.46261815324  # J/(kmol K), Cantera's GasConstant
 ONE_ATM = 101325.0
@@ -40,7 +58,7 @@ AVOGADRO = 6.02214076e26  # 1/kmol
 CAL = 4.184
 DEBYE = 1e-21 / 299792458.0  # C m
 EPS0_4PI = 1.0 / 8.9875517923e9  # 4 pi eps0
-ATOMIC_W = {"H": 1.008, "O": 15.999, "N": 14.007}
+ATOMIC_W = {"H": 1.008, "O": 15.999, "N": 14.007, "C": 12.011}
 T_FIT = np.linspace(250.0, 3500.0, 200)
 
 
@@ -49,10 +67,10 @@ def load(path):
         return yaml.safe_load(f)
 
 
-def species_table(doc):
+def species_table(doc, names):
     by_name = {str(s["name"]): s for s in doc["species"]}
     out = []
-    for n in SPECIES:
+    for n in names:
         s = by_name[n]
         comp = s["composition"]
         W = sum(ATOMIC_W[e] * c for e, c in comp.items())
@@ -79,9 +97,10 @@ def parse_side(s):
         if t == "M":
             third = True
             continue
-        m = re.match(r"^(\d+)\s+(\S+)$", t)
-        if m:
-            nu, sp = int(m.group(1)), m.group(2)
+        m = re.match(r"^(\d+(?:\.\d*)?)\s*(\S+)$", t)
+        if m and not m.group(2)[0].isdigit():
+            nu, sp = float(m.group(1)), m.group(2)
+            nu = int(nu) if nu.is_integer() else nu
         else:
             nu, sp = 1, t
         st[sp] = st.get(sp, 0) + nu
@@ -93,19 +112,35 @@ def arrhenius_si(rc, order):
     return {"A": rc["A"] * (1e-3) ** (order - 1), "b": rc["b"], "Ea_R": rc["Ea"] * CAL * 1000.0 / R_UNIV}
 
 
-def reactions_table(doc):
+def reactions_table(doc, names, count):
     out = []
-    for r in doc["reactions"][:N_REACTIONS]:
-        eq = r["equation"]
-        lhs, rhs = eq.split("<=>")
+    for r in doc["reactions"][:count]:
+        eq = r["equation"].strip()
+        if "<=>" in eq:
+            lhs, rhs = eq.split("<=>")
+            reversible = True
+        elif "=>" in eq:
+            lhs, rhs = eq.split("=>")
+            reversible = False
+        else:
+            lhs, rhs = eq.split("=")
+            reversible = True
         reac, tb_l = parse_side(lhs)
         prod, tb_r = parse_side(rhs)
         for sp in list(reac) + list(prod):
-            assert sp in SPECIES, (eq, sp)
+            assert sp in names, (eq, sp)
         kind = r.get("type", "elementary")
-        order = sum(reac.values())
+        orders = dict(reac)
+        for k, v in r.get("orders", {}).items():
+            assert k in names, (eq, k)
+            orders[k] = v
+        order = sum(orders.values())
         e = {"equation": eq, "type": kind, "reactants": reac, "products": prod,
              "duplicate": bool(r.get("duplicate", False))}
+        if not reversible or "orders" in r:
+            # fields absent for the reversible mass-action reactions of the H2/O2 module
+            e["reversible"] = reversible
+            e["orders"] = orders
         if kind == "elementary":
             e["k"] = arrhenius_si(r["rate-constant"], order)
         elif kind == "three-body":
@@ -118,7 +153,7 @@ def reactions_table(doc):
         else:
             raise ValueError(kind)
         if kind in ("three-body", "falloff"):
-            eff = {k: v for k, v in r.get("efficiencies", {}).items() if k in SPECIES}
+            eff = {k: v for k, v in r.get("efficiencies", {}).items() if k in names}
             e["efficiencies"] = eff
         out.append(e)
     return out
@@ -246,18 +281,20 @@ def transport_fits(sp):
 # ------------------------------------------------------------- emitters
 def emit_header(data, path):
     sp, rx = data["species"], data["reactions"]
+    names = [s["name"] for s in sp]
     ns, nr = len(sp), len(rx)
     L = []
     w = L.append
-    w("// Generated by lattice-boltzmann-method_amd/mech/gen_mech.py from the H2/O2")
-    w("// module of nh3.yaml (reactions 1-23); do not edit.  SI units with kmol.")
+    w("// Generated by lattice-boltzmann-method_amd/mech/gen_mech.py from")
+    w("// " + data["source"] + "; do not edit.  SI units with kmol.")
     w("#pragma once")
     w("namespace lbm { namespace mech {")
     w(f"constexpr int NS = {ns};")
     w(f"constexpr int NR = {nr};")
     w("constexpr double R_UNIV = %.17g;" % R_UNIV)
     w("constexpr double P_REF = %.17g;" % ONE_ATM)
-    w("// species: " + ", ".join(s["name"] for s in sp))
+    w(f'constexpr const char* MECH_NAME = "{data["name"]}";')
+    w("constexpr const char* SPECIES_NAMES[NS] = {" + ", ".join(f'"{n}"' for n in names) + "};")
     for i, s in enumerate(sp):
         w(f"constexpr int {s['name']} = {i};")
     w("constexpr double W[NS] = {" + ", ".join("%.17g" % s["W"] for s in sp) + "};")
@@ -279,14 +316,25 @@ def emit_header(data, path):
     kinds = {"elementary": 0, "three-body": 1, "falloff": 2}
     nu_f = [[0] * ns for _ in range(nr)]
     nu_r = [[0] * ns for _ in range(nr)]
+    ord_f = [[0] * ns for _ in range(nr)]
     for r, e in enumerate(rx):
         for k, v in e["reactants"].items():
-            nu_f[r][SPECIES.index(k)] = v
+            nu_f[r][names.index(k)] = v
         for k, v in e["products"].items():
-            nu_r[r][SPECIES.index(k)] = v
+            nu_r[r][names.index(k)] = v
+        for k, v in e.get("orders", e["reactants"]).items():
+            ord_f[r][names.index(k)] = v
+
+    def table(rows):
+        return ", ".join("{" + ", ".join("%.17g" % x for x in row) + "}" for row in rows)
+
     w("constexpr int KIND[NR] = {" + ", ".join(str(kinds[e["type"]]) for e in rx) + "};")
-    w("constexpr int NU_F[NR][NS] = {" + ", ".join("{" + ", ".join(map(str, row)) + "}" for row in nu_f) + "};")
-    w("constexpr int NU_R[NR][NS] = {" + ", ".join("{" + ", ".join(map(str, row)) + "}" for row in nu_r) + "};")
+    w("// stoichiometric coefficients; forward concentration exponents (the mechanism's")
+    w("// explicit orders, else NU_F); reverse rates from K_c for REVERSIBLE reactions")
+    w("constexpr double NU_F[NR][NS] = {" + table(nu_f) + "};")
+    w("constexpr double NU_R[NR][NS] = {" + table(nu_r) + "};")
+    w("constexpr double ORD_F[NR][NS] = {" + table(ord_f) + "};")
+    w("constexpr int REVERSIBLE[NR] = {" + ", ".join(str(int(e.get("reversible", True))) for e in rx) + "};")
 
     def arr(e, key):
         k = e.get(key) or e.get("k")
@@ -310,23 +358,36 @@ def emit_header(data, path):
         str(1 if ("troe" in e and e["troe"]["T2"] is not None) else 0) for e in rx) + "};")
     w("// third-body efficiencies (1 for species not listed; unused for elementary)")
     w("constexpr double EFF[NR][NS] = {" + ", ".join(
-        "{" + ", ".join("%.17g" % e.get("efficiencies", {}).get(s, 1.0) for s in SPECIES) + "}" for e in rx) + "};")
+        "{" + ", ".join("%.17g" % e.get("efficiencies", {}).get(s, 1.0) for s in names) + "}" for e in rx) + "};")
     w("}}  // namespace lbm::mech")
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
 
-def main():
-    src = sys.argv[1] if len(sys.argv) > 1 else DEFAULT_YAML
-    doc = load(src)
-    sp = species_table(doc)
-    rx = reactions_table(doc)
-    data = {"source": "nh3.yaml H2/O2 module, reactions 1-23", "species": sp, "reactions": rx,
+def generate(name, yaml_dir):
+    m = MECHS[name]
+    doc = load(os.path.join(yaml_dir, m["yaml"]))
+    sp = species_table(doc, m["species"])
+    count = m["reactions"] if m["reactions"] is not None else len(doc["reactions"])
+    rx = reactions_table(doc, m["species"], count)
+    data = {"source": m["source"], "species": sp, "reactions": rx,
             "R_univ": R_UNIV, "P_ref": ONE_ATM, "transport": transport_fits(sp)}
-    with open(os.path.join(HERE, "h2air.json"), "w") as f:
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
         json.dump(data, f, indent=1)
-    emit_header(data, os.path.join(os.path.dirname(HERE), "csrc", "mech_h2air.h"))
-    print("species", len(sp), "reactions", len(rx), "max fit residual (ln)", data["transport"]["max_ln_residual"])
+    emit_header(dict(data, name=name), os.path.join(os.path.dirname(HERE), "csrc", f"mech_{name}.h"))
+    print(name, "species", len(sp), "reactions", len(rx), "max fit residual (ln)",
+          data["transport"]["max_ln_residual"])
+
+
+def main():
+    args = sys.argv[1:]
+    yaml_dir = DEFAULT_YAML_DIR
+    if "--yaml-dir" in args:
+        i = args.index("--yaml-dir")
+        yaml_dir = args[i + 1]
+        del args[i:i + 2]
+    for name in args or list(MECHS):
+        generate(name, yaml_dir)
 
 
 if __name__ == "__main__":

@@ -11,10 +11,13 @@ lbm-init.cpp:99-134).  Cantera itself (unpinned version >= 3.0, not vendored)
 is absent here, so this is a restatement of the published models:
   * NASA7 ideal-gas thermo, standard pressure 1 atm;
   * mass-action kinetics with Arrhenius, third-body efficiencies, Troe
-    falloff, reverse rates from equilibrium constants (Gibbs energies);
+    falloff, reverse rates from equilibrium constants (Gibbs energies) for
+    reversible reactions, explicit fractional forward orders (a
+    concentration factor C^o of 0 for C <= 0 when o is not an integer,
+    Cantera's rule for such exponents);
   * mixture-averaged transport: Wilke viscosity, the combination-averaged
     conductivity 1/2 (sum X l + 1/sum(X/l)), binary diffusion from the
-    ln-polynomial fits of mech/h2air.json.
+    ln-polynomial fits of mech/<mechanism>.json (h2air, ch4_2s, c3h8).
 Vectorised over cells: every state argument is an array (..., NS) or (...).
 Pinned by tests/test_oracle_chem.py (closed forms, conservation, the
 reference-held adiabatic flame state, equilibrium, brute force).
@@ -24,12 +27,20 @@ import os
 
 import numpy as np
 
-_MECH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lattice-boltzmann-method_amd",
-                     "mech", "h2air.json")
+_MECH_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lattice-boltzmann-method_amd",
+                         "mech")
+_MECH = os.path.join(_MECH_DIR, "h2air.json")
+
+
+def mechanism_path(name):
+    """JSON tables of a mechanism by name (h2air, ch4_2s, c3h8)."""
+    return os.path.join(_MECH_DIR, name + ".json")
 
 
 class Mechanism:
     def __init__(self, path=_MECH):
+        if not os.path.sep in path and not path.endswith(".json"):
+            path = mechanism_path(path)
         with open(path) as f:
             d = json.load(f)
         self.names = [s["name"] for s in d["species"]]
@@ -45,6 +56,8 @@ class Mechanism:
         idx = {n: i for i, n in enumerate(self.names)}
         self.nuf = np.zeros((self.nr, self.ns))
         self.nur = np.zeros((self.nr, self.ns))
+        self.ordf = np.zeros((self.nr, self.ns))  # forward concentration exponents
+        self.rev = np.ones(self.nr, dtype=bool)
         self.kind = []
         self.arr = np.zeros((self.nr, 3))
         self.low = np.zeros((self.nr, 3))
@@ -55,6 +68,9 @@ class Mechanism:
                 self.nuf[r, idx[k]] += v
             for k, v in e["products"].items():
                 self.nur[r, idx[k]] += v
+            for k, v in e.get("orders", e["reactants"]).items():
+                self.ordf[r, idx[k]] = v
+            self.rev[r] = e.get("reversible", True)
             self.kind.append(e["type"])
             k = e.get("k_inf") or e["k"]
             self.arr[r] = (k["A"], k["b"], k["Ea_R"])
@@ -206,7 +222,17 @@ class Mechanism:
         dnu = self.nur - self.nuf  # (NR, NS)
         dG = g_RT @ dnu.T
         Kc = np.exp(-dG) * (self.Pref / RT[..., None]) ** dnu.sum(axis=1)
-        kr = kf / Kc
-        Cf = np.prod(np.where(self.nuf > 0, C[..., None, :] ** self.nuf, 1.0), axis=-1)
-        Cr = np.prod(np.where(self.nur > 0, C[..., None, :] ** self.nur, 1.0), axis=-1)
+        kr = np.where(self.rev, kf / Kc, 0.0)
+        Cf = np.prod(_conc_power(C[..., None, :], self.ordf), axis=-1)
+        Cr = np.prod(_conc_power(C[..., None, :], self.nur), axis=-1)
         return kf * Cf, kr * Cr
+
+
+def _conc_power(C, o):
+    """C^o per (reaction, species): 1 for o = 0, plain powers for integer o,
+    and C^o for C > 0 else 0 for fractional o."""
+    C, o = np.broadcast_arrays(C, o)
+    frac = o != np.round(o)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        p = np.where(o == 0, 1.0, C ** o)
+    return np.where(frac & (C <= 0), 0.0, p)

@@ -28,11 +28,31 @@ def test_header_declares_the_step_calls():
         assert n in names
 
 
-def test_library_exports_every_declared_symbol():
-    lib = capi.load_library()
+@pytest.mark.parametrize("mechanism", capi.MECHANISMS)
+def test_library_exports_every_declared_symbol(mechanism):
+    lib = capi.load_library(mechanism=mechanism)
     missing = [n for n in sorted(declared_entry_points()) if not hasattr(lib, n)]
     assert not missing, missing
     assert set(capi.EXPORTS) <= declared_entry_points()
+
+
+@pytest.mark.parametrize("mechanism", capi.MECHANISMS)
+def test_library_reports_its_mechanism(mechanism):
+    from lbm_amd import cases
+    from oracle.chem import Mechanism
+    lib = capi.load_library(mechanism=mechanism)
+    name, species = capi.mechanism_info(lib)
+    assert name == mechanism
+    assert species == list(cases.MECH_SPECIES[mechanism]) == Mechanism(mechanism).names
+    assert lib.lbm_species_name(-1) is None and lib.lbm_species_name(len(species)) is None
+
+
+def test_species_count_must_match_the_library():
+    lib = capi.load_library(mechanism="ch4_2s")
+    cfg = capi.Config(4, 4, 4, 9, 0, 1, 0, 0.0, 0.0, 1.4, 0.0, 1.0, 1.0, 5e-6, 1e-9, 1)
+    h = ctypes.c_void_p()
+    assert lib.lbm_create(ctypes.byref(cfg), ctypes.byref(h)) != 0
+    assert b"ch4_2s" in lib.lbm_create_error()
 
 
 def test_config_struct_matches_header():
