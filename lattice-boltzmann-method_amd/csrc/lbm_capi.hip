@@ -65,7 +65,7 @@ struct lbm_ctx {
     // multicomponent state (ns > 0): species populations A/B, species fields
     int ns = 0;
     void* sf[2] = {nullptr, nullptr};
-    double* sp_block = nullptr;  // X, rho_a [NS][n] and u_a [3][NS][n]
+    void* sp_block = nullptr;  // X, rho_a [NS][n] and u_a [3][NS][n], element type of the populations
     chem::Units units{};
     // TMS_BC outflow interface cells (cell, direction) and unsupported boundary kinds
     int* tms_cell = nullptr;
@@ -80,7 +80,7 @@ struct lbm_ctx {
     // single-component TMS_BC interface cells (fluid cells next to TYPE_O/I/S)
     int* sbc_cell = nullptr;
     int sbc_count = 0;
-    double* src = nullptr;  // [NS][n] chemistry mass sources
+    void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
     double mfref[mech::NS * Q] = {}, mgref[Q] = {};
@@ -199,13 +199,14 @@ MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
     for (int i = 0; i < LBM_FIELD_COUNT; ++i) v.fld[i] = c->fld[i];
     v.hrr = c->fld[LBM_FIELD_HRR];
     const int64_t n = c->d.n;
-    v.X = c->sp_block;
-    v.rhoa = c->sp_block + (size_t)c->ns * n;
-    v.ua = c->sp_block + (size_t)2 * c->ns * n;
+    P* const sp = static_cast<P*>(c->sp_block);
+    v.X = sp;
+    v.rhoa = sp + (size_t)c->ns * n;
+    v.ua = sp + (size_t)2 * c->ns * n;
     v.u = c->units;
     v.reaction = c->cfg.reaction;
     v.tau_count = c->tau_count;
-    v.src = c->src;
+    v.src = static_cast<P*>(c->src);
     v.tsp = static_cast<P*>(c->tsp);
     v.tchem = c->tchem;
     v.step = c->step;
@@ -306,7 +307,12 @@ void set_reference_state(lbm_ctx* c)
 int set_multi_reference_state(lbm_ctx* c)
 {
     const int64_t n = c->d.n;
-    launch_mc_state_sums(c->type, c->fld[LBM_FIELD_T], c->fld[LBM_FIELD_P], c->sp_block, n, c->sums_part, c->stream);
+    if (fp32(c))
+        launch_mc_state_sums(c->type, c->fld[LBM_FIELD_T], c->fld[LBM_FIELD_P], static_cast<const float*>(c->sp_block),
+                             n, c->sums_part, c->stream);
+    else
+        launch_mc_state_sums(c->type, c->fld[LBM_FIELD_T], c->fld[LBM_FIELD_P], static_cast<const double*>(c->sp_block),
+                             n, c->sums_part, c->stream);
     std::vector<double> part((size_t)MC_SUMS * MC_SUM_BLOCKS);
     HIP_TRY(c, hipMemcpyAsync(part.data(), c->sums_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
                               c->stream));
@@ -482,11 +488,11 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
             if ((e = hipMalloc(&c->sf[b], sbytes)) != hipSuccess) return bail("hipMalloc species populations", e);
             (void)hipMemsetAsync(c->sf[b], 0, sbytes, c->stream);
         }
-        const size_t fb = sizeof(double) * (size_t)n * c->ns * 5;
-        if ((e = hipMalloc((void**)&c->sp_block, fb)) != hipSuccess) return bail("hipMalloc species fields", e);
+        const size_t fb = c->esize * (size_t)n * c->ns * 5;
+        if ((e = hipMalloc(&c->sp_block, fb)) != hipSuccess) return bail("hipMalloc species fields", e);
         (void)hipMemsetAsync(c->sp_block, 0, fb, c->stream);
-        const size_t srcb = sizeof(double) * (size_t)n * c->ns;
-        if ((e = hipMalloc((void**)&c->src, srcb)) != hipSuccess) return bail("hipMalloc chemistry sources", e);
+        const size_t srcb = c->esize * (size_t)n * c->ns;
+        if ((e = hipMalloc(&c->src, srcb)) != hipSuccess) return bail("hipMalloc chemistry sources", e);
         (void)hipMemsetAsync(c->src, 0, srcb, c->stream);
         const size_t tspb = c->esize * (size_t)n * c->ns * 4;
         if ((e = hipMalloc(&c->tsp, tspb)) != hipSuccess) return bail("hipMalloc species transport", e);
@@ -821,45 +827,62 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
 }
 
 namespace {
-int species_field_ptr(lbm_ctx* c, int field, double** dev, size_t* count)
+// Device address and element count of a species plane (elements of c->esize
+// bytes: binary32 in FP32 contexts).
+int species_field_ptr(lbm_ctx* c, int field, void** dev, size_t* count)
 {
     if (!multi(c)) return fail(c, LBM_E_STATE, "no species in this context");
     const int64_t n = c->d.n;
+    char* const base = static_cast<char*>(c->sp_block);
+    int plane;
     switch (field) {
-    case LBM_SP_X: *dev = c->sp_block; *count = (size_t)c->ns * n; return LBM_OK;
-    case LBM_SP_RHO: *dev = c->sp_block + (size_t)c->ns * n; *count = (size_t)c->ns * n; return LBM_OK;
+    case LBM_SP_X: plane = 0; break;
+    case LBM_SP_RHO: plane = 1; break;
     case LBM_SP_U:
     case LBM_SP_V:
-    case LBM_SP_W:
-        *dev = c->sp_block + (size_t)(2 + field - LBM_SP_U) * c->ns * n;
-        *count = (size_t)c->ns * n;
-        return LBM_OK;
+    case LBM_SP_W: plane = 2 + field - LBM_SP_U; break;
     default: return fail(c, LBM_E_ARG, "species field id out of range");
     }
+    *count = (size_t)c->ns * n;
+    *dev = base + (size_t)plane * *count * c->esize;
+    return LBM_OK;
 }
 }  // namespace
 
 int lbm_set_species_field(lbm_ctx* c, int field, const double* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
-    double* dev;
+    void* dev;
     size_t cnt;
     if (int rc = species_field_ptr(c, field, &dev, &cnt)) return rc;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
-    HIP_TRY(c, hipMemcpyAsync(dev, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (fp32(c)) {
+        std::vector<float> tmp(host, host + cnt);
+        HIP_TRY(c, hipMemcpyAsync(dev, tmp.data(), cnt * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(dev, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return LBM_OK;
 }
 
 int lbm_get_species_field(lbm_ctx* c, int field, double* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
-    double* dev;
+    void* dev;
     size_t cnt;
     if (int rc = species_field_ptr(c, field, &dev, &cnt)) return rc;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
-    HIP_TRY(c, hipMemcpyAsync(host, dev, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (fp32(c)) {
+        std::vector<float> tmp(cnt);
+        HIP_TRY(c, hipMemcpyAsync(tmp.data(), dev, cnt * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        std::copy(tmp.begin(), tmp.end(), host);
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(host, dev, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return LBM_OK;
 }
 

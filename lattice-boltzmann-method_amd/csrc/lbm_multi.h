@@ -17,15 +17,17 @@ struct MultiView {
     P* go;
     double* fld[LBM_FIELD_COUNT];
     double* hrr;   // [n] heat release rate
-    double* X;     // [NS][n] mole fractions
-    double* rhoa;  // [NS][n] species densities
-    double* ua;    // [3][NS][n] species velocities
+    // species planes, stored in P (binary32 in FP32 contexts: every kernel
+    // computes in binary64 from the loaded values)
+    P* X;     // [NS][n] mole fractions
+    P* rhoa;  // [NS][n] species densities
+    P* ua;    // [3][NS][n] species velocities
     chem::Units u;
     int reaction;
     unsigned long long* tau_count;
     const double* fref;  // FP32 storage: rest-state shifts [NS][27] of the species populations
     const double* gref;  // and [27] of the energy populations (unused for FP64)
-    double* src;    // [NS][n] species mass added by the chemistry over the step (lattice units)
+    P* src;         // [NS][n] species mass added by the chemistry over the step (lattice units)
     P* tsp;         // [4][NS][n] Collide_Species transport of the step: omega_a, then the
                     // Maxwell-Stefan friction impulse per axis
     double* tchem;  // [n] temperature after the chemistry [K] (surrogate audit only)
@@ -84,7 +86,8 @@ void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsi
 // Per-block sums (MC_SUM_BLOCKS x MC_SUMS doubles into `part`) over TYPE_F
 // cells of 1, T, p and X_a.
 constexpr int MC_SUMS = 3 + chem::NS, MC_SUM_BLOCKS = 512;
-void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const double* X, int64_t n,
+template <typename S>
+void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const S* X, int64_t n,
                           double* part, hipStream_t s);
 
 }  // namespace lbm

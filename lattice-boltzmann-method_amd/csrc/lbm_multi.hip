@@ -1653,8 +1653,9 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
 // ------------------------------------------- rest state of FP32 storage
 // Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
 // the mean state the FP32 deviations are taken from.
+template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, const double* T, const double* p,
-                                                         const double* X, int64_t n, double* part)
+                                                         const S* X, int64_t n, double* part)
 {
     __shared__ double red[BLOCK];
     double acc[MC_SUMS] = {};
@@ -1678,11 +1679,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, co
     }
 }
 
-void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const double* X, int64_t n,
+template <typename S>
+void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const S* X, int64_t n,
                           double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_state_sums, dim3(MC_SUM_BLOCKS), dim3(BLOCK), 0, s, type, T, p, X, n, part);
+    hipLaunchKernelGGL(k_mc_state_sums<S>, dim3(MC_SUM_BLOCKS), dim3(BLOCK), 0, s, type, T, p, X, n, part);
 }
+template void launch_mc_state_sums<float>(const uint8_t*, const double*, const double*, const float*, int64_t,
+                                          double*, hipStream_t);
+template void launch_mc_state_sums<double>(const uint8_t*, const double*, const double*, const double*, int64_t,
+                                           double*, hipStream_t);
 
 // ------------------------------------------------------------- launchers
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
