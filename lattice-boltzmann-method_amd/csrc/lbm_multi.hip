@@ -781,18 +781,20 @@ __device__ inline void cholesky_solve3(double A[NS][NS], double B[NS][3])
 constexpr int NTRI = NS * (NS + 1) / 2;
 __host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 
-__device__ inline void cholesky_solve3(double L[NTRI], double B[NS][3])
+// (T = double: the FP64 path; T = float: the FP32 path's binary32 solve.)
+template <typename T>
+__device__ inline void cholesky_solve3(T L[NTRI], T B[NS][3])
 {
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-        double s = L[tri(j, j)];
+        T s = L[tri(j, j)];
 #pragma unroll
         for (int k = 0; k < j; ++k) s -= L[tri(j, k)] * L[tri(j, k)];
-        const double d = sqrt(s);
+        const T d = sqrt(s);
         L[tri(j, j)] = d;
 #pragma unroll
         for (int i = j + 1; i < NS; ++i) {
-            double t = L[tri(i, j)];
+            T t = L[tri(i, j)];
 #pragma unroll
             for (int k = 0; k < j; ++k) t -= L[tri(i, k)] * L[tri(j, k)];
             L[tri(i, j)] = t / d;
@@ -802,14 +804,14 @@ __device__ inline void cholesky_solve3(double L[NTRI], double B[NS][3])
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-            double t = B[i][r];
+            T t = B[i][r];
 #pragma unroll
             for (int k = 0; k < i; ++k) t -= L[tri(i, k)] * B[k][r];
             B[i][r] = t / L[tri(i, i)];
         }
 #pragma unroll
         for (int i = NS - 1; i >= 0; --i) {
-            double t = B[i][r];
+            T t = B[i][r];
 #pragma unroll
             for (int k = i + 1; k < NS; ++k) t -= L[tri(k, i)] * B[k][r];
             B[i][r] = t / L[tri(i, i)];
@@ -819,11 +821,12 @@ __device__ inline void cholesky_solve3(double L[NTRI], double B[NS][3])
 
 // The FP64 kernel holds more live doubles than fit two waves per SIMD: it
 // runs at one wave per SIMD with the accumulation registers and no scratch
-// spills (2.9x faster than the spilling default); the FP32 one keeps the
-// compiler's default (two waves per SIMD, some spills), which measured faster
-// than forcing either no spills or one wave.
+// spills (2.9x faster than the spilling default).  The FP32 one runs at two
+// waves per SIMD without scratch: its per-species sums go through LDS, the
+// Maxwell-Stefan solve is binary32 and the species planes are written before
+// the temperature iteration (2.10 -> 1.49 ms on the 256x128x128 layer).
 template <typename P>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof(P) == 8 ? 1 : 8))) void k_mc_moments(MultiView<P> v)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 8 ? 1 : 2, sizeof(P) == 8 ? 1 : 8))) void k_mc_moments(MultiView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
@@ -853,7 +856,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof
         if (cy(l)) qy += cy(l) * g;
         if (cz(l)) qz += cz(l) * g;
     }
-    // species moments
+    // species moments.  FP32: the per-species results go through LDS, one
+    // column per thread (the rolled loop would put register arrays it indexes
+    // by species into scratch memory)
+    constexpr bool LDS_STAGE = sizeof(P) == 4;
+    __shared__ double s_rr[LDS_STAGE ? NS : 1][BLOCK];
+    __shared__ float s_B[LDS_STAGE ? 3 * NS : 1][BLOCK];
     double Pt[6] = {0, 0, 0, 0, 0, 0};
     double rr[NS], B[NS][3];
     double rho = 0, m0 = 0, m1 = 0, m2 = 0;
@@ -873,19 +881,28 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof
             if (cx(l) * cz(l)) Pt[4] += (cx(l) * cz(l)) * f;
             if (cy(l) * cz(l)) Pt[5] += (cy(l) * cz(l)) * f;
         }
-        rr[a] = r > SPECIES_MIN ? (r > SPECIES_MIN ? r : SPECIES_MIN) : SPECIES_MIN;
-        B[a][0] = mx;
-        B[a][1] = my;
-        B[a][2] = mz;
+        const double rclamp = r > SPECIES_MIN ? (r > SPECIES_MIN ? r : SPECIES_MIN) : SPECIES_MIN;
+        if constexpr (LDS_STAGE) {
+            s_rr[a][threadIdx.x] = rclamp;
+            s_B[3 * a + 0][threadIdx.x] = (float)mx;
+            s_B[3 * a + 1][threadIdx.x] = (float)my;
+            s_B[3 * a + 2][threadIdx.x] = (float)mz;
+        } else {
+            rr[a] = rclamp;
+            B[a][0] = mx;
+            B[a][1] = my;
+            B[a][2] = mz;
+        }
         rho += r;
         m0 += mx;
         m1 += my;
         m2 += mz;
     }
     const double vel[3] = {m0 / rho, m1 / rho, m2 / rho};
-    // Maxwell-Stefan velocities: A u = (rho u)_a with the old state's D, p, X
-    // (storage by type: measured, the FP64 kernel runs best with the packed
-    // triangle at one wave per SIMD, the FP32 one with the full matrix at two)
+    // Maxwell-Stefan velocities: A u = (rho u)_a with the old state's D, p, X;
+    // then the internal energy U = rho e / rho - sum 1/2 rho_a |u_a|^2 / rho and
+    // the species velocities are written (not live across the T iteration)
+    double Uint = re / rho;
     if constexpr (sizeof(P) == 8) {
         double A[NTRI];  // lower triangle; A[a][b] = A[b][a] = A[tri(max, min)]
 #pragma unroll
@@ -907,41 +924,82 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof
                 if (b != a) A[tri(a, a)] -= A[b > a ? tri(b, a) : tri(a, b)];
         cholesky_solve3(A, B);
     } else {
-        double A[NS][NS];
+        // binary32 diffusivities (as the transport pass of the FP32 path) and a
+        // binary32 solve: the velocities are stored in binary32
+#pragma unroll
+        for (int a = 0; a < NS; ++a) rr[a] = s_rr[a][threadIdx.x];
+        const float lnTf = (float)lnT_old, pf = (float)p_old_si;
+        float A[NTRI], Bf[NS][3];
 #pragma unroll
         for (int a = 0; a < NS; ++a)
 #pragma unroll
             for (int b = 0; b < NS; ++b) {
                 if (b < a) continue;
                 if (b == a) {
-                    A[a][a] = rr[a] == 0.0 ? 1.0 : rr[a];
+                    A[tri(a, a)] = rr[a] == 0.0 ? 1.0f : (float)rr[a];
                 } else {
-                    const double D = v.u.nu(binary_diff(a, b, lnT_old, p_old_si));
-                    A[a][b] = -1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D;
-                    A[b][a] = A[a][b];
+                    const double D = v.u.nu(binary_diff_r<float>(a, b, lnTf, pf));
+                    A[tri(b, a)] = (float)(-1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D);
                 }
             }
 #pragma unroll
-        for (int a = 0; a < NS; ++a)
+        for (int a = 0; a < NS; ++a) {
 #pragma unroll
             for (int b = 0; b < NS; ++b)
-                if (b != a) A[a][a] -= A[a][b];
-        cholesky_solve3(A, B);
+                if (b != a) A[tri(a, a)] -= A[b > a ? tri(b, a) : tri(a, b)];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) Bf[a][d] = s_B[3 * a + d][threadIdx.x];
+        }
+        cholesky_solve3(A, Bf);
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            const double b0 = Bf[a][0], b1 = Bf[a][1], b2 = Bf[a][2];
+            Uint = Uint - 0.5 * rr[a] / rho * (b0 * b0 + b1 * b1 + b2 * b2);
+        }
+#pragma unroll
+        for (int a = 0; a < NS; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = Bf[a][d];
     }
     // temperature from energy at the new composition
-    double Y[NS];
+    double Y[NS], X[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) Y[a] = rr[a] / rho;
     normalize(Y);
-    double Uint = re / rho;
-    for (int a = 0; a < NS; ++a)
-        Uint = Uint - 0.5 * rr[a] / rho * (B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2]);
-    const double usi = v.u.si_energy_mass(Uint);
-    const double rho_si = v.u.si_rho(rho);
-    const double Tsi = T_from_UV(usi, Y, Tsi_old);
-    double X[NS];
+    if constexpr (sizeof(P) == 8) {  // (write order measured fastest at one wave per SIMD)
+        for (int a = 0; a < NS; ++a)
+            Uint = Uint - 0.5 * rr[a] / rho * (B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2]);
+        const double usi = v.u.si_energy_mass(Uint);
+        const double rho_si = v.u.si_rho(rho);
+        const double Tsi = T_from_UV(usi, Y, Tsi_old);
+        X_from_Y(Y, X);
+        Fd[LBM_FIELD_RHO][c] = rho;
+        Fd[LBM_FIELD_U][c] = vel[0];
+        Fd[LBM_FIELD_V][c] = vel[1];
+        Fd[LBM_FIELD_W][c] = vel[2];
+        Fd[LBM_FIELD_RHOE][c] = re;
+        Fd[LBM_FIELD_QX][c] = qx;
+        Fd[LBM_FIELD_QY][c] = qy;
+        Fd[LBM_FIELD_QZ][c] = qz;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) Fd[LBM_FIELD_PXX + a][c] = Pt[a];
+        Fd[LBM_FIELD_T][c] = v.u.temp(Tsi);
+        Fd[LBM_FIELD_P][c] = v.u.p(pressure(Tsi, rho_si, Y));
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            v.rhoa[a * n + c] = rr[a];
+            v.X[a * n + c] = X[a];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = B[a][d];
+        }
+        return;
+    }
     X_from_Y(Y, X);
-    // write back
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        v.rhoa[a * n + c] = rr[a];
+        v.X[a * n + c] = X[a];
+    }
     Fd[LBM_FIELD_RHO][c] = rho;
     Fd[LBM_FIELD_U][c] = vel[0];
     Fd[LBM_FIELD_V][c] = vel[1];
@@ -952,15 +1010,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, sizeof
     Fd[LBM_FIELD_QZ][c] = qz;
 #pragma unroll
     for (int a = 0; a < 6; ++a) Fd[LBM_FIELD_PXX + a][c] = Pt[a];
+    const double usi = v.u.si_energy_mass(Uint);
+    const double rho_si = v.u.si_rho(rho);
+    const double Tsi = T_from_UV(usi, Y, Tsi_old);
     Fd[LBM_FIELD_T][c] = v.u.temp(Tsi);
     Fd[LBM_FIELD_P][c] = v.u.p(pressure(Tsi, rho_si, Y));
-#pragma unroll
-    for (int a = 0; a < NS; ++a) {
-        v.rhoa[a * n + c] = rr[a];
-        v.X[a * n + c] = X[a];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) v.ua[(d * NS + a) * n + c] = B[a][d];
-    }
 }
 
 // calculate_moment_point (lbm-moment.cpp:379-537) of cell c, writing its
