@@ -31,6 +31,18 @@ constexpr InvW make_inv_w()
 constexpr InvW INV_W_TABLE = make_inv_w();
 __host__ __device__ inline double inv_w(int a) { return INV_W_TABLE.v[a]; }
 
+// Reciprocal in arithmetic type R: exact for double; for float (the FP32
+// path's binary32 transport) the hardware reciprocal, within 1 ulp.
+__host__ __device__ inline double rcp_r(double x) { return 1.0 / x; }
+__host__ __device__ inline float rcp_r(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // ---------------------------------------------------------------- units
 // Units_Conv after set_m_kg_s(dx, dt): m, kg, s, K stored as float; every
 // conversion rounds its argument to float and computes in float, or in
@@ -125,24 +137,35 @@ __host__ __device__ inline void normalize(double Y[NS])
     for (int k = 0; k < NS; ++k) Y[k] *= inv;
 }
 
+// FAST = true (the FP32 path): products with 1/W and one reciprocal in place
+// of a division per species (each result within a few ulp of the exact
+// form, which the plain path keeps).
+template <bool FAST = false>
 __host__ __device__ inline double mean_W(const double Y[NS])
 {
     double s = 0;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) s += Y[k] / W[k];
+    for (int k = 0; k < NS; ++k) s += FAST ? Y[k] * inv_w(k) : Y[k] / W[k];
     return 1.0 / s;
 }
 
+template <bool FAST = false>
 __host__ __device__ inline void X_from_Y(const double Y[NS], double X[NS])
 {
     double s = 0;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        X[k] = Y[k] / W[k];
+        X[k] = FAST ? Y[k] * inv_w(k) : Y[k] / W[k];
         s += X[k];
     }
+    if constexpr (FAST) {
+        const double is = 1.0 / s;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) X[k] /= s;
+        for (int k = 0; k < NS; ++k) X[k] *= is;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) X[k] /= s;
+    }
 }
 
 __host__ __device__ inline void Y_from_X(const double X[NS], double Y[NS])
@@ -157,12 +180,21 @@ __host__ __device__ inline void Y_from_X(const double X[NS], double Y[NS])
     for (int k = 0; k < NS; ++k) Y[k] /= s;
 }
 
+template <bool FAST = false>
 __host__ __device__ inline double cp_mass(double T, const double Y[NS])
 {
     double s = 0;
 #pragma unroll
-    for (int k = 0; k < NS; ++k) s += Y[k] * cp_R(k, T) / W[k];
+    for (int k = 0; k < NS; ++k) s += FAST ? Y[k] * cp_R(k, T) * inv_w(k) : Y[k] * cp_R(k, T) / W[k];
     return s * R_UNIV;
+}
+
+// h / RT with the reciprocal temperature given (one division per state
+// instead of one per species)
+__host__ __device__ inline double h_RT_inv(int k, double T, double invT)
+{
+    const double* a = nasa(k, T);
+    return a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] * invT;
 }
 
 __host__ __device__ inline double u_mass(double T, const double Y[NS])
@@ -175,9 +207,30 @@ __host__ __device__ inline double u_mass(double T, const double Y[NS])
 
 // setState_UV at fixed composition (tolerance 1e-15 as the reference asks):
 // Newton on u(T) = u_target from the current temperature T0.
+// (FAST: Y/W hoisted as products with 1/W, one reciprocal of T per
+// iteration; the same Newton and stopping rule.)
+template <bool FAST = false>
 __host__ __device__ inline double T_from_UV(double u_target, const double Y[NS], double T0)
 {
     double T = T0;
+    if constexpr (FAST) {
+        double yw[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) yw[k] = Y[k] * inv_w(k);
+        for (int it = 0; it < 60; ++it) {
+            const double invT = 1.0 / T;
+            double u = 0, cv = 0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                u += yw[k] * (h_RT_inv(k, T, invT) - 1.0);
+                cv += yw[k] * (cp_R(k, T) - 1.0);
+            }
+            const double dT = -(u * R_UNIV * T - u_target) / (cv * R_UNIV);
+            T += dT;
+            if (fabs(dT) <= 1e-15 * fabs(T)) break;
+        }
+        return T;
+    }
     for (int it = 0; it < 60; ++it) {
         double u = 0, cv = 0;
 #pragma unroll
@@ -193,8 +246,15 @@ __host__ __device__ inline double T_from_UV(double u_target, const double Y[NS],
     return T;
 }
 
+template <bool FAST = false>
 __host__ __device__ inline double pressure(double T, double rho, const double Y[NS])
 {
+    if constexpr (FAST) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s += Y[k] * inv_w(k);
+        return rho * R_UNIV * T * s;
+    }
     return rho * R_UNIV * T / mean_W(Y);
 }
 
@@ -292,7 +352,16 @@ __host__ __device__ inline R species_viscosity_r(int k, R lnT)
 template <typename R>
 __host__ __device__ inline R binary_diff_r(int a, int b, R lnT, R p)
 {
+    if constexpr (sizeof(R) == 4) return exp(poly4r<R>(DIFF_FIT[a][b], lnT)) * rcp_r(p);
     return exp(poly4r<R>(DIFF_FIT[a][b], lnT)) / p;
+}
+
+// Wilke weight Phi_ab with sqrt(mu_a / mu_b) given: (1 + sqrt(mu_a/mu_b)
+// (W_b/W_a)^(1/4))^2 / sqrt(8 (1 + W_a/W_b)), the W factors folded at compile time.
+__host__ __device__ inline float wilke_phi(int a, int b, float sq_ratio)
+{
+    const float r = 1.0f + sq_ratio * (float)sqrt(sqrt(W[b] / W[a]));
+    return r * r * (float)(1.0 / sqrt(8.0 * (1.0 + W[a] / W[b])));
 }
 
 // Wilke mixture viscosity with the direct Phi_ab formula.
@@ -318,6 +387,26 @@ __host__ __device__ inline double mix_viscosity(double lnT, const double X[NS])
 template <typename R>
 __host__ __device__ inline R mix_viscosity_r(R lnT, const double X[NS])
 {
+    if constexpr (sizeof(R) == 4) {
+        // sqrt(mu_a / mu_b) = sqrt(mu_a) / sqrt(mu_b): 9 square roots and
+        // reciprocals in place of 81 divisions and square roots
+        float mu[NS], sm[NS], rsm[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            mu[k] = species_viscosity_r<float>(k, lnT);
+            sm[k] = sqrt(mu[k]);
+            rsm[k] = rcp_r(sm[k]);
+        }
+        float tot = 0;
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+            float den = 0;
+#pragma unroll
+            for (int b = 0; b < NS; ++b) den += (float)X[b] * wilke_phi(a, b, sm[a] * rsm[b]);
+            tot += (float)X[a] * mu[a] * rcp_r(den);
+        }
+        return tot;
+    }
     R mu[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) mu[k] = species_viscosity_r<R>(k, lnT);
@@ -343,8 +432,12 @@ __host__ __device__ inline R mix_conductivity_r(R lnT, const double X[NS])
     for (int k = 0; k < NS; ++k) {
         const R l = exp(poly4r<R>(COND_FIT[k], lnT));
         s1 += (R)X[k] * l;
-        s2 += (R)X[k] / l;
+        if constexpr (sizeof(R) == 4)
+            s2 += (R)X[k] * rcp_r(l);
+        else
+            s2 += (R)X[k] / l;
     }
+    if constexpr (sizeof(R) == 4) return R(0.5) * (s1 + rcp_r(s2));
     return R(0.5) * (s1 + R(1) / s2);
 }
 

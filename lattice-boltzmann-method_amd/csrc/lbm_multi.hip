@@ -441,11 +441,94 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const un
 // omega_a and the impulse per species in v.tsp.  The collide pass then runs
 // one thread per (cell, species) (:452-520) with a small register footprint;
 // its per-species inputs are the species' own planes.
+// The transport pass of the FP32 path, all in binary32: species
+// viscosities and Wilke weights with sqrt(mu_a / mu_b) = sqrt(mu_a) / sqrt(mu_b)
+// (9 square roots and reciprocals in place of 45 divisions and square roots),
+// and the Maxwell-Stefan friction with one reciprocal per diffusivity.  The
+// same quantities as the plain kernel below, to binary32 rounding.
+template <typename P>
+__device__ __forceinline__ void species_transport_fp32(const MultiView<P>& v, int c)
+{
+    const int64_t n = v.d.n;
+    double* const* Fd = v.fld;
+    const double rho = Fd[LBM_FIELD_RHO][c], T = Fd[LBM_FIELD_T][c];
+    const float p = (float)Fd[LBM_FIELD_P][c];
+    const double irho = 1.0 / rho;
+    float ra[NS], Xa[NS];
+    double Y0[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        ra[a] = v.rhoa[a * n + c];
+        Xa[a] = v.X[a * n + c];
+        Y0[a] = ra[a] * irho;
+    }
+    normalize(Y0);
+    const double Tsi = v.u.si_temp(T);
+    const double rho_si = v.u.si_rho(rho);
+    const float lnT = (float)log(Tsi);
+    // lattice units of viscosity and diffusivity: s m / kg and s / m^2
+    const float kmu = v.u.s * v.u.m / v.u.kg;
+    const float knu = (float)((double)v.u.s / ((double)v.u.m * (double)v.u.m));
+    float mu[NS], sm[NS], rsm[NS], den[NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        mu[a] = species_viscosity_r<float>(a, lnT);
+        sm[a] = sqrtf(mu[a]);
+        rsm[a] = rcp_r(sm[a]);
+        den[a] = 0.0f;
+    }
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int b = a; b < NS; ++b) {
+            const float pab = wilke_phi(a, b, sm[a] * rsm[b]);
+            den[a] += Xa[b] * pab;
+            if (b != a) den[b] += Xa[a] * (mu[b] * rcp_r(mu[a]) * (float)(W[a] / W[b]) * pab);
+        }
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        const float visc = Xa[a] * (mu[a] * kmu) * rcp_r(den[a]);
+        v.tsp[a * n + c] = P(2.0f * Xa[a] * p * rcp_r(Xa[a] * p + 2.0f * visc));
+    }
+    const float rp_si = rcp_r((float)pressure<true>(Tsi, rho_si, Y0));
+    float ua[3][NS], acc[NS][3];
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ua[d][a] = v.ua[(d * NS + a) * n + c];
+            acc[a][d] = 0.0f;
+        }
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+        for (int b = a + 1; b < NS; ++b) {
+            const float D = exp(poly4r<float>(DIFF_FIT[a][b], lnT)) * rp_si * knu;
+            const float k = -p * Xa[a] * Xa[b] * rcp_r(D);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float du = ua[d][a] - ua[d][b];
+                acc[a][d] += k * du;
+                acc[b][d] -= k * du;
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+        const float ir = rcp_r(ra[a]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v.tsp[((1 + d) * NS + a) * n + c] = P(acc[a][d] * ir);
+    }
+}
+
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_mc_species_transport(MultiView<P> v)
 {
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    if constexpr (sizeof(P) == 4) {
+        species_transport_fp32(v, c);
+        return;
+    }
     const int64_t n = v.d.n;
     double* const* Fd = v.fld;
     const double rho = Fd[LBM_FIELD_RHO][c], T = Fd[LBM_FIELD_T][c], p = Fd[LBM_FIELD_P][c];
@@ -591,11 +674,14 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
         double* const* Fd = v.fld;
         const double rho = Fd[LBM_FIELD_RHO][c], T = Fd[LBM_FIELD_T][c], p = Fd[LBM_FIELD_P][c];
         const double vel[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
+        // FP32: products with reciprocals in place of divisions
+        constexpr bool FAST = sizeof(P) == 4;
+        const double irho = 1.0 / rho;
         double ra[NS], Y0[NS], X0[NS];
 #pragma unroll
         for (int a = 0; a < NS; ++a) {
             ra[a] = v.rhoa[a * n + c];
-            Y0[a] = ra[a] / rho;
+            Y0[a] = FAST ? ra[a] * irho : ra[a] / rho;
         }
         normalize(Y0);
         X_from_Y(Y0, X0);
@@ -616,7 +702,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
         for (int a = 0; a < NS; ++a) {
             const double ux = v.ua[(0 * NS + a) * n + c], uy = v.ua[(1 * NS + a) * n + c],
                          uz = v.ua[(2 * NS + a) * n + c];
-            Uint = Uint - 0.5 * ra[a] / rho * (ux * ux + uy * uy + uz * uz);
+            Uint = Uint - (FAST ? 0.5 * ra[a] * irho : 0.5 * ra[a] / rho) * (ux * ux + uy * uy + uz * uz);
         }
         const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
         const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
@@ -629,9 +715,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
             irn[d] = 1.0 / Fd[LBM_FIELD_RHO][nm[d]];
             irp[d] = 1.0 / Fd[LBM_FIELD_RHO][np[d]];
         }
-        const double irho = 1.0 / rho;
+        const double invT = 1.0 / Tsi;
         for (int a = 0; a < NS; ++a) {
-            const double ha = v.u.energy_mass(h_RT(a, Tsi) * R_UNIV * Tsi * inv_w(a));
+            const double ha = v.u.energy_mass((FAST ? h_RT_inv(a, Tsi, invT) : h_RT(a, Tsi)) * R_UNIV * Tsi * inv_w(a));
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double uad = v.ua[(d * NS + a) * n + c];
@@ -819,6 +905,47 @@ __device__ inline void cholesky_solve3(T L[NTRI], T B[NS][3])
     }
 }
 
+// The binary32 solve of the FP32 path: the same factorisation with the
+// reciprocal of each pivot taken once (hardware reciprocal, 1 ulp) and
+// multiplied in place of the divisions.
+__device__ inline void cholesky_solve3_rcp(float L[NTRI], float B[NS][3])
+{
+    float rd[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        float s = L[tri(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s -= L[tri(j, k)] * L[tri(j, k)];
+        const float d = sqrtf(s);
+        L[tri(j, j)] = d;
+        rd[j] = rcp_r(d);
+#pragma unroll
+        for (int i = j + 1; i < NS; ++i) {
+            float t = L[tri(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t -= L[tri(i, k)] * L[tri(j, k)];
+            L[tri(i, j)] = t * rd[j];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            float t = B[i][r];
+#pragma unroll
+            for (int k = 0; k < i; ++k) t -= L[tri(i, k)] * B[k][r];
+            B[i][r] = t * rd[i];
+        }
+#pragma unroll
+        for (int i = NS - 1; i >= 0; --i) {
+            float t = B[i][r];
+#pragma unroll
+            for (int k = i + 1; k < NS; ++k) t -= L[tri(k, i)] * B[k][r];
+            B[i][r] = t * rd[i];
+        }
+    }
+}
+
 // The FP64 kernel holds more live doubles than fit two waves per SIMD: it
 // runs at one wave per SIMD with the accumulation registers and no scratch
 // spills (2.9x faster than the spilling default).  The FP32 one runs at two
@@ -837,14 +964,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
     const double Tsi_old = v.u.si_temp(Fd[LBM_FIELD_T][c]);
     const double rho_old_si = v.u.si_rho(rho_old);
     const double p_old = Fd[LBM_FIELD_P][c];
+    constexpr bool FAST = sizeof(P) == 4;  // FP32: reciprocals in place of divisions
+    const double irho_old = 1.0 / rho_old;
     double Yold[NS], Xold[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
-        Yold[a] = v.rhoa[a * n + c] / rho_old;
+        Yold[a] = FAST ? v.rhoa[a * n + c] * irho_old : v.rhoa[a * n + c] / rho_old;
         Xold[a] = v.X[a * n + c];
     }
     normalize(Yold);
-    const double p_old_si = pressure(Tsi_old, rho_old_si, Yold);
+    const double p_old_si = pressure<FAST>(Tsi_old, rho_old_si, Yold);
     const double lnT_old = log(Tsi_old);
     // energy moments
     double re = 0, qx = 0, qy = 0, qz = 0;
@@ -928,7 +1057,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
         // binary32 solve: the velocities are stored in binary32
 #pragma unroll
         for (int a = 0; a < NS; ++a) rr[a] = s_rr[a][threadIdx.x];
-        const float lnTf = (float)lnT_old, pf = (float)p_old_si;
+        const float lnTf = (float)lnT_old, rpf = rcp_r((float)p_old_si);
+        const float knu = (float)((double)v.u.s / ((double)v.u.m * (double)v.u.m));
+        const float hp = -0.5f * (float)p_old;
         float A[NTRI], Bf[NS][3];
 #pragma unroll
         for (int a = 0; a < NS; ++a)
@@ -938,8 +1069,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
                 if (b == a) {
                     A[tri(a, a)] = rr[a] == 0.0 ? 1.0f : (float)rr[a];
                 } else {
-                    const double D = v.u.nu(binary_diff_r<float>(a, b, lnTf, pf));
-                    A[tri(b, a)] = (float)(-1.0 * 1 / 2.0 * p_old * Xold[a] * Xold[b] / D);
+                    const float D = exp(poly4r<float>(DIFF_FIT[a][b], lnTf)) * rpf * knu;
+                    A[tri(b, a)] = hp * (float)Xold[a] * (float)Xold[b] * rcp_r(D);
                 }
             }
 #pragma unroll
@@ -950,11 +1081,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
 #pragma unroll
             for (int d = 0; d < 3; ++d) Bf[a][d] = s_B[3 * a + d][threadIdx.x];
         }
-        cholesky_solve3(A, Bf);
+        cholesky_solve3_rcp(A, Bf);
+        const double irho = 1.0 / rho;
 #pragma unroll
         for (int a = 0; a < NS; ++a) {
             const double b0 = Bf[a][0], b1 = Bf[a][1], b2 = Bf[a][2];
-            Uint = Uint - 0.5 * rr[a] / rho * (b0 * b0 + b1 * b1 + b2 * b2);
+            Uint = Uint - 0.5 * rr[a] * irho * (b0 * b0 + b1 * b1 + b2 * b2);
         }
 #pragma unroll
         for (int a = 0; a < NS; ++a)
@@ -963,8 +1095,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
     }
     // temperature from energy at the new composition
     double Y[NS], X[NS];
+    const double irho_new = 1.0 / rho;
 #pragma unroll
-    for (int a = 0; a < NS; ++a) Y[a] = rr[a] / rho;
+    for (int a = 0; a < NS; ++a) Y[a] = FAST ? rr[a] * irho_new : rr[a] / rho;
     normalize(Y);
     if constexpr (sizeof(P) == 8) {  // (write order measured fastest at one wave per SIMD)
         for (int a = 0; a < NS; ++a)
@@ -994,7 +1127,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
         }
         return;
     }
-    X_from_Y(Y, X);
+    X_from_Y<true>(Y, X);
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
         v.rhoa[a * n + c] = rr[a];
@@ -1012,9 +1145,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
     for (int a = 0; a < 6; ++a) Fd[LBM_FIELD_PXX + a][c] = Pt[a];
     const double usi = v.u.si_energy_mass(Uint);
     const double rho_si = v.u.si_rho(rho);
-    const double Tsi = T_from_UV(usi, Y, Tsi_old);
+    const double Tsi = T_from_UV<true>(usi, Y, Tsi_old);
     Fd[LBM_FIELD_T][c] = v.u.temp(Tsi);
-    Fd[LBM_FIELD_P][c] = v.u.p(pressure(Tsi, rho_si, Y));
+    Fd[LBM_FIELD_P][c] = v.u.p(pressure<true>(Tsi, rho_si, Y));
 }
 
 // calculate_moment_point (lbm-moment.cpp:379-537) of cell c, writing its
