@@ -117,6 +117,15 @@ const char* lbm_mechanism(int* n_species);
  * (gas->speciesName(k), setup.cpp); null when k is out of range. */
 const char* lbm_species_name(int k);
 
+/* The conjugate heat-transfer variant (#define CONJUGATE, defines.hpp:62-63;
+ * lbm-collision.cpp:101-180, lbm-streaming.cpp:120-165, lbm-moment.cpp:85-120,
+ * lbm-bc.cpp:1230-1346): interior TYPE_S cells conduct heat with ten times the
+ * gas conductivity, stream g among themselves and couple to the gas through
+ * TMS_BC.  Single-component LBM_MODE_STRICT contexts only (LBM_E_UNSUPPORTED
+ * otherwise); 0 turns it off.  TYPE_S cells on the ghost layer stay isothermal
+ * walls. */
+int lbm_set_conjugate(lbm_ctx* ctx, int on);
+
 /* Cell types, uint8 per ghost-inclusive cell (MIXTURE::type, defines.hpp:136-145). */
 int lbm_set_types(lbm_ctx* ctx, const uint8_t* types);
 /* Set-up / output access to per-cell fields (lb.mixture[i][j][k].<field>). */

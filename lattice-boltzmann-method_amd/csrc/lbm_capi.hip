@@ -80,6 +80,11 @@ struct lbm_ctx {
     // single-component TMS_BC interface cells (fluid cells next to TYPE_O/I/S)
     int* sbc_cell = nullptr;
     int sbc_count = 0;
+    // conjugate variant: interior TYPE_S cells conduct heat; the solid
+    // interface cells of its TMS_BC part
+    bool conjugate = false;
+    int* cjc_cell = nullptr;
+    int cjc_count = 0;
     void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
@@ -180,6 +185,7 @@ SingleView<P> make_view(lbm_ctx* c, int src, int dst)
     }
     for (int i = 0; i < 5; ++i) v.fref_sum[i] = c->fref_sum[i];
     v.tau_count = c->tau_count;
+    v.conjugate = c->conjugate ? 1 : 0;
     return v;
 }
 
@@ -361,7 +367,7 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell};
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell, c->cjc_cell};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -622,6 +628,25 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                         }
                     }
                 }
+    // solid cells of the conjugate variant's TMS_BC part (lbm-bc.cpp:1246-1259):
+    // interior TYPE_S cells with some direction whose upstream cell is gas
+    // (TYPE_F or TYPE_Q) and whose downstream cell is solid
+    std::vector<int> jcell;
+    if (!multi(c))
+        for (int i = 1; i < d.nx - 1; ++i)
+            for (int j = 1; j < d.ny - 1; ++j)
+                for (int k = 1; k < d.nz - 1; ++k) {
+                    const int64_t cidx = ((int64_t)i * d.ny + j) * d.nz + k;
+                    if (types[cidx] != TYPE_S) continue;
+                    for (int l = 0; l < Q; ++l) {
+                        const uint8_t tn = types[((int64_t)(i - cx(l)) * d.ny + (j - cy(l))) * d.nz + (k - cz(l))];
+                        const uint8_t tp = types[((int64_t)(i + cx(l)) * d.ny + (j + cy(l))) * d.nz + (k + cz(l))];
+                        if ((tn == TYPE_F || tn == TYPE_Q) && tp == TYPE_S) {
+                            jcell.push_back((int)cidx);
+                            break;
+                        }
+                    }
+                }
     if (fused(c) && (needs_tms || !fused_ok))
         return fail(c, LBM_E_UNSUPPORTED,
                     "fused mode handles periodic, adiabatic (TYPE_A) and free-slip (TYPE_FS) neighbours; "
@@ -668,6 +693,13 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
         if (c->sbc_count) {
             HIP_TRY(c, hipMalloc((void**)&c->sbc_cell, sizeof(int) * scell.size()));
             HIP_TRY(c, hipMemcpy(c->sbc_cell, scell.data(), sizeof(int) * scell.size(), hipMemcpyHostToDevice));
+        }
+        if (c->cjc_cell) (void)hipFree(c->cjc_cell);
+        c->cjc_cell = nullptr;
+        c->cjc_count = (int)jcell.size();
+        if (c->cjc_count) {
+            HIP_TRY(c, hipMalloc((void**)&c->cjc_cell, sizeof(int) * jcell.size()));
+            HIP_TRY(c, hipMemcpy(c->cjc_cell, jcell.data(), sizeof(int) * jcell.size(), hipMemcpyHostToDevice));
         }
     }
     c->needs_tms = needs_tms;
@@ -991,12 +1023,29 @@ int lbm_tms_bc(lbm_ctx* c)
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
     // fused mode rejects TYPE_S/I/O neighbours at lbm_set_types; periodic,
     // adiabatic and free-slip walls need no reconstruction
-    if (fused(c) || c->sbc_count == 0) return LBM_OK;
+    if (fused(c)) return LBM_OK;
     // outflow, inflow and isothermal-wall sweeps (lbm-bc.cpp:341, 809, 1111) on
     // the streamed populations of buffer 0
-    return DISPATCH(c, launch(c, "tms", [&] {
-        launch_tms<P>(make_view<P>(c, 0, 0), c->sbc_cell, c->sbc_count, c->stream);
-    }));
+    if (c->sbc_count)
+        if (int rc = DISPATCH(c, launch(c, "tms", [&] {
+                launch_tms<P>(make_view<P>(c, 0, 0), c->sbc_cell, c->sbc_count, c->stream);
+            })))
+            return rc;
+    // then the conjugate variant's solid side (lbm-bc.cpp:1230-1346)
+    if (c->conjugate && c->cjc_count)
+        return DISPATCH(c, launch(c, "tms_conj", [&] {
+            launch_tms_conj<P>(make_view<P>(c, 0, 0), c->cjc_cell, c->cjc_count, c->stream);
+        }));
+    return LBM_OK;
+}
+
+int lbm_set_conjugate(lbm_ctx* c, int on)
+{
+    if (!c) return LBM_E_ARG;
+    if (on && multi(c)) return fail(c, LBM_E_UNSUPPORTED, "the conjugate variant is single-component");
+    if (on && fused(c)) return fail(c, LBM_E_UNSUPPORTED, "the conjugate variant needs LBM_MODE_STRICT");
+    c->conjugate = on != 0;
+    return LBM_OK;
 }
 
 int lbm_moments(lbm_ctx* c)

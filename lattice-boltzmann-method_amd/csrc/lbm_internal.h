@@ -58,6 +58,7 @@ struct SingleView {
     __host__ __device__ double fref_d(int l) const { return fref[l]; }
     __host__ __device__ double gref_d(int l) const { return gref[l]; }
     unsigned long long* tau_count;
+    int conjugate;  // CONJUGATE: TYPE_S cells conduct heat (strict mode)
 };
 
 // Kernel launchers (lbm_single.hip).  All enqueue on `s`.
@@ -70,6 +71,9 @@ template <typename P> void launch_fused_moments(const SingleView<P>& v, bool mat
 // TMS_BC on the interface cells `cells` (fluid cells with a TYPE_O, TYPE_I or
 // TYPE_S upstream neighbour whose downstream neighbour is fluid), in place on v.f/v.g.
 template <typename P> void launch_tms(const SingleView<P>& v, const int* cells, int count, hipStream_t s);
+// CONJUGATE part of TMS_BC on the solid interface cells `cells` (TYPE_S cells
+// with a gas cell upstream and a solid cell downstream of some direction).
+template <typename P> void launch_tms_conj(const SingleView<P>& v, const int* cells, int count, hipStream_t s);
 // gamma: ratio of specific heats used for the Mach number |u| / sqrt(gamma p / rho)
 void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
                  double* out8, hipStream_t s);

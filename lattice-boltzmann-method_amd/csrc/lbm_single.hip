@@ -221,14 +221,17 @@ __device__ inline P qflux(P rho, P ud, P T, P R)
 // The single-component collision of lbm-collision.cpp:23-97 for one cell.
 // fl/gl hold the (possibly shifted) populations and are overwritten with the
 // post-collision ones in the same storage convention.
+// kfac scales the conductivity: 1 for gas, 10 for the solid cells of the
+// conjugate variant (lbm-collision.cpp:107).
 template <typename P>
-__device__ inline bool collide_cell(const SingleView<P>& v, const Prim<P>& s, const P dQ[3], P fl[Q], P gl[Q])
+__device__ inline bool collide_cell(const SingleView<P>& v, const Prim<P>& s, const P dQ[3], P fl[Q], P gl[Q],
+                                    P kfac = P(1))
 {
     const P R = P(v.R), cp = P(v.cp);
     const P rho = s.rho, p = s.p;
     const P theta = R * s.T;
     const P mu = P(v.nu) * rho;
-    const P kappa = mu * cp / P(v.prtl);
+    const P kappa = kfac == P(1) ? mu * cp / P(v.prtl) : kfac * mu * cp / P(v.prtl);
     const P om1 = P(2) * p * cp / (p * cp + P(2) * kappa);
     const P om = P(2) * p / (p + P(2) * mu);
     const bool warn = (P(1) / om <= P(0.5)) || (P(1) / om1 <= P(0.5));
@@ -355,13 +358,27 @@ __device__ inline P field_q(const SingleView<P>& v, int c, int d)
     return qflux<P>(P(v.fld[LBM_FIELD_RHO][c]), P(v.fld[LBM_FIELD_U + d][c]), P(v.fld[LBM_FIELD_T][c]), P(v.R));
 }
 
+// A conducting solid cell of the conjugate variant: TYPE_S off the ghost
+// layer (ghost-layer TYPE_S cells stay isothermal walls; the reference's solid
+// loops would index outside the lattice there).
+template <typename P>
+__device__ inline bool conj_solid(const SingleView<P>& v, int c)
+{
+    if (!v.conjugate || v.type[c] != TYPE_S) return false;
+    const Ijk x = ijk_of(v.d, c);
+    return x.i > 0 && x.j > 0 && x.k > 0 && x.i < v.d.nx - 1 && x.j < v.d.ny - 1 && x.k < v.d.nz - 1;
+}
+
 // Collide from explicit f/g and the stored moments (the reference's Collide).
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_collide_explicit(SingleView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     bool warn = false;
-    if (c < v.d.n && v.type[c] == TYPE_F) {
+    // CONJUGATE: solid cells collide too, from their stored fields, with ten
+    // times the conductivity (lbm-collision.cpp:101-180)
+    const bool solid = c < v.d.n && conj_solid(v, c);
+    if (c < v.d.n && (v.type[c] == TYPE_F || solid)) {
         const Ijk x = ijk_of(v.d, c);
         double* const* F = v.fld;
         Prim<P> s;
@@ -387,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide_explicit(SingleView<P> v)
             fl[l] = ldp(v.f, l, v.d.n, c);
             gl[l] = ldp(v.g, l, v.d.n, c);
         }
-        warn = collide_cell(v, s, dQ, fl, gl);
+        warn = collide_cell(v, s, dQ, fl, gl, solid ? P(10) : P(1));
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
             stp(v.fo, l, v.d.n, c, fl[l]);
@@ -397,12 +414,39 @@ __global__ __launch_bounds__(BLOCK) void k_collide_explicit(SingleView<P> v)
     count_warn(v, warn);
 }
 
+// CONJUGATE streaming of a solid cell (lbm-streaming.cpp:120-165): g only;
+// nothing from gas cells (TMS_BC supplies those directions), bounce-back from
+// adiabatic walls, a pull from solid and periodic cells through the ghost wrap.
+template <typename P>
+__device__ inline void stream_solid(const SingleView<P>& v, int c, const Ijk& x)
+{
+    const int nx = v.d.nx, ny = v.d.ny, nz = v.d.nz;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        int in = x.i - cx(l), jn = x.j - cy(l), kn = x.k - cz(l);
+        const uint8_t tn = v.type[cell_of(v.d, in, jn, kn)];
+        if (tn == TYPE_A) {
+            stp(v.go, l, v.d.n, c, ldp(v.g, opp(l), v.d.n, c));
+        } else if (tn == TYPE_S || tn == TYPE_P) {
+            if (in == 0) in = nx - 2; else if (in == nx - 1) in = 1;
+            if (jn == 0) jn = ny - 2; else if (jn == ny - 1) jn = 1;
+            if (kn == 0) kn = nz - 2; else if (kn == nz - 1) kn = 1;
+            stp(v.go, l, v.d.n, c, ldp(v.g, l, v.d.n, cell_of(v.d, in, jn, kn)));
+        }
+    }
+}
+
 // Streaming: v.f/v.g = post-collision source, v.fo/v.go = f/g destination.
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_stream(SingleView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    if (c >= v.d.n) return;
+    if (conj_solid(v, c)) {
+        stream_solid(v, c, ijk_of(v.d, c));
+        return;
+    }
+    if (v.type[c] != TYPE_F) return;
     const Ijk x = ijk_of(v.d, c);
     P fl[Q], gl[Q];
     uint32_t kf = 0, kg = 0;
@@ -435,11 +479,41 @@ __device__ inline void store_fields(const SingleView<P>& v, int c, const Mom<P>&
     F[LBM_FIELD_QZ][c] = M.q[2];
 }
 
+// CONJUGATE moments of a solid cell (lbm-moment.cpp:85-120): energy and its
+// flux from g; T = (rho e / rho) / c_v and p = rho R T at the stored density.
+template <typename P>
+__device__ inline void moments_solid(const SingleView<P>& v, int c)
+{
+    double e = 0, qx = 0, qy = 0, qz = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const double g = double(ldp(v.g, l, v.d.n, c)) + v.gref_d(l);
+        e += g;
+        if (cx(l)) qx += cx(l) * g;
+        if (cy(l)) qy += cy(l) * g;
+        if (cz(l)) qz += cz(l) * g;
+    }
+    double* const* F = v.fld;
+    const double rho = F[LBM_FIELD_RHO][c];
+    const double T = e / rho / v.cv;
+    F[LBM_FIELD_RHOE][c] = e;
+    F[LBM_FIELD_QX][c] = qx;
+    F[LBM_FIELD_QY][c] = qy;
+    F[LBM_FIELD_QZ][c] = qz;
+    F[LBM_FIELD_T][c] = T;
+    F[LBM_FIELD_P][c] = rho * v.R * T;
+}
+
 template <typename P>
 __global__ __launch_bounds__(BLOCK) void k_moments(SingleView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    if (c >= v.d.n) return;
+    if (conj_solid(v, c)) {
+        moments_solid(v, c);
+        return;
+    }
+    if (v.type[c] != TYPE_F) return;
     P fl[Q], gl[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -756,6 +830,50 @@ __global__ __launch_bounds__(BLOCK) void k_tms(SingleView<P> v, const int* cells
     }
 }
 
+// CONJUGATE part of TMS_BC (lbm-bc.cpp:1230-1346) for one solid interface
+// cell: the target is the state at rest with the interface temperature
+// interpolated at q = 1/2 (averaged over the links with gas upstream and solid
+// downstream) and the cell's own density; every direction with gas upstream
+// is unknown.  Only g is rewritten (f keeps its value).
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_tms_conj(SingleView<P> v, const int* cells, int count)
+{
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t];
+    const Ijk x = ijk_of(v.d, c);
+    const double* Tf = v.fld[LBM_FIELD_T];
+    uint32_t unk = 0;
+    double T = 0, rbb = 0;
+    int nd = 0;
+    const double q = 0.5;
+    double f[Q], g[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        f[l] = double(ldp(v.f, l, v.d.n, c)) + v.fref_d(l);
+        g[l] = double(ldp(v.g, l, v.d.n, c)) + v.gref_d(l);
+        rbb += f[l];
+        const int up = cell_of(v.d, x.i - cx(l), x.j - cy(l), x.k - cz(l));
+        const uint8_t tu = v.type[up];
+        if (tu != TYPE_F && tu != TYPE_Q) continue;
+        unk |= 1u << l;
+        const int dn = cell_of(v.d, x.i + cx(l), x.j + cy(l), x.k + cz(l));
+        if (v.type[dn] != TYPE_S) continue;
+        ++nd;
+        T += (q * Tf[dn] + Tf[up]) / (1.0 + q);
+    }
+    if (nd == 0) return;
+    T = 1.0 / nd * T;
+    const double zero[3] = {0, 0, 0};
+    const EqSet b = eq_set(rbb, zero, T, v.R, v.cv);
+    double fk[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fk[l] = f[l];
+    tms_reconstruct(b, unk, false, zero, T, v.R, v.cv, fk, g);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) stp(v.g, l, v.d.n, c, P(g[l] - v.gref_d(l)));
+}
+
 // ---------------------------------------------------------------- launchers
 static inline dim3 grid_of(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
@@ -794,6 +912,12 @@ void launch_fused_moments(const SingleView<P>& v, bool materialize, hipStream_t 
 }
 
 template <typename P>
+void launch_tms_conj(const SingleView<P>& v, const int* cells, int count, hipStream_t s)
+{
+    if (count > 0) hipLaunchKernelGGL(k_tms_conj<P>, grid_of(count), dim3(BLOCK), 0, s, v, cells, count);
+}
+
+template <typename P>
 void launch_tms(const SingleView<P>& v, const int* cells, int count, hipStream_t s)
 {
     if (count > 0) hipLaunchKernelGGL(k_tms<P>, grid_of(count), dim3(BLOCK), 0, s, v, cells, count);
@@ -817,7 +941,8 @@ void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double 
     template void launch_moments<P>(const SingleView<P>&, hipStream_t);                    \
     template void launch_fused_collide<P>(const SingleView<P>&, hipStream_t);              \
     template void launch_fused_moments<P>(const SingleView<P>&, bool, hipStream_t);        \
-    template void launch_tms<P>(const SingleView<P>&, const int*, int, hipStream_t);
+    template void launch_tms<P>(const SingleView<P>&, const int*, int, hipStream_t);                 \
+    template void launch_tms_conj<P>(const SingleView<P>&, const int*, int, hipStream_t);
 LBM_INSTANTIATE(double)
 LBM_INSTANTIATE(float)
 

@@ -66,6 +66,9 @@ public:
     void set_gasconst(double v) { cfg_.gas_const = v; }
     void set_gamma(double v) { cfg_.gamma = v; }
     void set_prtl(double v) { cfg_.prtl = v; }
+    // the reference's #define CONJUGATE (defines.hpp:62-63) as a run-time switch;
+    // takes effect at the next upload()
+    void set_conjugate(bool on) { conjugate_ = on; }
     int get_step() const { return step; }
 
     size_t idx(int i, int j, int k) const { return ((size_t)i * Ny + j) * Nz + k; }
@@ -76,6 +79,7 @@ public:
     void upload()
     {
         ensure_ctx();
+        check(lbm_set_conjugate(ctx_, conjugate_ ? 1 : 0), "lbm_set_conjugate");
         check(lbm_set_types(ctx_, type.data()), "lbm_set_types");
         const int prim[] = {LBM_FIELD_RHO, LBM_FIELD_U, LBM_FIELD_V, LBM_FIELD_W, LBM_FIELD_P, LBM_FIELD_T};
         for (int f : prim) check(lbm_set_field(ctx_, f, fields_[f].data()), "lbm_set_field");
@@ -157,6 +161,7 @@ private:
     }
     lbm_config cfg_;
     lbm_ctx* ctx_ = nullptr;
+    bool conjugate_ = false;
     std::vector<std::vector<double>> fields_;
     int Nx, Ny, Nz;
     int step = 0;

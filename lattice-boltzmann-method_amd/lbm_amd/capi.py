@@ -39,7 +39,7 @@ EXPORTS = [
     "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
     "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
     "lbm_surrogate_forward", "lbm_surrogate_gradient", "lbm_write_vtk", "lbm_write_restart", "lbm_read_restart",
-    "lbm_mechanism", "lbm_species_name",
+    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate",
 ]
 
 LBM_VTK_SI, LBM_VTK_MASS_FRACTIONS = 1, 2
@@ -153,6 +153,7 @@ def load_library(path=None, mechanism="h2air"):
     L.lbm_read_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(i32)]
     L.lbm_surrogate_forward.argtypes = [vp, fp, i64, fp]
     L.lbm_surrogate_gradient.argtypes = [vp, fp, fp, i64, fp, fp]
+    L.lbm_set_conjugate.argtypes = [vp, i32]
     L.lbm_mechanism.argtypes = [ctypes.POINTER(i32)]
     L.lbm_mechanism.restype = ctypes.c_char_p
     L.lbm_species_name.argtypes = [i32]
@@ -295,6 +296,11 @@ class Lattice:
                 self.set_field(k, fields[k])
         if self.n_species and "X" in fields:
             self.set_species_field("X", fields["X"])
+
+    def set_conjugate(self, on=True):
+        """The CONJUGATE variant (defines.hpp:62-63): interior TYPE_S cells
+        conduct heat and couple to the gas through TMS_BC (strict mode)."""
+        self._check(self._L.lbm_set_conjugate(self._h, 1 if on else 0), "set_conjugate")
 
     # -- the reference's step calls
     def Init(self):

@@ -112,6 +112,41 @@ def channel(nx, ny, nz, U, T0=1.0, p0=1.0 / 3.0, T_in=1.1, T_wall=0.9, p_out=Non
     return fl
 
 
+def conjugate_box(nx, ny, nz, ns, U, T0=1.0, p0=1.0 / 3.0, T_hot=1.2, T_cold=0.9, R=1.0 / 3.0):
+    """Conjugate heat-transfer box (run with the conjugate solid enabled).
+
+    Periodic ghosts on every face; solid slabs (TYPE_S) of `ns` cells at the
+    bottom (j <= ns, T_hot) and the top (j >= ny+1-ns, T_cold) of the y range,
+    at rest; gas between them at T0, p0 with u = U and transverse waves
+    v = 0.1 U sin(2 pi i/nx) sin(pi y), w = 0.05 U cos(2 pi (k-1)/nz) sin(pi y),
+    y = (j - ns - 1/2) / (ny - 2 ns).  The slabs meet through the periodic y
+    wrap, so every solid cell has only solid, periodic or gas neighbours.
+    """
+    fl = default_fields(nx, ny, nz)
+    t = fl["type"]
+    shape = t.shape
+    i, j, k = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), np.arange(shape[2]), indexing="ij")
+    for name in ("u", "v", "w"):
+        fl[name][:] = 0.0
+    fl["T"][:] = T0
+    fl["p"][:] = p0
+    interior = (i > 0) & (i < shape[0] - 1) & (j > 0) & (j < shape[1] - 1) & (k > 0) & (k < shape[2] - 1)
+    lo = interior & (j <= ns)
+    hi = interior & (j >= shape[1] - 1 - ns)
+    t[lo | hi] = TYPE_S
+    periodic_ghosts(fl)
+    fl["T"][lo] = T_hot
+    fl["T"][hi] = T_cold
+    F = t == TYPE_F
+    msin, mcos = np.vectorize(math.sin), np.vectorize(math.cos)
+    sy = msin(math.pi * (j - ns - 0.5) / (ny - 2 * ns))
+    fl["u"][F] = U
+    fl["v"][F] = (0.1 * U * msin(2.0 * math.pi * i / nx) * sy)[F]
+    fl["w"][F] = (0.05 * U * mcos(2.0 * math.pi * (k - 1) / nz) * sy)[F]
+    fl["rho"][:] = fl["p"] / (R * fl["T"])
+    return fl
+
+
 def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.0 / 3.0, T0=1.0, p0=1.0 / 3.0,
                         gamma=1.4, Pr=0.71, nu=None, **_unused):
     """Thermal compressible decaying turbulence on an N^3 periodic box.

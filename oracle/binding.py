@@ -72,7 +72,7 @@ class _OrcSingle(ctypes.Structure):
                 ("w", ctypes.c_void_p), ("p", ctypes.c_void_p), ("T", ctypes.c_void_p),
                 ("rhoe", ctypes.c_void_p), ("ptens", ctypes.c_void_p), ("eflux", ctypes.c_void_p),
                 ("f", ctypes.c_void_p), ("fpc", ctypes.c_void_p), ("g", ctypes.c_void_p),
-                ("gpc", ctypes.c_void_p), ("tau_warnings", ctypes.c_int64)]
+                ("gpc", ctypes.c_void_p), ("tau_warnings", ctypes.c_int64), ("conjugate", ctypes.c_int)]
 
 
 class SingleState:
@@ -84,7 +84,7 @@ class SingleState:
 
     SCALARS = ("rho", "u", "v", "w", "p", "T", "rhoe")
 
-    def __init__(self, nx, ny, nz, R, gamma, prtl, nu):
+    def __init__(self, nx, ny, nz, R, gamma, prtl, nu, conjugate=False):
         self.shape = (nx, ny, nz)
         n = nx * ny * nz
         self.type = np.zeros(self.shape, np.uint8)
@@ -102,6 +102,7 @@ class SingleState:
         self.g = np.zeros((27,) + self.shape)
         self.gpc = np.zeros((27,) + self.shape)
         self._s = _OrcSingle(nx, ny, nz, R, gamma, prtl, nu)
+        self._s.conjugate = 1 if conjugate else 0
         for name in ("type", "rho", "u", "v", "w", "p", "T", "rhoe", "ptens", "eflux", "f", "fpc", "g", "gpc"):
             setattr(self._s, name, getattr(self, name).ctypes.data)
         self._n = n

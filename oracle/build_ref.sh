@@ -16,6 +16,8 @@
 #   oracle/_ref/ref_timer    the Makefile's -O3 -ffast-math -fopenmp flags
 #                            (-march=x86-64-v3 instead of -march=native so the
 #                            binary runs on the GPU host's CPU): CPU baseline
+#   oracle/_ref/ref_driver_conj   as ref_driver with -DCONJUGATE (the
+#                            conjugate heat-transfer variant of defines.hpp)
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 REF="${LBM_REFERENCE_DIR:-/root/reference}"
@@ -25,7 +27,7 @@ if [ ! -d "$SRC" ]; then
     echo "build_ref: $SRC not present; skipping" >&2
     exit 0
 fi
-mkdir -p "$OUT/obj_golden" "$OUT/obj_timer"
+mkdir -p "$OUT/obj_golden" "$OUT/obj_timer" "$OUT/obj_conj"
 CXX="${CXX:-g++}"
 DEFS="-DDEFINES_H -DCANTERA_H -DD3Q27 -DNDIM=3 -DPARALLEL -DNUM_THREADS=8 -DSPECIES_MIN=1E-8
       -DTYPE_F=0 -DTYPE_S=1 -DTYPE_P=2 -DTYPE_A=3 -DTYPE_FS=4 -DTYPE_I=5 -DTYPE_O=6 -DTYPE_I_C=7
@@ -44,5 +46,6 @@ build_one() {  # $1 = obj dir, $2 = flags, $3 = output binary
 
 build_one "$OUT/obj_golden" "-O2" "$OUT/ref_driver"
 build_one "$OUT/obj_timer" "-O3 -ffast-math -march=x86-64-v3" "$OUT/ref_timer"
-rm -rf "$OUT/obj_golden" "$OUT/obj_timer"
-echo "build_ref: built $OUT/ref_driver and $OUT/ref_timer"
+build_one "$OUT/obj_conj" "-O2 -DCONJUGATE" "$OUT/ref_driver_conj"
+rm -rf "$OUT/obj_golden" "$OUT/obj_timer" "$OUT/obj_conj"
+echo "build_ref: built $OUT/ref_driver, $OUT/ref_timer and $OUT/ref_driver_conj"

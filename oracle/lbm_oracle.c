@@ -150,11 +150,14 @@ void orc_collide_single(orc_single* s)
         for (int j = 0; j < s->ny; ++j)
             for (int k = 0; k < s->nz; ++k) {
                 size_t c = cell(s, i, j, k);
-                if (s->type[c] != ORC_F) continue;
+                /* CONJUGATE solid cells collide with the same formula and a
+                 * conductivity ten times the gas's (lbm-collision.cpp:101-180) */
+                int solid = s->conjugate && s->type[c] == ORC_S;
+                if (s->type[c] != ORC_F && !solid) continue;
                 double rho = s->rho[c], p = s->p[c];
                 double theta = s->R * s->T[c];
                 double mu = s->nu * rho;
-                double kappa = mu * cp / s->prtl;
+                double kappa = solid ? 10.0 * mu * cp / s->prtl : mu * cp / s->prtl;
                 double om1 = 2.0 * p * cp / (p * cp + 2.0 * kappa);
                 double om = 2.0 * p / (p + 2.0 * mu);
                 if (1.0 / om <= 0.5 || 1.0 / om1 <= 0.5) s->tau_warnings++;
@@ -218,6 +221,26 @@ void orc_stream_single(orc_single* s)
         for (int j = 0; j < s->ny; ++j)
             for (int k = 0; k < s->nz; ++k) {
                 size_t c = cell(s, i, j, k);
+                if (s->conjugate && s->type[c] == ORC_S) {
+                    /* CONJUGATE solid cells stream g only: nothing from gas
+                     * cells (TMS_BC supplies it), bounce-back from adiabatic
+                     * walls, a pull from solid and periodic cells through the
+                     * ghost wrap (lbm-streaming.cpp:120-165) */
+                    for (int l = 0; l < 27; ++l) {
+                        int in = i - CX[l], jn = j - CY[l], kn = k - CZ[l];
+                        int tn = s->type[cell(s, in, jn, kn)];
+                        if (tn == ORC_F || tn == ORC_Q) {
+                        } else if (tn == ORC_A) {
+                            s->g[l * N + c] = s->gpc[OPP[l] * N + c];
+                        } else if (tn == ORC_S || tn == ORC_P) {
+                            if (in == 0) in = s->nx - 2; else if (in == s->nx - 1) in = 1;
+                            if (jn == 0) jn = s->ny - 2; else if (jn == s->ny - 1) jn = 1;
+                            if (kn == 0) kn = s->nz - 2; else if (kn == s->nz - 1) kn = 1;
+                            s->g[l * N + c] = s->gpc[l * N + cell(s, in, jn, kn)];
+                        }
+                    }
+                    continue;
+                }
                 if (s->type[c] != ORC_F) continue;
                 for (int l = 0; l < 27; ++l) {
                     int in = i - CX[l], jn = j - CY[l], kn = k - CZ[l];
@@ -253,6 +276,26 @@ void orc_moments_single(orc_single* s)
     size_t N = ncell(s);
     double cv = s->R / (s->gamma - 1.0);
     for (size_t c = 0; c < N; ++c) {
+        if (s->conjugate && s->type[c] == ORC_S) {
+            /* CONJUGATE solid cells: energy and its flux from g, then
+             * T = (rho e / rho) / c_v and p = rho R T at the stored density
+             * (lbm-moment.cpp:85-120) */
+            double re = 0, qx = 0, qy = 0, qz = 0;
+            for (int l = 0; l < 27; ++l) {
+                double gl = s->g[l * N + c];
+                re += gl;
+                qx += gl * CX[l];
+                qy += gl * CY[l];
+                qz += gl * CZ[l];
+            }
+            s->rhoe[c] = re;
+            s->eflux[c] = qx;
+            s->eflux[N + c] = qy;
+            s->eflux[2 * N + c] = qz;
+            s->T[c] = s->rhoe[c] / s->rho[c] / cv;
+            s->p[c] = s->rho[c] * s->R * s->T[c];
+            continue;
+        }
         if (s->type[c] != ORC_F) continue;
         double rho = 0, mx = 0, my = 0, mz = 0, re = 0, qx = 0, qy = 0, qz = 0;
         double P[9] = {0};
@@ -408,6 +451,52 @@ void orc_tms_single(orc_single* s)
                 T = 1.0 / nd * T;
                 feq_geq(s, rbb, vel, T, fb, gb);
                 tms_correct(s, i, j, k, ORC_S, fb, gb, NULL, NULL);
+            }
+    if (!s->conjugate) return;
+    /* CONJUGATE (lbm-bc.cpp:1230-1346): solid cells with a gas neighbour
+     * upstream and solid downstream rebuild g (f is left as it is) from the
+     * target state at rest with the interface temperature at q = 1/2, the
+     * gas directions replaced by the target set in the local moments. */
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->ny; ++j)
+            for (int k = 0; k < s->nz; ++k) {
+                size_t N = ncell(s), c = cell(s, i, j, k);
+                if (s->type[c] != ORC_S) continue;
+                int nd = 0;
+                double T = 0, q = 0.5, zero[3] = {0, 0, 0};
+                for (int l = 0; l < 27; ++l) {
+                    size_t up = cell(s, i - CX[l], j - CY[l], k - CZ[l]), dn = cell(s, i + CX[l], j + CY[l], k + CZ[l]);
+                    if (!((s->type[up] == ORC_F || s->type[up] == ORC_Q) && s->type[dn] == ORC_S)) continue;
+                    ++nd;
+                    T += (q * s->T[dn] + s->T[up]) / (1.0 + q);
+                }
+                if (nd == 0) continue;
+                double rbb = 0;
+                for (int l = 0; l < 27; ++l) rbb += s->f[l * N + c];
+                T = 1.0 / nd * T;
+                feq_geq(s, rbb, zero, T, fb, gb);
+                double cv = s->R / (s->gamma - 1.0);
+                double rho = 0, mx = 0, my = 0, mz = 0, re = 0;
+                for (int l = 0; l < 27; ++l) {
+                    int ut = s->type[cell(s, i - CX[l], j - CY[l], k - CZ[l])];
+                    int gas = ut == ORC_F || ut == ORC_Q;
+                    double a = gas ? fb[l] : s->f[l * N + c];
+                    rho += a;
+                    mx += a * CX[l];
+                    my += a * CY[l];
+                    mz += a * CZ[l];
+                    re += gas ? gb[l] : s->g[l * N + c];
+                }
+                double vl[3] = {mx / rho, my / rho, mz / rho};
+                double Tl = (re / rho - 0.5 * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2])) / cv;
+                feq_geq(s, rho, vl, Tl, ft, gt);
+                for (int l = 0; l < 27; ++l) {
+                    int ut = s->type[cell(s, i - CX[l], j - CY[l], k - CZ[l])];
+                    if (ut == ORC_F || ut == ORC_Q)
+                        s->g[l * N + c] = 2 * gb[l] - gt[l];
+                    else
+                        s->g[l * N + c] = gb[l] + s->g[l * N + c] - gt[l];
+                }
             }
 }
 

@@ -19,6 +19,9 @@
  *       compiled reference on a periodic Taylor-Green box (ref_driver tgv)
  *   orc_stream wall branches (S, A, FS)         - permutation/mirror identities
  *       on hand-built cases (parity unpinned against the reference binary)
+ *   the CONJUGATE branches (solid cells)        - whole-state snapshots of the
+ *       reference compiled with -DCONJUGATE (oracle/_ref/ref_driver_conj) on a
+ *       fluid layer between two solid slabs
  */
 #ifndef LBM_ORACLE_H
 #define LBM_ORACLE_H
@@ -39,6 +42,7 @@ typedef struct {
     double* eflux;             /* [3][ncell] */
     double *f, *fpc, *g, *gpc; /* [27][ncell] */
     int64_t tau_warnings;      /* cells with 1/omega <= 0.5 seen by collide */
+    int conjugate;             /* CONJUGATE: TYPE_S cells conduct heat (defines.hpp:62-63) */
 } orc_single;
 
 double orc_feq(int l, double rho, const double vel[3], double theta, const double corr[3]);

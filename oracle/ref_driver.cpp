@@ -359,6 +359,86 @@ static int mode_time(int argc, char** argv)
     return 0;
 }
 
+// Conjugate heat-transfer box, for the build compiled with -DCONJUGATE:
+// periodic ghosts on every face; solid slabs of `ns` cells at the bottom
+// (T_hot) and the top (T_cold) of the y range, gas between them at T0 and p0
+// moving with u = U plus small transverse waves.  The two slabs meet through
+// the periodic y wrap, so a solid cell has only solid, periodic or gas
+// neighbours (the reference's solid loops index i +- 1 without bounds).
+struct ConjParams { int nx, ny, nz, ns; double U, nu, R, gamma, Pr, T0, p0, Thot, Tcold; };
+
+static void setup_conj(LBM& lb, const ConjParams& P)
+{
+    lb.set_gasconst(P.R);
+    lb.set_gamma(P.gamma);
+    lb.set_prtl(P.Pr);
+    int Nx = lb.get_Nx(), Ny = lb.get_Ny(), Nz = lb.get_Nz();
+    double H = Ny - 2 - 2 * P.ns;  // gas layer height
+    for (int i = 0; i < Nx; ++i)
+        for (int j = 0; j < Ny; ++j)
+            for (int k = 0; k < Nz; ++k) {
+                MIXTURE& m = lb.mixture[i][j][k];
+                bool ghost = i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1 || k == 0 || k == Nz - 1;
+                m.u = 0.0; m.v = 0.0; m.w = 0.0;
+                m.temp = P.T0;
+                m.p = P.p0;
+                if (ghost) {
+                    m.type = TYPE_P;
+                } else if (j <= P.ns) {
+                    m.type = TYPE_S;
+                    m.temp = P.Thot;
+                } else if (j >= Ny - 1 - P.ns) {
+                    m.type = TYPE_S;
+                    m.temp = P.Tcold;
+                } else {
+                    double y = (j - P.ns - 0.5) / H;
+                    m.u = P.U;
+                    m.v = 0.1 * P.U * sin(2.0 * M_PI * i / (Nx - 2)) * sin(M_PI * y);
+                    m.w = 0.05 * P.U * cos(2.0 * M_PI * (k - 1) / (Nz - 2)) * sin(M_PI * y);
+                }
+                m.rho = m.p / (P.R * m.temp);
+            }
+}
+
+// conj <nx> <ny> <nz> <ns> <nsteps> <snap csv> <outdir> <U> <nu> <R> <gamma> <Pr> <T0> <p0> <Thot> <Tcold>
+static int mode_conj(int argc, char** argv)
+{
+    if (argc < 18) return 2;
+    ConjParams P;
+    P.nx = atoi(argv[2]); P.ny = atoi(argv[3]); P.nz = atoi(argv[4]); P.ns = atoi(argv[5]);
+    int nsteps = atoi(argv[6]);
+    std::vector<int> snaps;
+    {
+        std::string s(argv[7]);
+        size_t pos = 0;
+        while (pos < s.size()) {
+            size_t e = s.find(',', pos);
+            if (e == std::string::npos) e = s.size();
+            snaps.push_back(atoi(s.substr(pos, e - pos).c_str()));
+            pos = e + 1;
+        }
+    }
+    std::string outdir(argv[8]);
+    P.U = atof(argv[9]); P.nu = atof(argv[10]); P.R = atof(argv[11]); P.gamma = atof(argv[12]);
+    P.Pr = atof(argv[13]); P.T0 = atof(argv[14]); P.p0 = atof(argv[15]); P.Thot = atof(argv[16]);
+    P.Tcold = atof(argv[17]);
+    LBM lb(P.nx, P.ny, P.nz, P.nu);
+    setup_conj(lb, P);
+    lb.Init();
+    for (int s : snaps) if (s == 0) dump_snapshot(lb, outdir + "/snap_0.bin");
+    for (int step = 1; step <= nsteps; ++step) {
+        lb.Collide();
+        if (std::find(snaps.begin(), snaps.end(), -step) != snaps.end())  // post-collision state
+            dump_snapshot(lb, outdir + "/post_" + std::to_string(step) + ".bin");
+        lb.Streaming();
+        lb.TMS_BC();
+        lb.calculate_moment();
+        for (int s : snaps)
+            if (s == step) dump_snapshot(lb, outdir + "/snap_" + std::to_string(step) + ".bin");
+    }
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) { fprintf(stderr, "usage: ref_driver fn|tgv|time ...\n"); return 2; }
@@ -367,5 +447,6 @@ int main(int argc, char** argv)
     if (!strcmp(argv[1], "time")) return mode_time(argc, argv);
     if (!strcmp(argv[1], "channel")) return mode_channel(argc, argv);
     if (!strcmp(argv[1], "io")) return mode_io(argc, argv);
+    if (!strcmp(argv[1], "conj")) return mode_conj(argc, argv);
     return 2;
 }

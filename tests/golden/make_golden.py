@@ -27,6 +27,11 @@ Fixtures
   channel_io.npz     6x3x3 channel after 2 steps, written by the reference's
                      OutputVTK (field000002.vtr) and write_restart
                      (restart000002.dat): the two files' bytes
+  conjugate.npz      6x10x3 periodic box with two 2-cell solid slabs (hot
+                     below, cold above) and gas between them, from the
+                     reference built with -DCONJUGATE (ref_driver_conj):
+                     interior moments and populations after steps 1, 5, 60,
+                     and the post-collision populations of step 1
 """
 import math
 import os
@@ -39,6 +44,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+DRIVER_CONJ = os.path.join(REPO, "oracle", "_ref", "ref_driver_conj")
 
 # 3-D TGV parameters of the 64^3 configuration: R = 1/3, T0 = 1, p0 = 1/3,
 # gamma = 1.4, Pr = 0.71, Ma = 0.1 -> U0 = 0.1 sqrt(gamma R T0), Re = 1600 on
@@ -71,6 +77,21 @@ def run_tgv(p, nsteps, snaps, env):
 CHANNEL = dict(nx=10, ny=5, nz=3, U=0.05, nu=0.02, R=R_GAS, gamma=GAMMA, Pr=PR, T0=1.0, p0=1.0 / 3.0,
                Tin=1.1, Tw=0.9, pout=0.999 / 3.0)
 CHANNEL_KEYS = ("nx", "ny", "nz", "U", "nu", "R", "gamma", "Pr", "T0", "p0", "Tin", "Tw", "pout")
+
+
+CONJ = dict(nx=6, ny=10, nz=3, ns=2, U=0.03, nu=0.02, R=R_GAS, gamma=GAMMA, Pr=PR, T0=1.0, p0=1.0 / 3.0,
+            Thot=1.2, Tcold=0.9)
+CONJ_KEYS = ("nx", "ny", "nz", "ns", "U", "nu", "R", "gamma", "Pr", "T0", "p0", "Thot", "Tcold")
+
+
+def run_conj(p, nsteps, snaps, env):
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [DRIVER_CONJ, "conj", str(p["nx"]), str(p["ny"]), str(p["nz"]), str(p["ns"]), str(nsteps),
+               ",".join(map(str, snaps)), d] + [repr(p[k]) for k in CONJ_KEYS[4:]]
+        subprocess.run(cmd, check=True, env=env, stdout=subprocess.DEVNULL)
+        n = p["nx"] * p["ny"] * p["nz"]
+        return {s: np.fromfile(os.path.join(d, ("post_%d" % -s if s < 0 else "snap_%d" % s) + ".bin")).reshape(n, 127)
+                for s in snaps}
 
 
 def run_channel(p, nsteps, snaps, env):
@@ -137,6 +158,15 @@ def main():
         rst = np.frombuffer(open(os.path.join(d, "restart000002.dat"), "rb").read(), np.uint8)
     np.savez_compressed(os.path.join(HERE, "channel_io.npz"), vtk=vtk, restart=rst, steps=np.array(2),
                         params=np.array([io[k] for k in CHANNEL_KEYS], np.float64))
+    sn = run_conj(CONJ, 60, [-1, 1, 5, 60], env)
+    arrays = {"params": np.array([CONJ[k] for k in CONJ_KEYS], np.float64)}
+    for s in (1, 5, 60):
+        arrays[f"mom_{s}"] = sn[s][:, :7]
+        arrays[f"f_{s}"] = sn[s][:, 19:46]
+        arrays[f"g_{s}"] = sn[s][:, 46:73]
+    arrays["fpc_post1"] = sn[-1][:, 73:100]
+    arrays["gpc_post1"] = sn[-1][:, 100:127]
+    np.savez_compressed(os.path.join(HERE, "conjugate.npz"), **arrays)
     print("golden fixtures written to", HERE)
 
 

@@ -418,3 +418,93 @@ def test_channel_fp32_tracks_fp64(golden_dir):
         ref = interior(a.get_field(name))
         scale = max(np.max(np.abs(interior(a.get_field("u")))), 1e-300) if name in ("u", "v") else np.max(np.abs(ref))
         assert np.max(np.abs(interior(b.get_field(name)) - ref)) / scale < 1e-5, name
+
+
+# --------------------------------------------------------------- CONJUGATE
+CJ_KEYS = ("nx", "ny", "nz", "ns", "U", "nu", "R", "gamma", "Pr", "T0", "p0", "Thot", "Tcold")
+
+
+def _conj(params, precision=LBM_FP64):
+    p = dict(zip(CJ_KEYS, params))
+    nx, ny, nz, ns = int(p["nx"]), int(p["ny"]), int(p["nz"]), int(p["ns"])
+    print(f"conjugate box {nx}x{ny}x{nz} slabs {ns} precision {precision}", flush=True)
+    fl = cases.conjugate_box(nx, ny, nz, ns, p["U"], T0=p["T0"], p0=p["p0"], T_hot=p["Thot"], T_cold=p["Tcold"],
+                             R=p["R"])
+    pr = dict(R=p["R"], gamma=p["gamma"], Pr=p["Pr"], nu=p["nu"])
+    lat = Lattice(nx, ny, nz, pr["nu"], gas_const=pr["R"], gamma=pr["gamma"], prtl=pr["Pr"], mode=LBM_MODE_STRICT,
+                  precision=precision, ref_rho=1.0, ref_T=p["T0"])
+    lat.set_conjugate(True)
+    lat.load_case(fl)
+    s = oracle_from(fl, pr)
+    s._s.conjugate = 1
+    return lat, s
+
+
+def _conj_params(golden_dir):
+    return np.load(os.path.join(golden_dir, "conjugate.npz"))["params"]
+
+
+def test_conjugate_calls_match_oracle(golden_dir):
+    """Collide, Streaming, TMS_BC and calculate_moment of the conjugate
+    variant, call by call against the oracle over 5 steps (solid cells
+    included), and the solid's f left untouched by every call."""
+    lat, s = _conj(_conj_params(golden_dir))
+    lat.Init()
+    s.init()
+    f0 = lat.get_populations(POP_F)
+    solid = s.type == ob.S
+    for step in range(1, 6):
+        lat.Collide()
+        s.collide()
+        assert relerr(interior(lat.get_populations(POP_FPC)), interior(s.fpc)) < 1e-12, step
+        assert relerr(interior(lat.get_populations(POP_GPC)), interior(s.gpc)) < 1e-12, step
+        lat.Streaming()
+        s.stream()
+        assert relerr(interior(lat.get_populations(POP_G)), interior(s.g)) < 1e-12, step
+        lat.TMS_BC()
+        s.tms_bc()
+        assert relerr(interior(lat.get_populations(POP_G)), interior(s.g)) < 1e-12, step
+        assert relerr(interior(lat.get_populations(POP_F)), interior(s.f)) < 1e-12, step
+        lat.calculate_moment()
+        s.moments()
+        compare_fields(lat, s, 1e-12, f"step {step}")
+    assert np.array_equal(lat.get_populations(POP_F)[:, solid], f0[:, solid])
+
+
+def test_conjugate_matches_reference_snapshots(golden_dir):
+    """The reference built with -DCONJUGATE: moments and populations of every
+    interior cell after steps 1, 5 and 60."""
+    gd = np.load(os.path.join(golden_dir, "conjugate.npz"))
+    lat, _ = _conj(gd["params"])
+    lat.Init()
+    step = 0
+    for target in (1, 5, 60):
+        lat.step(target - step)
+        step = target
+        ref = gd[f"mom_{target}"]
+        uscale = np.max(np.abs(ref[:, 1:4]))
+        for col, name in enumerate(SCAL):
+            got = interior(lat.get_field(name)).reshape(-1)
+            scale = uscale if 1 <= col <= 3 else np.max(np.abs(ref[:, col]))
+            assert np.max(np.abs(got - ref[:, col])) / scale < 1e-10, (target, name)
+        f = interior(lat.get_populations(POP_F)).reshape(27, -1).T
+        g = interior(lat.get_populations(POP_G)).reshape(27, -1).T
+        assert relerr(f, gd[f"f_{target}"]) < 1e-10 and relerr(g, gd[f"g_{target}"]) < 1e-10
+
+
+def test_conjugate_fp32_tracks_fp64(golden_dir):
+    params = _conj_params(golden_dir)
+    a, _ = _conj(params, LBM_FP64)
+    b, _ = _conj(params, LBM_FP32_SHIFTED)
+    for lat in (a, b):
+        lat.Init()
+        lat.step(60)
+    for name in ("rho", "T", "p"):
+        ref = interior(a.get_field(name))
+        assert np.max(np.abs(interior(b.get_field(name)) - ref)) / np.max(np.abs(ref)) < 1e-5, name
+
+
+def test_conjugate_needs_strict_single_component():
+    lat = Lattice(4, 4, 4, 0.02, gas_const=1.0 / 3.0, mode=LBM_MODE_FUSED)
+    with pytest.raises(LbmError, match="STRICT"):
+        lat.set_conjugate(True)

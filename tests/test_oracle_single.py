@@ -208,6 +208,73 @@ def test_channel_state_matches_reference(golden_dir):
     assert T[0, 1:-1].mean() > 1.05 and T[:, 0].mean() < 1.0
 
 
+# --------------------------------------------------------------- CONJUGATE
+CJ_KEYS = ("nx", "ny", "nz", "ns", "U", "nu", "R", "gamma", "Pr", "T0", "p0", "Thot", "Tcold")
+
+
+def _conj_state(params, conjugate=True):
+    p = dict(zip(CJ_KEYS, params))
+    nx, ny, nz, ns = int(p["nx"]), int(p["ny"]), int(p["nz"]), int(p["ns"])
+    fl = cases.conjugate_box(nx, ny, nz, ns, p["U"], T0=p["T0"], p0=p["p0"], T_hot=p["Thot"], T_cold=p["Tcold"],
+                             R=p["R"])
+    s = ob.SingleState(nx + 2, ny + 2, nz + 2, p["R"], p["gamma"], p["Pr"], p["nu"], conjugate=conjugate)
+    for k in ("type", "rho", "u", "v", "w", "p", "T"):
+        getattr(s, k)[...] = fl[k]
+    return s
+
+
+def test_conjugate_matches_reference(golden_dir):
+    """Solid slabs conducting heat (the reference built with -DCONJUGATE):
+    the post-collision state of step 1 and the moments and populations of
+    every cell, solid ones included, after steps 1, 5 and 60."""
+    gd = np.load(os.path.join(golden_dir, "conjugate.npz"))
+    s = _conj_state(gd["params"])
+    s.init()
+    f0 = s.f.copy()
+    s.collide()
+    assert _relerr(_interior(s.fpc).T, gd["fpc_post1"]) < 1e-14
+    assert _relerr(_interior(s.gpc).T, gd["gpc_post1"]) < 1e-14
+    s.stream()
+    s.tms_bc()
+    s.moments()
+    step = 1
+    for target in (1, 5, 60):
+        while step < target:
+            s.step()
+            step += 1
+        mom = np.stack([_interior(getattr(s, k)) for k in ob.SingleState.SCALARS], axis=1)
+        ref = gd[f"mom_{target}"]
+        uscale = np.max(np.abs(ref[:, 1:4]))
+        for col, name in enumerate(ob.SingleState.SCALARS):
+            scale = uscale if 1 <= col <= 3 else np.max(np.abs(ref[:, col]))
+            err = np.max(np.abs(mom[:, col] - ref[:, col])) / scale
+            assert err < 1e-12, (target, name, err)
+        assert _relerr(_interior(s.f).T, gd[f"f_{target}"]) < 1e-12
+        assert _relerr(_interior(s.g).T, gd[f"g_{target}"]) < 1e-12
+    solid = s.type == ob.S
+    # the solid's f is never streamed: it keeps its initial equilibrium
+    assert np.array_equal(s.f[:, solid], f0[:, solid])
+    # heat has crossed the gas: the hot slab cooled, the cold one warmed
+    T = s.T
+    ny = s.type.shape[1]
+    assert T[1:-1, 1, 1:-1].max() < 1.2 and T[1:-1, ny - 2, 1:-1].min() > 0.9
+
+
+def test_conjugate_flag_moves_the_solid():
+    """Without CONJUGATE the solid cells are walls with fixed fields; with it
+    their temperature evolves (a check that the solid branches run)."""
+    gd_params = [6, 10, 3, 2, 0.03, 0.02, 1.0 / 3.0, 1.4, 0.71, 1.0, 1.0 / 3.0, 1.2, 0.9]
+    out = {}
+    for on in (False, True):
+        s = _conj_state(gd_params, conjugate=on)
+        s.init()
+        for _ in range(5):
+            s.step()
+        out[on] = s.T[s.type == ob.S].copy()
+    assert np.all(out[False] == np.where(out[False] > 1.0, 1.2, 0.9))
+    assert np.max(np.abs(out[True] - out[False])) > 1e-4
+
+
 def _uniform(s, rho, u, T):
     """Every cell at the equilibrium of (rho, u, T), fields included."""
     n = s.f.shape[1:]
