@@ -566,9 +566,6 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
     std::vector<int> tcell, tdir, icell, idir, wcell;
     std::string tms_unsupported;
     if (multi(c)) {
-        for (int64_t i = 0; i < d.n; ++i)
-            if (types[i] == TYPE_FS)
-                return fail(c, LBM_E_UNSUPPORTED, "free-slip walls are not provided for the multicomponent state");
         // the outflow sweep's interface detection (lbm-bc.cpp:141-155): the first
         // direction whose upstream cell is an outflow ghost and whose downstream
         // cell is fluid; only axis-aligned interfaces are reconstructed

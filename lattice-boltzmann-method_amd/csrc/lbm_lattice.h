@@ -32,6 +32,16 @@ __host__ __device__ constexpr int cz(int l)
     return t[l];
 }
 __host__ __device__ constexpr int opp(int l) { return l == 0 ? 0 : (l & 1 ? l + 1 : l - 1); }
+
+// Index of the lattice direction (a, b, c), each in {-1, 0, 1}.
+__host__ __device__ inline int dir_index(int a, int b, int c)
+{
+    int r = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        if (cx(l) == a && cy(l) == b && cz(l) == c) r = l;
+    return r;
+}
 __host__ __device__ constexpr int c2(int l) { return cx(l) * cx(l) + cy(l) * cy(l) + cz(l) * cz(l); }
 
 // Per-axis factors of the product-form equilibrium (lbm.hpp:276-305):

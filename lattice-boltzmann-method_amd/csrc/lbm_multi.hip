@@ -811,9 +811,27 @@ __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
         if (tn == TYPE_S || tn == TYPE_A) {
             // bounce-back; the energy populations bounce only at adiabatic walls
             if (set < NS || tn == TYPE_A) dst_base[(int64_t)l * n + c] = src_base[(int64_t)opp(l) * n + c];
-        } else if (tn == TYPE_O || tn == TYPE_O_C || tn == TYPE_I || tn == TYPE_I_C || tn == TYPE_FS) {
-            // inflow/outflow sources are left for TMS_BC; free-slip walls are
-            // rejected for the multicomponent state at set-up
+        } else if (tn == TYPE_FS) {
+            // free-slip wall (lbm-streaming.cpp:34-49): the mirrored direction
+            // from the mirrored source (dirSlip, lbm-bc.cpp:1352-1399)
+            int a = cx(l), b = cy(l), e = cz(l);
+            int ip = x.i - a, jp = x.j - b, kp = x.k - e;
+            if (v.type[cell(v.d, x.i - a, x.j, x.k)] == TYPE_FS) {
+                a = -a;
+                ip = x.i;
+            } else if (v.type[cell(v.d, x.i, x.j - b, x.k)] == TYPE_FS) {
+                b = -b;
+                jp = x.j;
+            } else if (v.type[cell(v.d, x.i, x.j, x.k - e)] == TYPE_FS) {
+                e = -e;
+                kp = x.k;
+            }
+            ip = ((ip - 1 + (nx - 2)) % (nx - 2)) + 1;
+            jp = ((jp - 1 + (ny - 2)) % (ny - 2)) + 1;
+            kp = ((kp - 1 + (nz - 2)) % (nz - 2)) + 1;
+            dst_base[(int64_t)l * n + c] = src_base[(int64_t)dir_index(a, b, e) * n + cell(v.d, ip, jp, kp)];
+        } else if (tn == TYPE_O || tn == TYPE_O_C || tn == TYPE_I || tn == TYPE_I_C) {
+            // inflow/outflow sources are left for TMS_BC
         } else {
             if (in == 0) in = nx - 2; else if (in == nx - 1) in = 1;
             if (jn == 0) jn = ny - 2; else if (jn == ny - 1) jn = 1;

@@ -52,14 +52,6 @@ __device__ inline Src3 src_axis(int x, int n)
     return r;
 }
 
-__device__ inline int dir_index(int a, int b, int c)
-{
-    int r = 0;
-#pragma unroll
-    for (int l = 0; l < Q; ++l)
-        if (cx(l) == a && cy(l) == b && cz(l) == c) r = l;
-    return r;
-}
 
 template <typename P>
 __device__ inline P ldp(const P* base, int l, int64_t n, int c)

@@ -404,6 +404,25 @@ class MultiState:
         self.gpc[:, c] = g + om1 * (ge - g) + (om - om1) * (ge - gs)
 
     # ---------------------------------------------------------- Streaming
+    def _dir_slip(self, l, i, j, k):
+        """LBM::dirSlip (lbm-bc.cpp:1352-1399): the direction mirrored at the
+        first free-slip neighbour along x, y, z, and its source cell wrapped
+        into the interior."""
+        nx, ny, nz = self.dims
+        a, b, e = int(CX[l]), int(CY[l]), int(CZ[l])
+        ip, jp, kp = i - a, j - b, k - e
+        if self.type[self._idx(i - a, j, k)] == TYPE_FS:
+            a, ip = -a, i
+        elif self.type[self._idx(i, j - b, k)] == TYPE_FS:
+            b, jp = -b, j
+        elif self.type[self._idx(i, j, k - e)] == TYPE_FS:
+            e, kp = -e, k
+        ip = (ip - 1 + (nx - 2)) % (nx - 2) + 1
+        jp = (jp - 1 + (ny - 2)) % (ny - 2) + 1
+        kp = (kp - 1 + (nz - 2)) % (nz - 2) + 1
+        lp = int(np.nonzero((CX == a) & (CY == b) & (CZ == e))[0][0])
+        return lp, int(self._idx(ip, jp, kp))
+
     def stream(self):
         nx, ny, nz = self.dims
         c = np.nonzero(self.type == TYPE_F)[0]
@@ -416,8 +435,13 @@ class MultiState:
             self.f[:, l, c[sel]] = self.fpc[:, OPP[l], c[sel]]
             selA = tn == TYPE_A
             self.g[l, c[selA]] = self.gpc[OPP[l], c[selA]]
-            if np.any(tn == TYPE_FS):
-                raise NotImplementedError("free-slip walls are not used by the multicomponent cases")
+            # free-slip walls (lbm-streaming.cpp:34-49): the population of the
+            # mirrored direction from the mirrored source (dirSlip,
+            # lbm-bc.cpp:1352-1399), species and energy alike
+            for cc, ci, cj, ck in zip(c[tn == TYPE_FS], i[tn == TYPE_FS], j[tn == TYPE_FS], k[tn == TYPE_FS]):
+                lp, src = self._dir_slip(l, int(ci), int(cj), int(ck))
+                self.f[:, l, cc] = self.fpc[:, lp, src]
+                self.g[l, cc] = self.gpc[lp, src]
             per = ~np.isin(tn, (TYPE_S, TYPE_A, TYPE_FS, TYPE_O, TYPE_O_C, TYPE_I, TYPE_I_C))
             ii = np.where(inb == 0, nx - 2, np.where(inb == nx - 1, 1, inb))
             jj = np.where(jnb == 0, ny - 2, np.where(jnb == ny - 1, 1, jnb))

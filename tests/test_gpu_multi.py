@@ -195,13 +195,34 @@ def test_mc_mixing_layer_outflow_calls_match_oracle(reaction):
         check_state(lat, s, TOL, f"step {step}")
 
 
-def test_mc_unsupported_boundaries_fail_loudly():
-    """Free-slip walls are not provided with species: rejected at set-up."""
-    fl = cases.mixing_layer(4, 6, 4)
+def test_mc_free_slip_walls_match_oracle():
+    """Free-slip walls (TYPE_FS) on both y faces of a reacting box: species
+    and energy populations mirrored by dirSlip at the walls, the walls
+    one-sided in the finite differences; 6 steps against the oracle."""
+    fl = cases.reacting_box(6, 5, 4, T_K=1500.0, amp_T=0.1, amp_u=0.01, X=SEEDED)
     fl["type"][:, 0, :] = cases.TYPE_FS
-    from lbm_amd.capi import LbmError
-    with pytest.raises(LbmError, match="free-slip"):
-        pair(fl)
+    fl["type"][:, -1, :] = cases.TYPE_FS
+    lat, s = pair(fl)
+    lat.Init()
+    s.init()
+    F = fluid(s)
+    lat.calculate_moment()
+    s.moments()
+    for step in range(1, 7):
+        lat.Collide_Species()
+        s.collide_species(True)
+        lat.Collide()
+        s.collide()
+        lat.Streaming()
+        s.stream()
+        e = relerr(lat.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], s.f[:, :, F])
+        assert e < TOL, f"step {step} species f after streaming {e:.3e}"
+        e = relerr(lat.get_populations(POP_G).reshape(27, -1)[:, F], s.g[:, F])
+        assert e < TOL, f"step {step} g after streaming {e:.3e}"
+        lat.TMS_BC()
+        lat.calculate_moment()
+        s.moments()
+        check_state(lat, s, TOL, f"step {step}")
 
 
 @pytest.mark.parametrize("reaction", [False, True])

@@ -356,3 +356,39 @@ def test_plain_inflow_imposes_the_ghost_velocity():
     fa = s.f[:, :, c]
     m = (fa @ C3.T.astype(np.float64)).sum(axis=0)
     np.testing.assert_allclose(m / fa.sum(), ug, rtol=1e-9, atol=1e-15)
+
+
+def test_free_slip_streaming_mirrors_and_conserves():
+    """Free-slip walls on the y faces (dirSlip, lbm-bc.cpp:1352-1399): every
+    population of a cell next to the lower wall that comes from the wall takes
+    the mirrored direction (cy -> -cy) of the neighbour at the same height,
+    and streaming conserves every species' mass and the total energy."""
+    fl = cases.reacting_box(4, 5, 3, T_K=1400.0, amp_T=0.1, amp_u=0.01, X=SEEDED)
+    fl["type"][:, 0, :] = cases.TYPE_FS
+    fl["type"][:, -1, :] = cases.TYPE_FS
+    s = make_state(fl)
+    s.init()
+    rng = np.random.default_rng(4)
+    s.fpc[:] = s.f * (1.0 + 0.01 * rng.standard_normal(s.f.shape))
+    s.gpc[:] = s.g * (1.0 + 0.01 * rng.standard_normal(s.g.shape))
+    F = s.type == 0
+    mass0 = s.fpc[:, :, F].sum(axis=(1, 2))
+    energy0 = s.gpc[:, F].sum()
+    s.stream()
+    np.testing.assert_allclose(s.f[:, :, F].sum(axis=(1, 2)), mass0, rtol=1e-13)
+    np.testing.assert_allclose(s.g[:, F].sum(), energy0, rtol=1e-13)
+    nx, ny, nz = s.dims
+    idx = lambda i, j, k: (i * ny + j) * nz + k
+    checked = 0
+    for l in range(27):
+        if CY[l] != 1:
+            continue
+        lp = int(np.nonzero((CX == CX[l]) & (CY == -1) & (CZ == CZ[l]))[0][0])
+        for i in range(1, nx - 1):
+            for k in range(1, nz - 1):
+                ii = (i - CX[l] - 1) % (nx - 2) + 1
+                kk = (k - CZ[l] - 1) % (nz - 2) + 1
+                np.testing.assert_array_equal(s.f[:, l, idx(i, 1, k)], s.fpc[:, lp, idx(ii, 1, kk)])
+                assert s.g[l, idx(i, 1, k)] == s.gpc[lp, idx(ii, 1, kk)]
+                checked += 1
+    assert checked == 9 * (nx - 2) * (nz - 2)
