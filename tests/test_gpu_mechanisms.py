@@ -148,3 +148,41 @@ def test_fp32_shifted_tracks_fp64(mech):
     for k in ("rho", "T", "p"):
         assert relerr(out[LBM_FP32_SHIFTED][k].ravel()[F], out[LBM_FP64][k].ravel()[F]) < 1e-5, k
     assert relerr(out[LBM_FP32_SHIFTED]["rho_a"][:, F], out[LBM_FP64]["rho_a"][:, F]) < 1e-5
+
+
+@pytest.mark.parametrize("mech", MECHS)
+def test_surrogate_runs_on_six_species(mech):
+    """The chemistry surrogate on a 6-species build: capture rows carry T, p,
+    the 6 mass fractions and dt (NS + 3 features, zero padding to 16) and dT,
+    dY (NS + 1 targets); training lowers the loss; the in-step network
+    replaces the plain chemistry with every cell accounted for (predicted or
+    fallback) and finite results."""
+    fl = box(mech, 8, 6, 6)
+    nx, ny, nz = (n - 2 for n in fl["type"].shape)
+    print(f"mechanism {mech} surrogate capture", flush=True)
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, mode=LBM_MODE_STRICT, precision=LBM_FP64,
+                         mechanism=mech)
+    lat.load_case(fl)
+    lat.Init()
+    lat.calculate_moment()
+    lat.surrogate_capture(1 << 16, 1)
+    lat.Collide_Species()
+    rows = lat.surrogate_captured()
+    ns = len(lat.SPECIES)
+    n = int((fl["type"] == 0).sum())
+    assert rows.shape == (n, 32)
+    np.testing.assert_allclose(rows[:, 2:2 + ns].sum(axis=1), 1.0, rtol=1e-5)
+    assert np.all(rows[:, ns + 2] == rows[0, ns + 2]) and rows[0, ns + 2] > 0  # dt [s]
+    assert np.all(rows[:, ns + 3:16] == 0) and np.all(rows[:, 16 + ns + 1:] == 0)
+    fuel = lat.SPECIES.index("CH4" if mech == "ch4_2s" else "C3H8")
+    assert np.all(rows[:, 17 + fuel] <= 0) and rows[:, 17 + fuel].min() < 0  # the fuel is consumed
+    losses = lat.surrogate_train(300, batch=256, lr=2e-3, seed=3)
+    assert np.all(np.isfinite(losses)) and losses[-20:].mean() < 0.5 * losses[:20].mean()
+    lat.surrogate_capture_stop()
+    lat.surrogate_enable(True, audit_stride=1, audit_tol=0.5)
+    lat.Collide_Species()
+    st = lat.surrogate_stats()
+    assert st["predicted"] + st["fallback"] == n and st["audited"] > 0
+    F = fl["type"].ravel() == 0
+    assert np.all(np.isfinite(lat.get_field("HRR").ravel()[F]))
+    assert np.all(np.isfinite(lat.get_populations(POP_FPC).reshape(ns, 27, -1)[:, :, F]))
