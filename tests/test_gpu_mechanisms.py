@@ -6,7 +6,7 @@ Each library is compared call by call with the numpy oracle running the same
 mechanism (oracle/chem.py, pinned in test_oracle_mechanisms.py) on seeded
 boxes hot enough (2000 K) for both steps to react within a few lattice steps.
 Bar: max relative error 1e-10 in rho, u, T, p, rho_a and X after each step
-the post-collision populations on the same scale.
+(the FP64 parity bar), the post-collision populations on the same scale.
 """
 import numpy as np
 import pytest

@@ -3,7 +3,7 @@
 The HIP kernels are compared call by call with the numpy oracle
 (oracle/multicomp.py, pinned in test_oracle_multicomp.py) on the same seeded
 boxes.  Bar: max relative error 1e-10 in rho, u, T and the species densities
-after N steps; populations are compared on the
+after N steps (the FP64 parity bar); populations are compared on the
 same scale.  Both sides round the lattice/SI crossings to binary32 exactly
 as the reference does, so they agree far below that bar unless a value lands
 on a float rounding boundary.

@@ -4,7 +4,7 @@ The HIP kernels are compared with the CPU oracle (pinned to the reference's
 compiled code in test_oracle_single.py) on the same seeded inputs, and with
 the reference's own golden fixtures directly.
 
-Tolerances (max relative error 1e-10 in rho, u, T):
+Tolerances (FP64 parity bar: max relative error 1e-10 in rho, u, T):
   * per-step fields vs oracle: 1e-12 normwise (max|a-b| / max|b|) while the
     flow is smooth; the reference's Taylor-Green set-up turns unstable after
     ~40 (16^3) / ~100 (64^3) steps and amplifies rounding differences there,
