@@ -21,6 +21,9 @@ Workloads
             plain path is timed beside it for the speedup and the FP32 error
             against FP64 after the timed steps.
   tgv64     3-D Taylor-Green vortex, 64^3, FP64 plain path.
+  mixing_c5share  the same mixing layer on one GPU's share of the 768x384x384
+            layer split into 8 x-slabs: 96x384x384 (14.2 M cells), periodic
+            in x in place of the slab exchange; FP64 on the same share beside it.
 """
 import argparse
 import json
@@ -47,7 +50,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="mixing256", choices=["mixing256", "turb256", "tgv64", "copy1g"])
+    ap.add_argument("--workload", default="mixing256",
+                    choices=["mixing256", "mixing_c5share", "turb256", "tgv64", "copy1g"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -173,11 +177,10 @@ def cpu_baseline_mixing():
                       f"reacting mixing layer, {steps} steps, {sec:.1f} s, one process"}
 
 
-def mixing_layer_bench(args):
+def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_256x128x128"):
     """The reacting mixing layer: FP64 plain beside FP32 + surrogate."""
     from lbm_amd import cases
     from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, MixtureLattice
-    nx, ny, nz = 256, 128, 128
     cells = nx * ny * nz
     fl = cases.mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
     F = (fl["type"] == 0).ravel()
@@ -232,7 +235,7 @@ def mixing_layer_bench(args):
         bpl = MC_POP_TRAFFIC[dom] * 4
         achieved = bpl * cells / (kern[dom]["avg_ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("mixing256", dom),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, dom),
                     "kernel": dom, "bytes_per_launch": bpl * cells, "avg_ms": round(kern[dom]["avg_ms"], 4)}
 
     if not args.no_compare:
@@ -257,7 +260,7 @@ def mixing_layer_bench(args):
         "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(sec / steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "reacting_mixing_layer_256x128x128_9sp_fp32_shifted_" + args.chemistry, "cells": cells,
+        "config": {"workload": tag + "_9sp_fp32_shifted_" + args.chemistry, "cells": cells,
                    "grid": [nx, ny, nz], "mode": "strict", "parallelism": "none"},
         "roofline": roofline, "cpu_baseline": None, "extra": extra, "host": platform.node(),
     }
@@ -288,6 +291,9 @@ def main():
         sec = time.perf_counter() - t0
         print(json.dumps({"workload": "copy1g", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
                           "copies": args.steps, "GBs": 8 * n * args.steps / sec / 1e9}))
+        return
+    if args.workload == "mixing_c5share":
+        print(json.dumps(mixing_layer_bench(args, 96, 384, 384, "reacting_mixing_layer_c5_share_96x384x384")))
         return
     if args.workload == "mixing256":
         rec = mixing_layer_bench(args)
