@@ -319,19 +319,36 @@ int set_multi_reference_state(lbm_ctx* c)
     else
         launch_mc_state_sums(c->type, c->fld[LBM_FIELD_T], c->fld[LBM_FIELD_P], static_cast<const double*>(c->sp_block),
                              n, c->sums_part, c->stream);
-    std::vector<double> part((size_t)MC_SUMS * MC_SUM_BLOCKS);
+    std::vector<double> part((size_t)MC_STATS * MC_SUM_BLOCKS);
     HIP_TRY(c, hipMemcpyAsync(part.data(), c->sums_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    double sum[MC_SUMS] = {};
-    for (int b = 0; b < MC_SUM_BLOCKS; ++b)
-        for (int q = 0; q < MC_SUMS; ++q) sum[q] += part[(size_t)b * MC_SUMS + q];
+    double sum[MC_SUMS] = {}, xmin[mech::NS];
+    for (int a = 0; a < mech::NS; ++a) xmin[a] = HUGE_VAL;
+    for (int b = 0; b < MC_SUM_BLOCKS; ++b) {
+        for (int q = 0; q < MC_SUMS; ++q) sum[q] += part[(size_t)b * MC_STATS + q];
+        for (int a = 0; a < mech::NS; ++a) xmin[a] = std::min(xmin[a], part[(size_t)b * MC_STATS + MC_SUMS + a]);
+    }
     if (!(sum[0] > 0)) return fail(c, LBM_E_STATE, "no TYPE_F cells");
     using namespace chem;
     double X[NS], Y[NS], xs = 0;
     for (int a = 0; a < NS; ++a) xs += (X[a] = std::max(sum[3 + a], 0.0));
     if (!(xs > 0)) return fail(c, LBM_E_STATE, "mole fractions of the set-up are all zero");
     for (int a = 0; a < NS; ++a) X[a] /= xs;
+    // Binary32 deviations from the mean state resolve a species population only
+    // to ~6e-8 of the mean's: a species whose set-up mole fraction falls below
+    // 1e-6 of its mean somewhere is rounding noise there, and the run diverges
+    // within steps (measured: a premixed flame with 1e-8 traces).  Refuse it.
+    constexpr double kTraceFloor = 1e-6;
+    for (int a = 0; a < NS; ++a)
+        if (X[a] > 0 && !(xmin[a] >= kTraceFloor * X[a])) {
+            char msg[256];
+            snprintf(msg, sizeof msg,
+                     "LBM_FP32_SHIFTED cannot resolve %s: its smallest set-up mole fraction %.3g is below 1e-6 of "
+                     "its mean %.3g (raise the trace floor of the set-up or use LBM_FP64)",
+                     mech::SPECIES_NAMES[a], xmin[a], X[a]);
+            return fail(c, LBM_E_UNSUPPORTED, msg);
+        }
     Y_from_X(X, Y);
     const Units& u = c->units;
     const double Tsi = u.si_temp(sum[1] / sum[0]);
@@ -507,7 +524,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
             if ((e = hipMalloc((void**)&c->mref_dev, sizeof(double) * (mech::NS + 1) * Q)) != hipSuccess)
                 return bail("hipMalloc", e);
             (void)hipMemsetAsync(c->mref_dev, 0, sizeof(double) * (mech::NS + 1) * Q, c->stream);
-            if ((e = hipMalloc((void**)&c->sums_part, sizeof(double) * MC_SUMS * MC_SUM_BLOCKS)) != hipSuccess)
+            if ((e = hipMalloc((void**)&c->sums_part, sizeof(double) * MC_STATS * MC_SUM_BLOCKS)) != hipSuccess)
                 return bail("hipMalloc", e);
         }
         c->bytes += 2 * sbytes + fb + srcb;

@@ -1863,23 +1863,31 @@ __global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, co
                                                          const S* X, int64_t n, double* part)
 {
     __shared__ double red[BLOCK];
-    double acc[MC_SUMS] = {};
+    double acc[MC_STATS] = {};
+#pragma unroll
+    for (int a = 0; a < NS; ++a) acc[MC_SUMS + a] = HUGE_VAL;
     for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < n; c += (int64_t)gridDim.x * BLOCK) {
         if (type[c] != TYPE_F) continue;
         acc[0] += 1.0;
         acc[1] += T[c];
         acc[2] += p[c];
 #pragma unroll
-        for (int a = 0; a < NS; ++a) acc[3 + a] += X[a * n + c];
+        for (int a = 0; a < NS; ++a) {
+            const double x = X[a * n + c];
+            acc[3 + a] += x;
+            acc[MC_SUMS + a] = fmin(acc[MC_SUMS + a], x);
+        }
     }
-    for (int q = 0; q < MC_SUMS; ++q) {
+    for (int q = 0; q < MC_STATS; ++q) {
         red[threadIdx.x] = acc[q];
         __syncthreads();
         for (int w = BLOCK / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            if ((int)threadIdx.x < w)
+                red[threadIdx.x] = q < MC_SUMS ? red[threadIdx.x] + red[threadIdx.x + w]
+                                               : fmin(red[threadIdx.x], red[threadIdx.x + w]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * MC_SUMS + q] = red[0];
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * MC_STATS + q] = red[0];
         __syncthreads();
     }
 }

@@ -351,3 +351,89 @@ def test_mc_flame_runs_stably_with_the_inflow_held():
     assert np.all(np.isfinite(T))
     assert abs(T[0] - 300.0) < 30.0 and T[-1] > 1800.0
     assert lat.diagnostics()["nan_count"] == 0
+
+
+def test_mc_flame_full_size_calls_match_oracle():
+    """The 2048x4x4 premixed flame at its configured size (LODI inflow,
+    NSCBC outflow, 9 species): two steps call by call against the oracle,
+    re-seeded before each call."""
+    fl = cases.premixed_flame_1d(2048, 4, 4)
+    lat, s = pair(fl, reaction=True)
+    lat.Init()
+    s.init()
+    F = fluid(s)
+
+    def check(where):
+        # velocities on the scale of the velocity vector: the transverse
+        # components of this quasi-1-D flame are rounding-level
+        us = max(np.max(np.abs(s.u[F])), np.max(np.abs(s.v[F])), np.max(np.abs(s.w[F])))
+        for k in ("rho", "u", "v", "w", "T", "p", "rhoe"):
+            ref = getattr(s, k)[F]
+            scale = us if k in ("u", "v", "w") else np.max(np.abs(ref))
+            e = float(np.max(np.abs(lat.get_field(k).ravel()[F] - ref)) / scale)
+            assert e < TOL, f"{where}: {k} {e:.3e}"
+        e = relerr(lat.get_species_field("rho").reshape(9, -1)[:, F], s.rho_a[:, F])
+        assert e < TOL, f"{where}: rho_a {e:.3e}"
+
+    lat.calculate_moment()
+    s.moments()
+    check("moments after init")
+    for step in range(1, 3):
+        print(f"flame 2048x4x4 step {step}", flush=True)
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide_Species()
+        s.collide_species(True)
+        e = relerr(lat.get_populations(POP_FPC).reshape(9, 27, -1)[:, :, F], s.fpc[:, :, F])
+        assert e < TOL, f"step {step} species fpc {e:.3e}"
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide()
+        s.collide()
+        seed_gpu_from_oracle(lat, s)
+        lat.Streaming()
+        s.stream()
+        seed_gpu_from_oracle(lat, s)
+        lat.TMS_BC()
+        s.tms_bc()
+        e = relerr(lat.get_populations(POP_F).reshape(9, 27, -1)[:, :, F], s.f[:, :, F])
+        assert e < TOL, f"step {step} species f after TMS {e:.3e}"
+        seed_gpu_from_oracle(lat, s)
+        lat.calculate_moment()
+        s.moments()
+        check(f"step {step}")
+
+
+def test_mc_fp32_refuses_unresolvable_trace_species():
+    """Trace species 1e-10 of their mean (the premixed flame's default) are
+    below what binary32 deviations from the mean state resolve: FP32 lbm_init
+    refuses the set-up by name instead of running into NaN."""
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LbmError
+    fl = cases.premixed_flame_1d(64, 3, 3)
+    lat = MixtureLattice(64, 3, 3, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED)
+    lat.load_case(fl)
+    with pytest.raises(LbmError, match="cannot resolve"):
+        lat.Init()
+
+
+def test_mc_fp32_full_size_flame_tracks_fp64():
+    """The 2048x4x4 premixed flame with a 1e-6 trace floor: FP32 shifted
+    storage against FP64 over 300 steps (both finite, flame temperature and
+    species close)."""
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64
+    fl = cases.premixed_flame_1d(2048, 4, 4, minim=1e-6)
+    out = {}
+    for prec in (LBM_FP64, LBM_FP32_SHIFTED):
+        print(f"flame 2048x4x4 precision {prec}", flush=True)
+        lat = MixtureLattice(2048, 4, 4, *cases.UNITS_FLAME, reaction=True, precision=prec)
+        lat.load_case(fl)
+        lat.Init()
+        lat.calculate_moment()
+        lat.step(300)
+        rho_a = lat.get_species_field("rho")[:, 1:-1, 2, 2]
+        out[prec] = (lat.get_field("T")[1:-1, 2, 2].copy(), rho_a / rho_a.sum(axis=0))
+        assert lat.diagnostics()["nan_count"] == 0
+        lat.close()
+    (Ta, Ya), (Tb, Yb) = out[LBM_FP64], out[LBM_FP32_SHIFTED]
+    eT = float(np.max(np.abs(Tb - Ta) / Ta))
+    eY = float(np.sqrt(np.mean((Yb - Ya) ** 2)))
+    print(f"FP32 vs FP64 after 300 steps: T max rel {eT:.3e}, Y L2 {eY:.3e}", flush=True)
+    assert eT < 5e-4 and eY < 5e-6
