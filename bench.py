@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--mode", default="fused", choices=["fused", "strict"],
+                    help="multicomponent workloads: the fused mode (Collide_Species writes the streamed slots; "
+                         "bitwise equal to strict) or one kernel per reference call")
     ap.add_argument("--chemistry", default="surrogate", choices=["surrogate", "plain"],
                     help="mixing256: learned chemistry (default) or the plain integrator in the FP32 run "
                          "(no capture or training; used for the PMC byte-counter passes)")
@@ -94,7 +97,8 @@ def make_lattice(fl, pr, precision, mode):
 
 SINGLE_KERNELS = ("fused_collide_stream", "fused_moments", "collide_explicit")
 MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_fallback", "mc_collide_species",
-                 "mc_collide_mixture", "mc_stream", "mc_tms_outflow", "mc_tms_inflow", "mc_tms_wall", "mc_moments")
+                 "mc_collide_species_push", "mc_collide_mixture", "mc_stream", "mc_stream_energy", "mc_tms_outflow",
+                 "mc_tms_inflow", "mc_tms_wall", "mc_moments")
 
 
 def timed(lat, steps, warmup, names=SINGLE_KERNELS):
@@ -112,6 +116,12 @@ def timed(lat, steps, warmup, names=SINGLE_KERNELS):
         ms, n = lat.profile_read(name)
         if n:
             kernels[name] = {"avg_ms": ms / n, "launches": n, "total_ms": ms}
+    # profile_read matches names as substrings: drop a name whose launches
+    # are all those of a longer name (mc_stream of mc_stream_energy)
+    for name in list(kernels):
+        if any(o != name and name in o and kernels[o]["launches"] == kernels[name]["launches"]
+               and kernels[o]["total_ms"] == kernels[name]["total_ms"] for o in kernels):
+            del kernels[name]
     lat.profile(False)
     return (t1 - t0), kernels
 
@@ -149,7 +159,8 @@ def cpu_baseline(threads):
 # Algorithmic HBM bytes per cell of the multicomponent kernels: the
 # populations each one reads and writes (9 species x 27 f, 27 g), per byte
 # of storage.  Moment and property planes are excluded.
-MC_POP_TRAFFIC = {"mc_collide_species": 2 * 9 * 27, "mc_collide_mixture": 2 * 27, "mc_stream": 2 * 10 * 27,
+MC_POP_TRAFFIC = {"mc_collide_species": 2 * 9 * 27, "mc_collide_species_push": 2 * 9 * 27,
+                  "mc_collide_mixture": 2 * 27, "mc_stream": 2 * 10 * 27, "mc_stream_energy": 2 * 27,
                   "mc_moments": 10 * 27}
 
 
@@ -180,8 +191,9 @@ def cpu_baseline_mixing():
 def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_256x128x128"):
     """The reacting mixing layer: FP64 plain beside FP32 + surrogate."""
     from lbm_amd import cases
-    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, MixtureLattice
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_STRICT, MixtureLattice
     cells = nx * ny * nz
+    mode = LBM_MODE_FUSED if args.mode == "fused" else LBM_MODE_STRICT
     fl = cases.mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
     F = (fl["type"] == 0).ravel()
     cap_steps, cap_stride = 4, 4          # ~4.2 M training states
@@ -193,7 +205,7 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
         return lat.get_field("T").ravel()[F], rho_a / rho_a.sum(axis=0, keepdims=True)
 
     # FP32 shifted populations + learned chemistry
-    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED)
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED, mode=mode)
     lat.load_case(fl)
     lat.Init()
     lat.calculate_moment()
@@ -239,17 +251,26 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
                     "kernel": dom, "bytes_per_launch": bpl * cells, "avg_ms": round(kern[dom]["avg_ms"], 4)}
 
     if not args.no_compare:
-        ref = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP64)
-        ref.load_case(fl)
-        ref.Init()
-        ref.calculate_moment()
-        sec64, kern64 = timed(ref, steps, args.warmup + cap_steps, MULTI_KERNELS)
-        T64, Y64 = fields(ref)
-        ref.close()
-        mlups64 = cells * steps / sec64 / 1e6
+        def fp64_run(m):
+            ref = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP64, mode=m)
+            ref.load_case(fl)
+            ref.Init()
+            ref.calculate_moment()
+            sec64, kern64 = timed(ref, steps, args.warmup + cap_steps, MULTI_KERNELS)
+            T64, Y64 = fields(ref)
+            ref.close()
+            return cells * steps / sec64 / 1e6, kern64, T64, Y64
+
+        # FP64 plain chemistry in the same mode; in the fused mode also the
+        # strict calls (the two modes give bitwise equal states)
+        mlups64, kern64, T64, Y64 = fp64_run(mode)
         extra["fp64_mlups"] = round(mlups64, 2)
         extra["speedup_fp32_surrogate_vs_fp64"] = round(mlups / mlups64, 3)
         extra["fp64_kernels"] = {n: {"avg_ms": round(v["avg_ms"], 4)} for n, v in kern64.items()}
+        if mode == LBM_MODE_FUSED:
+            mlups64s, kern64s, _, _ = fp64_run(LBM_MODE_STRICT)
+            extra["fp64_strict_mlups"] = round(mlups64s, 2)
+            extra["speedup_fp32_surrogate_vs_fp64_strict"] = round(mlups / mlups64s, 3)
         extra["error_vs_fp64"] = {
             "flame_T_rel": float(f"{abs(T32.max() - T64.max()) / T64.max():.3e}"),
             "T_max_rel": float(f"{np.max(np.abs(T32 - T64) / T64):.3e}"),
@@ -261,7 +282,7 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
         "warmup": args.warmup, "ms_per_step": round(sec / steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": tag + "_9sp_fp32_shifted_" + args.chemistry, "cells": cells,
-                   "grid": [nx, ny, nz], "mode": "strict", "parallelism": "none"},
+                   "grid": [nx, ny, nz], "mode": args.mode, "parallelism": "none"},
         "roofline": roofline, "cpu_baseline": None, "extra": extra, "host": platform.node(),
     }
     return rec

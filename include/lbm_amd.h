@@ -46,9 +46,16 @@ enum lbm_precision {
     LBM_FP32_SHIFTED = 1   /* FP32 deviations f - f_ref from the rest state */
 };
 
-/* How the five step-calls map onto kernels. */
+/* How the five step-calls map onto kernels.  Single component, fused: the
+ * pull-stream runs inside the collision and lbm_stream is a no-op (periodic,
+ * TYPE_A and TYPE_FS neighbours only).  With species, fused: Collide_Species
+ * writes each species population to the slot Streaming would move it to,
+ * lbm_stream swaps the species buffers and streams only the energy
+ * populations; every state is bitwise that of the strict calls (TYPE_FS and
+ * TYPE_Q neighbours need LBM_MODE_STRICT), and the post-collision species set
+ * (LBM_POP_FPC) is rebuilt when it is read. */
 enum lbm_mode {
-    LBM_MODE_FUSED = 0,   /* pull-stream fused into collide; lbm_stream is a no-op */
+    LBM_MODE_FUSED = 0,
     LBM_MODE_STRICT = 1   /* one kernel per reference call, explicit f and fpc */
 };
 

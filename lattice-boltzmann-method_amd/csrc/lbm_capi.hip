@@ -85,6 +85,18 @@ struct lbm_ctx {
     bool conjugate = false;
     int* cjc_cell = nullptr;
     int cjc_count = 0;
+    // multicomponent fused mode: Collide_Species writes each species
+    // population to its streamed slot (buffer 1) and Streaming swaps the
+    // species buffers and streams only the energy populations.  Cells without
+    // NB_PUSH keep their post-collision species sets in the side store.  The
+    // first step after Init or an upload runs strictly (push_ready false), so
+    // that the streamed populations and the side store hold the post-collision
+    // values a low species keeps.
+    int* side_slot = nullptr;  // [n] side slot of a cell, -1 for none
+    int* side_cell = nullptr;  // [nside]
+    void* fside = nullptr;     // [NS][27][nside]
+    int64_t nside = 0;
+    bool push_ready = false;
     void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
@@ -226,6 +238,10 @@ MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
     v.sur_count = c->sur_count;
     v.audit_stride = c->audit_stride;
     v.audit_tol = c->audit_tol;
+    v.side_slot = c->side_slot;
+    v.side_cell = c->side_cell;
+    v.fside = static_cast<P*>(c->fside);
+    v.nside = c->nside;
     return v;
 }
 
@@ -384,7 +400,8 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell, c->cjc_cell};
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell, c->cjc_cell,
+                    c->side_slot, c->side_cell, c->fside};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->net_ready) sur::mlp_free(c->net);
@@ -431,10 +448,6 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return LBM_E_UNSUPPORTED;
     }
     if (cfg->n_species == mech::NS) {
-        if (cfg->mode != LBM_MODE_STRICT) {
-            g_create_error = "the multicomponent path runs in LBM_MODE_STRICT";
-            return LBM_E_UNSUPPORTED;
-        }
         if (!(cfg->dx_si > 0.0) || !(cfg->dt_si > 0.0)) {
             g_create_error = "multicomponent state needs dx_si > 0 and dt_si > 0 (units.set_m_kg_s)";
             return LBM_E_ARG;
@@ -555,6 +568,61 @@ void lbm_destroy(lbm_ctx* ctx)
     delete ctx;
 }
 
+namespace {
+
+// The fused mode's plan on the host.  A fluid cell gets NB_PUSH when every
+// population it pulls comes from a plain move (upstream neither a wall nor
+// an in/outflow ghost) and every population it sends lands in a fluid cell,
+// by a plain move, in a cell TMS_BC does not rewrite; the other fluid cells
+// go to the side list.  Free-slip and TYPE_Q neighbours, and periodic
+// images that are not fluid, are refused.
+int push_plan(lbm_ctx* c, const uint8_t* types, const std::vector<int>& tcell, const std::vector<int>& icell,
+              const std::vector<int>& wcell, std::vector<uint8_t>& flag, std::vector<int>& side)
+{
+    const Dims& d = c->d;
+    std::vector<uint8_t> tms(d.n, 0);
+    for (const auto* v : {&tcell, &icell, &wcell})
+        for (int x : *v) tms[x] = 1;
+    auto at = [&](int i, int j, int k) { return ((int64_t)i * d.ny + j) * d.nz + k; };
+    auto wrap = [](int i, int n) { return i == 0 ? n - 2 : (i == n - 1 ? 1 : i); };
+    auto pullable = [](uint8_t t) {
+        return !(t == TYPE_S || t == TYPE_A || t == TYPE_FS || t == TYPE_O || t == TYPE_O_C || t == TYPE_I ||
+                 t == TYPE_I_C);
+    };
+    side.clear();
+    for (int i = 1; i < d.nx - 1; ++i)
+        for (int j = 1; j < d.ny - 1; ++j)
+            for (int k = 1; k < d.nz - 1; ++k) {
+                const int64_t cidx = at(i, j, k);
+                if (types[cidx] != TYPE_F) continue;
+                bool push = !tms[cidx];
+                for (int l = 0; l < Q; ++l) {
+                    const uint8_t tu = types[at(i - cx(l), j - cy(l), k - cz(l))];
+                    if (tu == TYPE_FS || tu == TYPE_Q)
+                        return fail(c, LBM_E_UNSUPPORTED,
+                                    "the multicomponent fused mode handles periodic, wall (TYPE_S, TYPE_A) and "
+                                    "in/outflow neighbours; TYPE_FS/Q neighbours need LBM_MODE_STRICT");
+                    if (pullable(tu) &&
+                        types[at(wrap(i - cx(l), d.nx), wrap(j - cy(l), d.ny), wrap(k - cz(l), d.nz))] != TYPE_F)
+                        return fail(c, LBM_E_UNSUPPORTED,
+                                    "a periodic image of a fluid cell's neighbour is not fluid; the fused mode "
+                                    "needs LBM_MODE_STRICT here");
+                    if (!pullable(tu)) push = false;
+                    const uint8_t tt = types[at(i + cx(l), j + cy(l), k + cz(l))];
+                    const int di = wrap(i + cx(l), d.nx), dj = wrap(j + cy(l), d.ny), dk = wrap(k + cz(l), d.nz);
+                    const int64_t dc = at(di, dj, dk);
+                    if (tt == TYPE_S || tt == TYPE_A || types[dc] != TYPE_F || tms[dc] ||
+                        !pullable(types[at(di - cx(l), dj - cy(l), dk - cz(l))]))
+                        push = false;
+                }
+                if (push) flag[cidx] |= NB_PUSH;
+                else side.push_back((int)cidx);
+            }
+    return LBM_OK;
+}
+
+}  // namespace
+
 int lbm_set_types(lbm_ctx* c, const uint8_t* types)
 {
     if (!c || !types) return fail(c, LBM_E_ARG, "null argument");
@@ -661,7 +729,10 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                         }
                     }
                 }
-    if (fused(c) && (needs_tms || !fused_ok))
+    std::vector<int> side;
+    if (multi(c) && fused(c))
+        if (int rc = push_plan(c, types, tcell, icell, wcell, flag, side)) return rc;
+    if (!multi(c) && fused(c) && (needs_tms || !fused_ok))
         return fail(c, LBM_E_UNSUPPORTED,
                     "fused mode handles periodic, adiabatic (TYPE_A) and free-slip (TYPE_FS) neighbours; "
                     "TYPE_S/I/O/I_C/O_C/Q neighbours need LBM_MODE_STRICT");
@@ -699,6 +770,24 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
         }
         c->tms_unsupported = tms_unsupported;
         needs_tms = false;
+        for (void** p : {(void**)&c->side_slot, (void**)&c->side_cell, &c->fside}) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+        c->nside = (int64_t)side.size();
+        c->push_ready = false;
+        if (fused(c)) {
+            std::vector<int> slot(d.n, -1);
+            for (size_t s = 0; s < side.size(); ++s) slot[side[s]] = (int)s;
+            HIP_TRY(c, hipMalloc((void**)&c->side_slot, sizeof(int) * d.n));
+            HIP_TRY(c, hipMemcpy(c->side_slot, slot.data(), sizeof(int) * d.n, hipMemcpyHostToDevice));
+            if (c->nside) {
+                const size_t sb = (size_t)c->nside * Q * c->esize;
+                HIP_TRY(c, hipMalloc((void**)&c->side_cell, sizeof(int) * side.size()));
+                HIP_TRY(c, hipMemcpy(c->side_cell, side.data(), sizeof(int) * side.size(), hipMemcpyHostToDevice));
+                HIP_TRY(c, hipMalloc(&c->fside, sb * c->ns));
+            }
+        }
     }
     if (!multi(c)) {
         if (c->sbc_cell) (void)hipFree(c->sbc_cell);
@@ -793,6 +882,7 @@ int lbm_set_populations(lbm_ctx* c, int which, const double* host)
             HIP_TRY(c, hipMemcpyAsync(dev, host, count * 8, hipMemcpyHostToDevice, c->stream));
         }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->push_ready = false;
         return LBM_OK;
     }
     if (fused(c)) {
@@ -838,6 +928,10 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
     if (multi(c)) {
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
+        // the fused mode keeps no post-collision species buffer: rebuild it
+        // from the streamed populations and the side store
+        if (which == LBM_POP_FPC && fused(c) && c->push_ready)
+            if (int rc = run_mc(c, McKernel::Unstream, "mc_unstream", 0, 1)) return rc;
         const void* dev = sp ? c->sf[slot] : c->pg[slot];
         const size_t count = (size_t)c->d.n * Q * (sp ? c->ns : 1);
         if (fp32(c)) {
@@ -942,6 +1036,7 @@ int lbm_init(lbm_ctx* c)
             if (int rc = set_multi_reference_state(c)) return rc;
         if (int rc = run_mc(c, McKernel::Init, "mc_init", 0, 0)) return rc;
         c->fields_valid = true;
+        c->push_ready = false;
         return LBM_OK;
     }
     if (int rc = materialize(c)) return rc;
@@ -979,7 +1074,9 @@ int lbm_collide_species(lbm_ctx* c)
         }
         if (rc) return rc;
     }
-    const int rc = run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+    const int rc = fused(c) && c->push_ready
+                       ? run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1)
+                       : run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
     ++c->step;
     return rc;
 }
@@ -1010,7 +1107,17 @@ int lbm_stream(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
-    if (multi(c)) return run_mc(c, McKernel::Stream, "mc_stream", 1, 0);
+    if (multi(c)) {
+        if (fused(c) && c->push_ready) {  // Collide_Species wrote the streamed species slots of buffer 1
+            std::swap(c->sf[0], c->sf[1]);
+            return run_mc(c, McKernel::StreamEnergy, "mc_stream_energy", 1, 0);
+        }
+        if (int rc = run_mc(c, McKernel::Stream, "mc_stream", 1, 0)) return rc;
+        if (!fused(c)) return LBM_OK;
+        if (int rc = run_mc(c, McKernel::SideFill, "mc_side_fill", 0, 1)) return rc;
+        c->push_ready = true;
+        return LBM_OK;
+    }
     if (fused(c)) return LBM_OK;  // streaming happens in the next pull
     return run_stream(c, 1, 0);
 }

@@ -33,7 +33,9 @@ inline unsigned xcd_grid(int64_t n, int block)
 // nbflag bits, computed once per lbm_set_types.
 enum : uint8_t {
     NB_SIMPLE = 1,  // TYPE_F cell whose 26 neighbours are all TYPE_F or TYPE_P
-    NB_KEEP = 2     // TYPE_F cell with an S/O/O_C/I/I_C neighbour (stream keeps stale values)
+    NB_KEEP = 2,    // TYPE_F cell with an S/O/O_C/I/I_C neighbour (stream keeps stale values)
+    NB_PUSH = 4     // multicomponent fused mode: every population of the cell streams to a
+                    // fluid cell that TMS_BC leaves alone, and it pulls only from such moves
 };
 
 // Device view of the single-component state; P is the population storage and

@@ -41,6 +41,12 @@ struct MultiView {
     unsigned long long* sur_count; // SUR_* counters
     int audit_stride;              // one 32-cell chunk in audit_stride is audited
     float audit_tol;               // audit tolerance in target standard deviations
+    // fused mode: the cells without NB_PUSH keep their post-collision species
+    // sets here, side slot s of cell side_cell[s] (side_slot[c] = s, else -1)
+    const int* side_slot;  // [n]
+    const int* side_cell;  // [nside]
+    P* fside;              // [NS][27][nside]
+    int64_t nside;
 };
 
 // counters of the surrogate path (cumulative)
@@ -49,7 +55,14 @@ enum { SUR_PREDICTED = 0, SUR_FALLBACK = 1, SUR_AUDITED = 2, SUR_AUDIT_FAIL = 3,
 // bit 31 of a chemistry-list entry marks an audit (else a fallback)
 constexpr unsigned CHEM_AUDIT = 0x80000000u;
 
-enum class McKernel { Init, Chemistry, CollideSpecies, Collide, Stream, Moments };
+// CollideSpeciesPush: the fused mode's Collide_Species, each post-collision
+// species population written to the slot the next Streaming would move it to
+// (buffer fo), so that only the energy populations are streamed
+// (StreamEnergy); SideFill copies the post-collision species sets of the side
+// cells out of fo after a strict step; Unstream rebuilds the post-collision
+// species sets in fo from the streamed f.
+enum class McKernel { Init, Chemistry, CollideSpecies, Collide, Stream, Moments, CollideSpeciesPush, StreamEnergy,
+                      SideFill, Unstream };
 
 template <typename P>
 void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);

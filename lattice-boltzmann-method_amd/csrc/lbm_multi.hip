@@ -104,6 +104,50 @@ __device__ inline void feq_axes(Axis3<T> F[3], const T u[3], T theta, const T co
     for (int d = 0; d < 3; ++d) F[d] = psi_axis<T>(u[d], theta + u[d] * u[d] + corr[d]);
 }
 
+// ------------------------------------------------- fused mode (push)
+// Species streaming (lbm-streaming.cpp:3-119) read backwards.  The pull of
+// cell d in direction l takes the post-collision population (wrap(d - e_l),
+// l) when the upstream cell d - e_l is not a wall or an in/outflow ghost, the
+// bounce-back (d, opp(l)) when it is TYPE_S or TYPE_A, and nothing when it is
+// an in/outflow ghost: that slot keeps its value.  Free-slip neighbours are
+// not handled (the fused mode refuses them).
+__device__ inline bool pullable(uint8_t t)
+{
+    return !(t == TYPE_S || t == TYPE_A || t == TYPE_FS || t == TYPE_O || t == TYPE_O_C || t == TYPE_I ||
+             t == TYPE_I_C);
+}
+__device__ inline int wrap1(int i, int n) { return i == 0 ? n - 2 : (i == n - 1 ? 1 : i); }
+
+// Writes the post-collision value pv of (c, l) of one species (A, B: the
+// species' planes in the source and destination buffers) to every slot the
+// pull would fill from it, and keeps the slot (c, l) of a cell whose
+// upstream is an in/outflow ghost as the pull does.
+template <typename P>
+__device__ inline void push_general(const MultiView<P>& v, const P* A, P* B, int c, const Ijk& x, int l, P pv)
+{
+    const int64_t n = v.d.n;
+    const uint8_t tt = v.type[cell(v.d, x.i + cx(l), x.j + cy(l), x.k + cz(l))];
+    if (tt == TYPE_A || tt == TYPE_S) B[(int64_t)opp(l) * n + c] = pv;
+    const int di = wrap1(x.i + cx(l), v.d.nx), dj = wrap1(x.j + cy(l), v.d.ny), dk = wrap1(x.k + cz(l), v.d.nz);
+    const int dc = cell(v.d, di, dj, dk);
+    if (v.type[dc] == TYPE_F && pullable(v.type[cell(v.d, di - cx(l), dj - cy(l), dk - cz(l))]))
+        B[(int64_t)l * n + dc] = pv;
+    const uint8_t tu = v.type[cell(v.d, x.i - cx(l), x.j - cy(l), x.k - cz(l))];
+    if (tu == TYPE_O || tu == TYPE_O_C || tu == TYPE_I || tu == TYPE_I_C) B[(int64_t)l * n + c] = A[(int64_t)l * n + c];
+}
+
+// Post-collision species population in storage form (shared by the strict
+// and the fused collisions, so that both round alike).
+template <typename P>
+__device__ __forceinline__ P species_post(const MultiView<P>& v, int a, int l, P fl, double om, double fe, double fdu,
+                                          double fr)
+{
+    const double f = (double)fl + fshift(v, a, l);
+    double x = f + om * (fe - f) + (fdu - fe) + fr;
+    if constexpr (sizeof(P) == 4) x -= fshift(v, a, l);
+    return (P)x;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ Init
@@ -600,8 +644,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_species_transport(MultiView<P> v)
         for (int d = 0; d < 3; ++d) v.tsp[((1 + d) * NS + a) * n + c] = P(acc[a][d] * 1.0 / ra[a]);
 }
 
-// grid: (xcd_grid(n), NS); species a = blockIdx.y
-template <typename P>
+// grid: (xcd_grid(n), NS); species a = blockIdx.y.  PUSH: the fused mode's
+// form, each post-collision population written to its streamed slot.
+template <typename P, bool PUSH>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
 {
     const int c = xcd_block() * BLOCK + threadIdx.x;
@@ -651,14 +696,46 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
     feq_axes<double>(Fu, udu, theta, corr);
     feq_axes<double>(Fr, umix, theta, zero);
     const bool low = ra <= SPECIES_MIN;
+    if constexpr (!PUSH) {
 #pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const double fe = feq_dir<double>(Fe, l, ra);
-        const double fdu = feq_dir<double>(Fu, l, ra);
-        const double fr = feq_dir<double>(Fr, l, src);
-        if (low && fr <= 0.0) continue;  // post-collision value left as it was
-        const double f = (double)fl[l] + fshift(v, a, l);
-        stf(v, v.fo, a, l, c, f + om * (fe - f) + (fdu - fe) + fr);
+        for (int l = 0; l < Q; ++l) {
+            const double fe = feq_dir<double>(Fe, l, ra);
+            const double fdu = feq_dir<double>(Fu, l, ra);
+            const double fr = feq_dir<double>(Fr, l, src);
+            if (low && fr <= 0.0) continue;  // post-collision value left as it was
+            spop(v.fo, a, l, n, c) = species_post(v, a, l, fl[l], om, fe, fdu, fr);
+        }
+    } else {
+        // A cell with NB_PUSH streams every population to a fluid cell that
+        // TMS_BC leaves alone: the post-collision value a low species keeps
+        // is the one streamed there last step.  The other cells collide into
+        // the side store as the strict form does into fo; k_mc_side_push
+        // then streams them.
+        if (!(v.nbflag[c] & NB_PUSH)) {
+            P* side = v.fside + (int64_t)a * Q * v.nside + v.side_slot[c];
+#pragma unroll
+            for (int l = 0; l < Q; ++l) {
+                const double fe = feq_dir<double>(Fe, l, ra);
+                const double fdu = feq_dir<double>(Fu, l, ra);
+                const double fr = feq_dir<double>(Fr, l, src);
+                if (low && fr <= 0.0) continue;
+                side[(int64_t)l * v.nside] = species_post(v, a, l, fl[l], om, fe, fdu, fr);
+            }
+            return;
+        }
+        const P* A = v.f + (int64_t)a * Q * n;
+        P* B = v.fo + (int64_t)a * Q * n;
+        const int ti[3] = {wrap1(x.i - 1, v.d.nx), x.i, wrap1(x.i + 1, v.d.nx)};
+        const int tj[3] = {wrap1(x.j - 1, v.d.ny), x.j, wrap1(x.j + 1, v.d.ny)};
+        const int tk[3] = {wrap1(x.k - 1, v.d.nz), x.k, wrap1(x.k + 1, v.d.nz)};
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const double fe = feq_dir<double>(Fe, l, ra);
+            const double fdu = feq_dir<double>(Fu, l, ra);
+            const double fr = feq_dir<double>(Fr, l, src);
+            const int64_t dst = (int64_t)l * n + cell(v.d, ti[cx(l) + 1], tj[cy(l) + 1], tk[cz(l) + 1]);
+            B[dst] = (low && fr <= 0.0) ? A[dst] : species_post(v, a, l, fl[l], om, fe, fdu, fr);
+        }
     }
 }
 
@@ -784,12 +861,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
 // rest-state shifts of FP32 storage are the same for opposite directions, so
 // stored values move unchanged.
 template <typename P>
-__global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
+__global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v, int first)
 {
-    // one thread per (cell, population set): blockIdx.y = species 0..NS-1, NS = g
+    // one thread per (cell, population set): first + blockIdx.y = species
+    // 0..NS-1, NS = g
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
-    const int set = blockIdx.y;
+    const int set = first + blockIdx.y;
     const int64_t n = v.d.n;
     const Ijk x = ijk(v.d, c);
     const int nx = v.d.nx, ny = v.d.ny, nz = v.d.nz;
@@ -839,6 +917,67 @@ __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v)
             dst_base[(int64_t)l * n + c] = src_base[(int64_t)l * n + cell(v.d, in, jn, kn)];
         }
     }
+}
+
+// Fused mode: the post-collision species sets of the side cells, copied out
+// of the strict step's fo.  grid: (ceil(nside / BLOCK), NS).
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_side_fill(MultiView<P> v)
+{
+    const int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= v.nside) return;
+    const int c = v.side_cell[s];
+    const int a = blockIdx.y;
+    const int64_t n = v.d.n;
+    const P* src = v.fo + (int64_t)a * Q * n;
+    P* dst = v.fside + (int64_t)a * Q * v.nside;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) dst[(int64_t)l * v.nside + s] = src[(int64_t)l * n + c];
+}
+
+// Fused mode: the side cells' post-collision species sets streamed from the
+// side store into fo (push_general).  grid: (ceil(nside / BLOCK), NS).
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_side_push(MultiView<P> v)
+{
+    const int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= v.nside) return;
+    const int c = v.side_cell[s];
+    const int a = blockIdx.y;
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    const P* S = v.fside + (int64_t)a * Q * v.nside;
+    const P* A = v.f + (int64_t)a * Q * n;
+    P* B = v.fo + (int64_t)a * Q * n;
+    for (int l = 0; l < Q; ++l) push_general<P>(v, A, B, c, x, l, S[(int64_t)l * v.nside + s]);
+}
+
+// Fused mode: the post-collision species sets rebuilt into fo from the
+// streamed f (NB_PUSH cells) and the side store (the others).  grid:
+// (xcd_grid(n), NS).
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_unstream(MultiView<P> v)
+{
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int a = blockIdx.y;
+    const int64_t n = v.d.n;
+    const P* A = v.f + (int64_t)a * Q * n;
+    P* B = v.fo + (int64_t)a * Q * n;
+    if (v.nbflag[c] & NB_PUSH) {
+        const Ijk x = ijk(v.d, c);
+        const int ti[3] = {wrap1(x.i - 1, v.d.nx), x.i, wrap1(x.i + 1, v.d.nx)};
+        const int tj[3] = {wrap1(x.j - 1, v.d.ny), x.j, wrap1(x.j + 1, v.d.ny)};
+        const int tk[3] = {wrap1(x.k - 1, v.d.nz), x.k, wrap1(x.k + 1, v.d.nz)};
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            B[(int64_t)l * n + c] = A[(int64_t)l * n + cell(v.d, ti[cx(l) + 1], tj[cy(l) + 1], tk[cz(l) + 1])];
+        return;
+    }
+    const int s = v.side_slot[c];
+    const P* S = v.fside + (int64_t)a * Q * v.nside;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) B[(int64_t)l * n + c] = S[(int64_t)l * v.nside + s];
 }
 
 // ------------------------------------------------------- calculate_moment
@@ -1914,11 +2053,29 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
     case McKernel::Chemistry: hipLaunchKernelGGL(k_mc_chemistry<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     case McKernel::CollideSpecies:
         hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
-        hipLaunchKernelGGL(k_mc_collide_species<P>, dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_mc_collide_species<P, false>), dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+        break;
+    case McKernel::CollideSpeciesPush:
+        hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_mc_collide_species<P, true>), dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+        if (v.nside > 0)
+            hipLaunchKernelGGL(k_mc_side_push<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
+                               s, v);
         break;
     case McKernel::Collide: hipLaunchKernelGGL(k_mc_collide<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::StreamEnergy:
+        hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), 1), dim3(BLOCK), 0, s, v, NS);
+        break;
+    case McKernel::SideFill:
+        if (v.nside > 0)
+            hipLaunchKernelGGL(k_mc_side_fill<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
+                               s, v);
+        break;
+    case McKernel::Unstream:
+        hipLaunchKernelGGL(k_mc_unstream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+        break;
     case McKernel::Stream:
-        hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS + 1), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS + 1), dim3(BLOCK), 0, s, v, 0);
         break;
     case McKernel::Moments:
         hipLaunchKernelGGL(k_mc_moments<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);

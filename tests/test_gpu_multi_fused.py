@@ -1,0 +1,130 @@
+"""The multicomponent fused mode (LBM_MODE_FUSED) against the strict mode,
+through the C ABI.
+
+In the fused mode Collide_Species writes every post-collision species
+population straight into the slot the next Streaming would move it to, and
+Streaming only swaps the species buffers and streams the energy populations.
+It computes the same numbers in the same order as the strict calls (which the
+oracle pins at 1e-10 in test_gpu_multi.py), so the bar here is bitwise
+equality of every fluid cell's populations (f, g and the post-collision sets,
+rebuilt on download), fields and species fields after each step.  The cases
+cover the periodic and NSCBC faces of the mixing layer, the LODI inflow and
+NSCBC outflow of the premixed flame, the fixed-state inflow, outflow and
+isothermal walls of the channel, adiabatic walls, species held at the
+SPECIES_MIN floor (the post-collision value a low species keeps,
+lbm-collision.cpp:512-513), FP32 storage, and a restart in the middle.
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import cases
+from lbm_amd.capi import (LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_STRICT, POP_F, POP_FPC, POP_G,
+                          POP_GPC, LbmError, MixtureLattice)
+from lbm_amd.cases import TYPE_A, TYPE_FS
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "T", "p", "rhoe", "HRR")
+SPECIES_FIELDS = ("X", "rho", "u", "v", "w")
+
+
+def make(fl, mode, precision=LBM_FP64, reaction=True):
+    nx, ny, nz = (s - 2 for s in fl["type"].shape)
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=reaction, mode=mode, precision=precision)
+    lat.load_case(fl)
+    lat.Init()
+    lat.calculate_moment()
+    return lat
+
+
+def snapshot(lat, F):
+    ns = len(lat.SPECIES)
+    out = {}
+    for k in FIELDS:
+        out[k] = lat.get_field(k).ravel()[F]
+    for k in SPECIES_FIELDS:
+        out["sp_" + k] = lat.get_species_field(k).reshape(ns, -1)[:, F]
+    for which, nm in ((POP_F, "f"), (POP_FPC, "fpc")):
+        out[nm] = lat.get_populations(which).reshape(ns, 27, -1)[:, :, F]
+    for which, nm in ((POP_G, "g"), (POP_GPC, "gpc")):
+        out[nm] = lat.get_populations(which).reshape(27, -1)[:, F]
+    return out
+
+
+def assert_same(a, b, where):
+    for k in a:
+        if not np.array_equal(a[k], b[k], equal_nan=True):
+            d = np.abs(a[k] - b[k])
+            raise AssertionError(f"{where}: {k} differs (max |diff| {np.nanmax(d):.3e}, "
+                                 f"{int(np.count_nonzero(d))} values)")
+
+
+def compare(fl, steps, precision=LBM_FP64, reaction=True, tag=""):
+    print(f"fused vs strict {tag} grid {fl['type'].shape} steps {steps} precision {precision}", flush=True)
+    F = fl["type"].ravel() == 0
+    strict = make(fl, LBM_MODE_STRICT, precision, reaction)
+    fused = make(fl, LBM_MODE_FUSED, precision, reaction)
+    for step in range(1, steps + 1):
+        strict.step(1)
+        fused.step(1)
+        assert_same(snapshot(fused, F), snapshot(strict, F), f"{tag} step {step}")
+    strict.close()
+    fused.close()
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_fused_mixing_layer_matches_strict(reaction):
+    compare(cases.mixing_layer(6, 12, 5, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=3.0), 5, reaction=reaction,
+            tag="mixing layer")
+
+
+def test_fused_flame_matches_strict():
+    compare(cases.premixed_flame_1d(24, 3, 3), 5, tag="flame")
+
+
+def test_fused_channel_walls_matches_strict():
+    compare(cases.reacting_channel(8, 5, 3, p_out_factor=0.995), 5, tag="channel")
+
+
+def test_fused_adiabatic_walls_and_floor_species_match_strict():
+    """TYPE_A walls on the y faces and species at the SPECIES_MIN floor: the
+    collision leaves their populations unwritten (no chemistry source), so
+    they keep the value streamed into them before."""
+    fl = cases.reacting_box(6, 5, 4, T_K=1300.0, amp_T=0.1, amp_u=0.01,
+                            X=dict(cases.X_UNBURNT_PHI1, H2O=1e-3))
+    fl["type"][:, 0, :] = TYPE_A
+    fl["type"][:, -1, :] = TYPE_A
+    compare(fl, 5, reaction=False, tag="adiabatic walls")
+
+
+def test_fused_fp32_mixing_layer_matches_strict():
+    compare(cases.mixing_layer(6, 12, 5, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=3.0), 4,
+            precision=LBM_FP32_SHIFTED, tag="mixing layer fp32")
+
+
+def test_fused_restart_in_the_middle(tmp_path):
+    """Three fused steps, a restart file (its post-collision records rebuilt
+    from the streamed populations), a fresh fused context resumed from it and
+    three more steps: the same state as six strict steps."""
+    fl = cases.premixed_flame_1d(24, 3, 3)
+    F = fl["type"].ravel() == 0
+    strict = make(fl, LBM_MODE_STRICT)
+    strict.step(6)
+    fused = make(fl, LBM_MODE_FUSED)
+    fused.step(3)
+    path = fused.write_restart(3, str(tmp_path))
+    fused.close()
+    nx, ny, nz = (s - 2 for s in fl["type"].shape)
+    resumed = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, mode=LBM_MODE_FUSED)
+    assert resumed.read_restart(path) == 3
+    resumed.step(3)
+    assert_same(snapshot(resumed, F), snapshot(strict, F), "resumed")
+
+
+def test_fused_refuses_free_slip():
+    fl = cases.reacting_box(4, 4, 4, T_K=1300.0)
+    fl["type"][:, 0, :] = TYPE_FS
+    nx, ny, nz = (s - 2 for s in fl["type"].shape)
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=False, mode=LBM_MODE_FUSED)
+    with pytest.raises(LbmError, match="TYPE_FS"):
+        lat.load_case(fl)
