@@ -96,8 +96,8 @@ def make_lattice(fl, pr, precision, mode):
 
 
 SINGLE_KERNELS = ("fused_collide_stream", "fused_moments", "collide_explicit")
-MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_fallback", "mc_collide_species",
-                 "mc_collide_species_push", "mc_collide_mixture", "mc_stream", "mc_stream_energy", "mc_tms_outflow",
+MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_fallback", "mc_species_transport",
+                 "mc_collide_species", "mc_collide_species_push", "mc_collide_mixture", "mc_stream", "mc_stream_energy", "mc_tms_outflow",
                  "mc_tms_inflow", "mc_tms_wall", "mc_moments")
 
 

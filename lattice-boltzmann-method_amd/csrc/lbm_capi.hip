@@ -1074,9 +1074,10 @@ int lbm_collide_species(lbm_ctx* c)
         }
         if (rc) return rc;
     }
-    const int rc = fused(c) && c->push_ready
-                       ? run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1)
-                       : run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+    int rc = run_mc(c, McKernel::SpeciesTransport, "mc_species_transport", 0, 1);
+    if (!rc)
+        rc = fused(c) && c->push_ready ? run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1)
+                                       : run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
     ++c->step;
     return rc;
 }

@@ -61,8 +61,10 @@ constexpr unsigned CHEM_AUDIT = 0x80000000u;
 // (StreamEnergy); SideFill copies the post-collision species sets of the side
 // cells out of fo after a strict step; Unstream rebuilds the post-collision
 // species sets in fo from the streamed f.
-enum class McKernel { Init, Chemistry, CollideSpecies, Collide, Stream, Moments, CollideSpeciesPush, StreamEnergy,
-                      SideFill, Unstream };
+// SpeciesTransport runs first in Collide_Species (per cell: omega_a and the
+// Maxwell-Stefan friction), then CollideSpecies or CollideSpeciesPush.
+enum class McKernel { Init, Chemistry, SpeciesTransport, CollideSpecies, Collide, Stream, Moments, CollideSpeciesPush,
+                      StreamEnergy, SideFill, Unstream };
 
 template <typename P>
 void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);

@@ -2051,12 +2051,13 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
     switch (k) {
     case McKernel::Init: hipLaunchKernelGGL(k_mc_init<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
     case McKernel::Chemistry: hipLaunchKernelGGL(k_mc_chemistry<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
-    case McKernel::CollideSpecies:
+    case McKernel::SpeciesTransport:
         hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
+        break;
+    case McKernel::CollideSpecies:
         hipLaunchKernelGGL((k_mc_collide_species<P, false>), dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
         break;
     case McKernel::CollideSpeciesPush:
-        hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
         hipLaunchKernelGGL((k_mc_collide_species<P, true>), dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
         if (v.nside > 0)
             hipLaunchKernelGGL(k_mc_side_push<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
