@@ -97,6 +97,11 @@ struct lbm_ctx {
     void* fside = nullptr;     // [NS][27][nside]
     int64_t nside = 0;
     bool push_ready = false;
+    // Collide_Species has pushed this step's post-collision species sets into
+    // their streamed slots of buffer 1 and Streaming has not run yet: buffer 0
+    // still holds the pre-collision f, the post-collision set lives at the
+    // downstream slots of buffer 1 (and in the side store)
+    bool pushed = false;
     void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
@@ -776,6 +781,7 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
         }
         c->nside = (int64_t)side.size();
         c->push_ready = false;
+        c->pushed = false;
         if (fused(c)) {
             std::vector<int> slot(d.n, -1);
             for (size_t s = 0; s < side.size(); ++s) slot[side[s]] = (int)s;
@@ -859,6 +865,55 @@ int pop_slot(lbm_ctx* c, int which, bool for_write, void** buf, const double** r
     return slot;
 }
 
+// The post-collision species sets between a fused-mode Collide_Species and
+// Streaming, gathered on the host: cell c's population l of a NB_PUSH cell
+// sits in buffer 1 at its streamed slot wrap(c + e_l) (the value a low
+// species keeps included), the side cells' sets in the side store.  Other
+// cells keep what buffer 1 holds there.
+int read_pushed_fpc(lbm_ctx* c, double* host)
+{
+    const Dims& d = c->d;
+    const int64_t n = d.n;
+    const size_t count = (size_t)n * Q * c->ns;
+    std::vector<uint8_t> type(n), flag(n);
+    std::vector<int> side_cell(c->nside);
+    std::vector<double> B(count), S((size_t)c->nside * Q * c->ns);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(type.data(), c->type, n, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(flag.data(), c->nbflag, n, hipMemcpyDeviceToHost));
+    if (c->nside) HIP_TRY(c, hipMemcpy(side_cell.data(), c->side_cell, sizeof(int) * c->nside, hipMemcpyDeviceToHost));
+    const size_t nsz = (size_t)c->nside * Q * c->ns;
+    if (fp32(c)) {
+        std::vector<float> tb(count), ts(nsz);
+        HIP_TRY(c, hipMemcpy(tb.data(), c->sf[1], count * 4, hipMemcpyDeviceToHost));
+        if (nsz) HIP_TRY(c, hipMemcpy(ts.data(), c->fside, nsz * 4, hipMemcpyDeviceToHost));
+        for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl) {
+            for (int64_t i = 0; i < n; ++i) B[pl * n + i] = (double)tb[pl * n + i] + c->mfref[pl];
+            for (int64_t s = 0; s < c->nside; ++s) S[pl * c->nside + s] = (double)ts[pl * c->nside + s] + c->mfref[pl];
+        }
+    } else {
+        HIP_TRY(c, hipMemcpy(B.data(), c->sf[1], count * 8, hipMemcpyDeviceToHost));
+        if (nsz) HIP_TRY(c, hipMemcpy(S.data(), c->fside, nsz * 8, hipMemcpyDeviceToHost));
+    }
+    std::copy(B.begin(), B.end(), host);
+    auto wrap = [](int i, int m) { return i == 0 ? m - 2 : (i == m - 1 ? 1 : i); };
+    for (int i = 1; i < d.nx - 1; ++i)
+        for (int j = 1; j < d.ny - 1; ++j)
+            for (int k = 1; k < d.nz - 1; ++k) {
+                const int64_t ci = ((int64_t)i * d.ny + j) * d.nz + k;
+                if (type[ci] != TYPE_F || !(flag[ci] & NB_PUSH)) continue;
+                for (int l = 0; l < Q; ++l) {
+                    const int64_t dst =
+                        ((int64_t)wrap(i + cx(l), d.nx) * d.ny + wrap(j + cy(l), d.ny)) * d.nz + wrap(k + cz(l), d.nz);
+                    for (int a = 0; a < c->ns; ++a)
+                        host[((size_t)a * Q + l) * n + ci] = B[((size_t)a * Q + l) * n + dst];
+                }
+            }
+    for (int64_t s = 0; s < c->nside; ++s)
+        for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl) host[pl * n + side_cell[s]] = S[pl * c->nside + s];
+    return LBM_OK;
+}
+
 }  // namespace
 
 int lbm_set_populations(lbm_ctx* c, int which, const double* host)
@@ -867,6 +922,10 @@ int lbm_set_populations(lbm_ctx* c, int which, const double* host)
     if (which < 0 || which > 3) return fail(c, LBM_E_ARG, "population set out of range");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (multi(c)) {
+        if (c->pushed)
+            return fail(c, LBM_E_STATE,
+                        "populations cannot be replaced between lbm_collide_species and lbm_stream in the fused "
+                        "mode (the species are already pushed to their streamed slots)");
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
         void* dev = sp ? c->sf[slot] : c->pg[slot];
@@ -928,6 +987,7 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
     if (multi(c)) {
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
+        if (which == LBM_POP_FPC && c->pushed) return read_pushed_fpc(c, host);
         // the fused mode keeps no post-collision species buffer: rebuild it
         // from the streamed populations and the side store
         if (which == LBM_POP_FPC && fused(c) && c->push_ready)
@@ -1037,6 +1097,7 @@ int lbm_init(lbm_ctx* c)
         if (int rc = run_mc(c, McKernel::Init, "mc_init", 0, 0)) return rc;
         c->fields_valid = true;
         c->push_ready = false;
+        c->pushed = false;
         return LBM_OK;
     }
     if (int rc = materialize(c)) return rc;
@@ -1075,9 +1136,11 @@ int lbm_collide_species(lbm_ctx* c)
         if (rc) return rc;
     }
     int rc = run_mc(c, McKernel::SpeciesTransport, "mc_species_transport", 0, 1);
+    const bool push = fused(c) && c->push_ready;
     if (!rc)
-        rc = fused(c) && c->push_ready ? run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1)
-                                       : run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+        rc = push ? run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1)
+                  : run_mc(c, McKernel::CollideSpecies, "mc_collide_species", 0, 1);
+    if (!rc && push) c->pushed = true;
     ++c->step;
     return rc;
 }
@@ -1110,7 +1173,10 @@ int lbm_stream(lbm_ctx* c)
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
     if (multi(c)) {
         if (fused(c) && c->push_ready) {  // Collide_Species wrote the streamed species slots of buffer 1
+            if (!c->pushed)
+                return fail(c, LBM_E_STATE, "lbm_collide_species must run before lbm_stream in the fused mode");
             std::swap(c->sf[0], c->sf[1]);
+            c->pushed = false;
             return run_mc(c, McKernel::StreamEnergy, "mc_stream_energy", 1, 0);
         }
         if (int rc = run_mc(c, McKernel::Stream, "mc_stream", 1, 0)) return rc;

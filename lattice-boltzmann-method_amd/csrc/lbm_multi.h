@@ -98,9 +98,9 @@ void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, uint16_
 template <typename P>
 void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsigned* count, hipStream_t s);
 
-// Per-block statistics over TYPE_F cells (MC_SUM_BLOCKS x MC_STATS doubles
-// into `part`): the sums of 1, T, p and X_a (MC_SUMS values), then the minima
-// of X_a (NS values).
+// Per-block statistics (MC_SUM_BLOCKS x MC_STATS doubles into `part`): the
+// sums of 1, T, p and X_a over TYPE_F cells (MC_SUMS values), then the minima
+// of X_a over TYPE_F cells and inflow ghosts (NS values).
 constexpr int MC_SUMS = 3 + chem::NS, MC_STATS = MC_SUMS + chem::NS, MC_SUM_BLOCKS = 512;
 template <typename S>
 void launch_mc_state_sums(const uint8_t* type, const double* T, const double* p, const S* X, int64_t n,

@@ -1996,7 +1996,8 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
 
 // ------------------------------------------- rest state of FP32 storage
 // Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
-// the mean state the FP32 deviations are taken from.
+// the mean state the FP32 deviations are taken from, and the minima of X_a
+// over the TYPE_F cells and the inflow ghosts (TYPE_I, TYPE_I_C).
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, const double* T, const double* p,
                                                          const S* X, int64_t n, double* part)
@@ -2006,6 +2007,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, co
 #pragma unroll
     for (int a = 0; a < NS; ++a) acc[MC_SUMS + a] = HUGE_VAL;
     for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < n; c += (int64_t)gridDim.x * BLOCK) {
+        if (type[c] == TYPE_I || type[c] == TYPE_I_C) {  // inflow ghosts: TMS_BC imposes their composition
+#pragma unroll
+            for (int a = 0; a < NS; ++a) acc[MC_SUMS + a] = fmin(acc[MC_SUMS + a], (double)X[a * n + c]);
+            continue;
+        }
         if (type[c] != TYPE_F) continue;
         acc[0] += 1.0;
         acc[1] += T[c];

@@ -395,3 +395,81 @@ def premixed_flame_1d(nx, ny=4, nz=4, u_in=None, noise=0.01, seed=20261204, midp
     fl["v"][...] = 0.0
     fl["w"][...] = 0.0
     return fl
+
+
+# units.set_m_kg_s(dx, dt) of the transport wave columns: dt/dx = 4e-4 keeps
+# theta of the lightest species (H, 300 K) at 0.4, and dx puts tau - 1/2 of air
+# near 0.9, so viscous, thermal and diffusive decays take a few hundred steps
+UNITS_TRANSPORT = (5e-7, 2e-10)
+
+
+def transport_column(N, X, T_K=300.0, p_si=ONE_ATM, units=UNITS_TRANSPORT, minim=1e-10):
+    """Quasi-1-D periodic column of N cells along y (1 x N x 1 interior,
+    periodic ghosts everywhere) holding a uniform mixture X (mole fractions by
+    species name) at rest at T_K and p_si: the shape of the reference's
+    VISCOSITY_TEST_MULTICOMP (setup.cpp:1672-1742, 1 x 200 x 1, periodic).
+    Returns the fields and the y index of each interior cell from 0."""
+    fl = default_fields(1, N, 1)
+    periodic_ghosts(fl)
+    shape = fl["type"].shape
+    x = mole_vector(X, minim)
+    fl["X"] = np.zeros((len(x),) + shape)
+    fl["X"][:] = x[:, None, None, None]
+    fl["T"][...] = T_K / LATTICE_K
+    fl["p"][...] = lattice_p(p_si, *units)
+    return fl, np.arange(N, dtype=np.float64)
+
+
+def wave_column(kind, N, amp, X=None, T_K=300.0, units=UNITS_TRANSPORT, species=None):
+    """Single-mode waves on transport_column, phase 2 pi j / N over the
+    interior cells j = 0..N-1 (ghost-inclusive index j + 1):
+      'shear'      u = amp sin           (VISCOSITY_TEST_MULTICOMP, setup.cpp:1672)
+      'thermal'    T = T0 (1 + amp sin), p uniform   (CONDUCTIVITY_TEST, setup.cpp:396)
+      'diffusion'  X_a = X0_a + amp sin for the first named species and
+                   X0_b - amp sin for the second, p and T uniform
+                   (TERNARY_DIFFUSION reduced to two species, setup.cpp:768)
+      'sound'      p = p0 (1 + amp cos), T = T0 (1 + (1 - 1/gamma_guess) amp cos)
+                   (SOUNDSPEED_TEST, setup.cpp:463)"""
+    X = X or X_AIR
+    fl, j = transport_column(N, X, T_K, units=units)
+    ph = 2.0 * np.pi * j / N
+    sl = (1, slice(1, -1), 1)
+    if kind == "shear":
+        fl["u"][sl] = amp * np.sin(ph)
+    elif kind == "thermal":
+        fl["T"][sl] *= 1.0 + amp * np.sin(ph)
+    elif kind == "diffusion":
+        names = list(X)
+        a, b = names[0], names[1]
+        for q in range(N):
+            xq = dict(X)
+            xq[a] = X[a] + amp * np.sin(ph[q])
+            xq[b] = X[b] - amp * np.sin(ph[q])
+            fl["X"][:, 1, q + 1, 1] = mole_vector(xq, species=species)
+    elif kind == "sound":
+        fl["p"][sl] *= 1.0 + amp * np.cos(ph)
+        fl["T"][sl] *= 1.0 + (1.0 - 1.0 / 1.4) * amp * np.cos(ph)
+    else:
+        raise ValueError(kind)
+    return fl
+
+
+def outflow_column(N, X=None, T_K=300.0, dp_in=0.0, dv_in=0.0, p_ghost_rel=1.0, units=UNITS_TRANSPORT):
+    """Column of N cells along y at rest with NSCBC outflow ghosts (TYPE_O_C) on
+    both y faces, periodic in x and z (the y boundaries of the mixing layer),
+    for single TMS_BC calls: uniform T and composition; pressure
+    p0 (1 + dp_in q) and velocity v = dv_in q at distance q = 0, 1, ... from
+    the lower face, mirrored at the upper face; the ghosts hold p0 p_ghost_rel."""
+    X = X or X_AIR
+    fl, j = transport_column(N, X, T_K, units=units)
+    t = fl["type"]
+    t[:, 0, :] = TYPE_O_C
+    t[:, -1, :] = TYPE_O_C
+    p0 = fl["p"][1, 1, 1]
+    q = np.minimum(j, N - 1 - j)
+    sign = np.where(j < N / 2, 1.0, -1.0)
+    fl["p"][1, 1:-1, 1] = p0 * (1.0 + dp_in * q)
+    fl["v"][1, 1:-1, 1] = sign * dv_in * q
+    fl["p"][:, 0, :] = p0 * p_ghost_rel
+    fl["p"][:, -1, :] = p0 * p_ghost_rel
+    return fl

@@ -414,6 +414,25 @@ def test_mc_fp32_refuses_unresolvable_trace_species():
         lat.Init()
 
 
+def test_mc_fp32_refuses_traces_held_only_by_the_inflow_ghost():
+    """A 1e-6 trace floor in the domain but 1e-10 traces in the TYPE_I_C
+    inflow ghost, whose composition TMS_BC imposes on the first plane every
+    step: the FP32 init refuses it as well."""
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LbmError
+    fl = cases.premixed_flame_1d(64, 3, 3, minim=1e-6)
+    fl["X"][:, 0] = cases.mole_vector(cases.X_UNBURNT_PHI1, 1e-10)[:, None, None]
+    lat = MixtureLattice(64, 3, 3, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED)
+    lat.load_case(fl)
+    with pytest.raises(LbmError, match="cannot resolve"):
+        lat.Init()
+    lat.close()
+    fl["X"][:, 0] = cases.mole_vector(cases.X_UNBURNT_PHI1, 1e-6)[:, None, None]
+    lat = MixtureLattice(64, 3, 3, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED)
+    lat.load_case(fl)
+    lat.Init()
+    lat.close()
+
+
 def test_mc_fp32_full_size_flame_tracks_fp64():
     """The 2048x4x4 premixed flame with a 1e-6 trace floor: FP32 shifted
     storage against FP64 over 300 steps (both finite, flame temperature and

@@ -128,3 +128,43 @@ def test_fused_refuses_free_slip():
     lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=False, mode=LBM_MODE_FUSED)
     with pytest.raises(LbmError, match="TYPE_FS"):
         lat.load_case(fl)
+
+
+def test_fused_reads_between_collide_species_and_streaming():
+    """Between a fused Collide_Species (species already pushed to their
+    streamed slots) and Streaming, LBM_POP_FPC is gathered from those slots
+    and the side store, and equals the strict mode's post-collision set; F,
+    G and GPC read as in strict mode; a population upload there is refused
+    (LBM_E_STATE); stepping on after the reads stays bitwise equal to
+    strict mode."""
+    fl = cases.mixing_layer(6, 12, 5, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=3.0)
+    F = fl["type"].ravel() == 0
+    ns = 9
+    strict = make(fl, LBM_MODE_STRICT)
+    fused = make(fl, LBM_MODE_FUSED)
+    try:
+        strict.step(2)
+        fused.step(2)  # the first fused step runs strictly; the second pushes
+        for lat in (strict, fused):
+            lat.Collide_Species()
+            lat.Collide()
+        for which in (POP_FPC, POP_F):
+            a = fused.get_populations(which).reshape(ns, 27, -1)[:, :, F]
+            b = strict.get_populations(which).reshape(ns, 27, -1)[:, :, F]
+            assert np.array_equal(a, b), f"population set {which} differs before Streaming"
+        for which in (POP_G, POP_GPC):
+            assert np.array_equal(fused.get_populations(which).reshape(27, -1)[:, F],
+                                  strict.get_populations(which).reshape(27, -1)[:, F])
+        with pytest.raises(LbmError, match="lbm_collide_species and lbm_stream"):
+            fused.set_populations(POP_F, fused.get_populations(POP_F))
+        for lat in (strict, fused):
+            lat.Streaming()
+            lat.TMS_BC()
+            lat.calculate_moment()
+        assert_same(snapshot(fused, F), snapshot(strict, F), "after the mid-step reads")
+        strict.step(2)
+        fused.step(2)
+        assert_same(snapshot(fused, F), snapshot(strict, F), "two steps later")
+    finally:
+        strict.close()
+        fused.close()

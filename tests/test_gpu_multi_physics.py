@@ -1,0 +1,247 @@
+"""The multicomponent closed forms on the HIP path, through the C ABI.
+
+GPU mirrors of test_oracle_multicomp_physics.py: the same 1 x N x 1 columns
+(the reference's VISCOSITY_TEST_MULTICOMP, CONDUCTIVITY_TEST, TERNARY_DIFFUSION
+and SOUNDSPEED_TEST shapes, setup.cpp) run by the kernels at N = 32 and 64,
+each rate extrapolated to the continuum (Richardson) and compared with the
+textbook value computed from the mechanism data (tests/physics_waves.py);
+the FP32-shifted storage is held to the same closed forms at N = 64.  The
+NSCBC outflow target and the Maxwell-Stefan residual are checked call by
+call, and the columns are compared with the oracle step by step at 1e-10.
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import cases
+from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_STRICT, POP_F, MixtureLattice
+from oracle.chem import Mechanism
+from oracle.multicomp import C3, MultiState, MultiStateBC, Units
+
+import physics_waves as pw
+
+pytestmark = pytest.mark.gpu
+
+MECH = Mechanism()
+SP = cases.H2AIR_SPECIES
+NS = len(SP)
+P_ATM = cases.ONE_ATM
+
+
+def gpu_column(fl, precision=LBM_FP64, mode=LBM_MODE_FUSED):
+    nx, ny, nz = (s - 2 for s in fl["type"].shape)
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_TRANSPORT, reaction=False, mode=mode, precision=precision)
+    lat.load_case(fl)
+    lat.Init()
+    lat.calculate_moment()
+    return lat
+
+
+def gcol(lat, name):
+    return lat.get_field(name)[1, 1:-1, 1]
+
+
+def series(fl, steps, probe, every=1, precision=LBM_FP64):
+    lat = gpu_column(fl, precision)
+    out = []
+    try:
+        for _ in range(steps // every):
+            lat.step(every)
+            out.append(probe(lat))
+    finally:
+        lat.close()
+    return np.array(out)
+
+
+def ratio(kind, N, steps, amp, X, expected, precision=LBM_FP64, every=4):
+    fl = cases.wave_column(kind, N, amp, X=X)
+    if kind == "shear":
+        probe = lambda lat: pw.y_mode(gcol(lat, "u"), N)  # noqa: E731
+    elif kind == "thermal":
+        probe = lambda lat: pw.y_mode(gcol(lat, "T"), N)  # noqa: E731
+    else:
+        a = SP.index(list(X)[0])
+        probe = lambda lat: pw.y_mode(lat.get_species_field("X")[a, 1, 1:-1, 1], N)  # noqa: E731
+    s = series(fl, steps, probe, every, precision)
+    # decay per sample of `every` steps
+    return pw.decay_rate(s) / every / expected
+
+
+AIR = cases.mole_vector(cases.X_AIR)
+
+
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_gpu_shear_wave_decays_at_the_wilke_mixture_viscosity(precision):
+    """omega_a of k_mc_species_transport: nu k^2 with nu = mu_Wilke / rho."""
+    if precision == LBM_FP32_SHIFTED:
+        r = ratio("shear", 64, 1600, 1e-3, cases.X_AIR, pw.expected_shear_rate(MECH, 64, 300.0, P_ATM, AIR), precision)
+        assert abs(r - 1.0) < 0.012  # second-order lattice error at N = 64 is 0.6 %
+        return
+    r = [ratio("shear", N, st, 1e-3, cases.X_AIR, pw.expected_shear_rate(MECH, N, 300.0, P_ATM, AIR))
+         for N, st in ((32, 400), (64, 1600))]
+    assert abs(r[1] - 1.0) < 0.012
+    assert abs(pw.richardson(*r) - 1.0) < 0.004
+
+
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_gpu_thermal_wave_decays_at_kappa_over_rho_cp(precision):
+    """omega1 of k_mc_collide: kappa k^2 / (rho c_p), mixture-averaged kappa."""
+    if precision == LBM_FP32_SHIFTED:
+        r = ratio("thermal", 64, 1600, 1e-3, cases.X_AIR, pw.expected_thermal_rate(MECH, 64, 300.0, P_ATM, AIR),
+                  precision)
+        assert abs(r - 1.0) < 0.02
+        return
+    r = [ratio("thermal", N, st, 1e-3, cases.X_AIR, pw.expected_thermal_rate(MECH, N, 300.0, P_ATM, AIR))
+         for N, st in ((32, 400), (64, 1600))]
+    assert abs(r[1] - 1.0) < 0.02
+    assert abs(pw.richardson(*r) - 1.0) < 0.004
+
+
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_gpu_binary_diffusion_decays_at_the_binary_diffusivity(precision):
+    """Friction of k_mc_species_transport with the Maxwell-Stefan solve of
+    k_mc_moments: D_ab k^2 for an H2/N2 composition mode."""
+    mix = {"H2": 0.5, "N2": 0.5}
+    a, b = SP.index("H2"), SP.index("N2")
+    if precision == LBM_FP32_SHIFTED:
+        r = ratio("diffusion", 64, 800, 0.02, mix, pw.expected_diffusion_rate(MECH, 64, 300.0, P_ATM, a, b),
+                  precision)
+        assert abs(r - 1.0) < 0.01
+        return
+    r = [ratio("diffusion", N, st, 0.02, mix, pw.expected_diffusion_rate(MECH, N, 300.0, P_ATM, a, b))
+         for N, st in ((32, 300), (64, 800))]
+    assert abs(r[1] - 1.0) < 0.01
+    assert abs(pw.richardson(*r) - 1.0) < 0.004
+
+
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_gpu_sound_wave_travels_at_the_frozen_sound_speed(precision):
+    """Energy equilibria and the UV temperature of k_mc_moments: the standing
+    pressure mode's period is N / sqrt(gamma R T)."""
+    r = []
+    for N, st in ((32, 500), (64, 1000)):
+        fl = cases.wave_column("sound", N, 1e-4)
+        s = series(fl, st, lambda lat: pw.y_mode(gcol(lat, "p"), N, "cos"), 1, precision)
+        r.append(pw.expected_sound_period(MECH, N, 300.0, P_ATM, AIR) / pw.zero_crossing_period(s))
+    assert abs(r[1] - 1.0) < 0.002
+    assert abs(pw.richardson(*r) - 1.0) < 0.001
+
+
+def test_gpu_diffusion_carries_the_species_enthalpy():
+    """q_diff and q_corr of k_mc_collide: H2O/N2 inter-diffusion keeps T
+    uniform to a fifth of the frozen-energy estimate away from the periodic
+    rows (quirk Q2)."""
+    N, steps = 32, 200
+    fl = cases.wave_column("diffusion", N, 0.02, X={"H2O": 0.3, "N2": 0.7})
+    lat = gpu_column(fl)
+    try:
+        T0 = gcol(lat, "T").copy()
+        rho0 = gcol(lat, "rho")
+        Y0 = lat.get_species_field("rho")[:, 1, 1:-1, 1] / rho0
+        lat.step(steps)
+        T1 = gcol(lat, "T")
+        Y1 = lat.get_species_field("rho")[:, 1, 1:-1, 1] / gcol(lat, "rho")
+    finally:
+        lat.close()
+    u_a = (MECH.h_RT(np.array(300.0)) - 1.0) * MECH.R * 300.0 / MECH.W
+    cv = pw.cp_mass(MECH, 300.0, Y0[:, 0]) - MECH.R / (1.0 / np.sum(Y0[:, 0] / MECH.W))
+    frozen = np.abs((u_a[:, None] * (Y1 - Y0)).sum(axis=0) / cv) / 300.0
+    inner = slice(2, N - 2)
+    assert frozen[inner].max() > 0.05
+    assert np.abs(T1 / T0.mean() - 1.0)[inner].max() < 0.2 * frozen[inner].max()
+
+
+def test_gpu_maxwell_stefan_velocities_solve_their_system():
+    """k_mc_moments (FP64): the species velocities it writes solve
+    A u = (rho u)_a with the previous state's p, X and diffusivities, to
+    1e-12 of the momenta (matrix built here from the mechanism's D_ab)."""
+    fl = cases.wave_column("diffusion", 16, 0.05, X={"H2": 0.3, "O2": 0.2, "N2": 0.5})
+    lat = gpu_column(fl, mode=LBM_MODE_STRICT)
+    try:
+        lat.step(3)
+        lat.Collide_Species()
+        lat.Collide()
+        lat.Streaming()
+        old = {k: gcol(lat, k).copy() for k in ("p", "T", "rho")}
+        X_old = lat.get_species_field("X")[:, 1, 1:-1, 1].copy()
+        ra_old = lat.get_species_field("rho")[:, 1, 1:-1, 1].copy()
+        f = lat.get_populations(POP_F)[:, :, 1, 1:-1, 1]
+        lat.calculate_moment()
+        ua = np.stack([lat.get_species_field(n)[:, 1, 1:-1, 1] for n in "uvw"])
+        ra = lat.get_species_field("rho")[:, 1, 1:-1, 1]
+    finally:
+        lat.close()
+    U = Units(*cases.UNITS_TRANSPORT)
+    mom = np.einsum("dl,alc->dac", C3.astype(float), f)
+    worst = 0.0
+    for q in range(f.shape[-1]):
+        Tsi = np.float64(U.si_temp(old["T"][q]))
+        Y = MECH.normalize_Y(ra_old[:, q] / old["rho"][q])
+        p_si = MECH.pressure(np.array(Tsi), np.float64(U.si_rho(old["rho"][q])), Y)
+        D = U.nu(MECH.binary_diff(np.array(Tsi), np.array(p_si))).astype(np.float64)
+        A = -0.5 * old["p"][q] * np.outer(X_old[:, q], X_old[:, q]) / np.where(np.eye(NS) > 0, 1.0, D)
+        np.fill_diagonal(A, 0.0)
+        A[np.diag_indices(NS)] = ra[:, q] - A.sum(axis=1)
+        res = A @ ua[:, :, q].T - mom[:, :, q].T
+        worst = max(worst, np.abs(res).max() / np.abs(mom[:, :, q]).max())
+    assert worst < 1e-12
+    assert np.ptp(ua[1][:3], axis=0).max() > 1e-6
+
+
+@pytest.mark.parametrize("isign", [1, -1])
+def test_gpu_nscbc_outflow_target_is_the_lodi_step(isign):
+    """k_mc_tms for a TYPE_O_C face: after one TMS_BC call the interface cell
+    holds the density, velocity and pressure of one LODI step with the
+    outgoing wave from one-sided derivatives and the incoming one relaxed
+    with K = 0.25 (1 - M^2) a / L (closed form of physics_waves)."""
+    N = 10
+    fl = cases.outflow_column(N, dp_in=2e-4, dv_in=3e-4, p_ghost_rel=0.995)
+    lat = gpu_column(fl, mode=LBM_MODE_STRICT)
+    try:
+        j = 1 if isign > 0 else N
+        idx = [j, j + 1, j + 2] if isign > 0 else [j, j - 1, j - 2]
+        fields = {k: lat.get_field(k)[1, idx, 1].copy() for k in ("rho", "v", "p", "T")}
+        Xc = lat.get_species_field("X")[:, 1, j, 1]
+        p_inf = lat.get_field("p")[1, 0, 1]
+        lat.TMS_BC()
+        lat.calculate_moment()
+        rho, v, p = (lat.get_field(k)[1, j, 1] for k in ("rho", "v", "p"))
+    finally:
+        lat.close()
+    _, _, Rm, gam = pw.gas_state(MECH, float(fields["T"][0] * cases.LATTICE_K), 1.0, Xc)
+    a = np.sqrt(gam * Rm * fields["T"][0] * cases.LATTICE_K) * pw.lattice_scale()[1]
+    tgt = pw.lodi_outflow_target(fields, a, p_inf, isign, L=N + 2)
+    assert abs(p / fields["p"][0] - 1.0) > 1e-5
+    assert abs(rho / tgt["rho"] - 1.0) < 1e-9
+    assert abs(v - tgt["v"]) < 1e-6 * abs(tgt["v"])
+    assert abs(p / tgt["p"] - 1.0) < 5e-7
+
+
+@pytest.mark.parametrize("kind,X", [("shear", None), ("thermal", None), ("diffusion", {"H2": 0.5, "N2": 0.5}),
+                                    ("sound", None)])
+def test_gpu_wave_columns_match_the_oracle_step_by_step(kind, X):
+    """The four columns, 20 steps each, GPU (FP64, fused mode) against the
+    oracle at the FP64 parity bar (1e-10 max relative in rho, u, v, T, p and
+    the species densities)."""
+    N = 16
+    fl = cases.wave_column(kind, N, 0.02 if kind == "diffusion" else 1e-3, X=X)
+    lat = gpu_column(fl)
+    s = MultiState(*fl["type"].shape, Units(*cases.UNITS_TRANSPORT))
+    s.type[:] = fl["type"].ravel()
+    for k in ("rho", "u", "v", "w", "p", "T"):
+        getattr(s, k)[:] = fl[k].ravel()
+    s.X[:] = fl["X"].reshape(NS, -1)
+    s.init()
+    s.moments()
+    F = s.type == 0
+    try:
+        for step in range(20):
+            lat.step(1)
+            s.step(reaction=False)
+            for k in ("rho", "u", "v", "T", "p"):
+                a, b = lat.get_field(k).ravel()[F], getattr(s, k)[F]
+                scale = max(np.abs(b).max(), 1e-300) if k not in ("u", "v") else max(np.abs(b).max(), 1e-12)
+                assert np.abs(a - b).max() / scale < 1e-10, f"step {step}: {k}"
+            ra = lat.get_species_field("rho").reshape(NS, -1)[:, F]
+            assert np.abs(ra - s.rho_a[:, F]).max() / s.rho_a[:, F].max() < 1e-10, f"step {step}: rho_a"
+    finally:
+        lat.close()
