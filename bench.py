@@ -51,13 +51,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="mixing256",
-                    choices=["mixing256", "mixing_c5share", "turb256", "tgv64", "copy1g"])
+                    choices=["mixing256", "mixing768", "mixing_c5share", "turb256", "turb256_report", "tgv64",
+                             "copy1g"])
+    ap.add_argument("--report-every", type=int, default=250, help="turb256_report: steps between error samples")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--mode", default="fused", choices=["fused", "strict"],
+    ap.add_argument("--mode", default="fused", choices=["fused", "strict", "inplace"],
                     help="multicomponent workloads: the fused mode (Collide_Species writes the streamed slots; "
-                         "bitwise equal to strict) or one kernel per reference call")
+                         "bitwise equal to strict), one kernel per reference call, or the in-place mode (one "
+                         "species buffer streamed by the AA pattern; bitwise equal to strict)")
     ap.add_argument("--chemistry", default="surrogate", choices=["surrogate", "plain"],
                     help="mixing256: learned chemistry (default) or the plain integrator in the FP32 run "
                          "(no capture or training; used for the PMC byte-counter passes)")
@@ -97,8 +100,9 @@ def make_lattice(fl, pr, precision, mode):
 
 SINGLE_KERNELS = ("fused_collide_stream", "fused_moments", "collide_explicit")
 MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_fallback", "mc_species_transport",
-                 "mc_collide_species", "mc_collide_species_push", "mc_collide_mixture", "mc_stream", "mc_stream_energy", "mc_tms_outflow",
-                 "mc_tms_inflow", "mc_tms_wall", "mc_moments")
+                 "mc_aa_save", "mc_collide_species", "mc_collide_species_push", "mc_collide_species_inplace",
+                 "mc_collide_mixture", "mc_stream", "mc_stream_energy", "mc_tms_outflow", "mc_tms_inflow",
+                 "mc_tms_wall", "mc_moments")
 
 
 def timed(lat, steps, warmup, names=SINGLE_KERNELS):
@@ -160,6 +164,7 @@ def cpu_baseline(threads):
 # populations each one reads and writes (9 species x 27 f, 27 g), per byte
 # of storage.  Moment and property planes are excluded.
 MC_POP_TRAFFIC = {"mc_collide_species": 2 * 9 * 27, "mc_collide_species_push": 2 * 9 * 27,
+                  "mc_collide_species_inplace": 2 * 9 * 27,
                   "mc_collide_mixture": 2 * 27, "mc_stream": 2 * 10 * 27, "mc_stream_energy": 2 * 27,
                   "mc_moments": 10 * 27}
 
@@ -191,9 +196,10 @@ def cpu_baseline_mixing():
 def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_256x128x128"):
     """The reacting mixing layer: FP64 plain beside FP32 + surrogate."""
     from lbm_amd import cases
-    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_STRICT, MixtureLattice
+    from lbm_amd.capi import (LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_INPLACE, LBM_MODE_STRICT,
+                              MixtureLattice)
     cells = nx * ny * nz
-    mode = LBM_MODE_FUSED if args.mode == "fused" else LBM_MODE_STRICT
+    mode = {"fused": LBM_MODE_FUSED, "strict": LBM_MODE_STRICT, "inplace": LBM_MODE_INPLACE}[args.mode]
     fl = cases.mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
     F = (fl["type"] == 0).ravel()
     cap_steps, cap_stride = 4, 4          # ~4.2 M training states
@@ -267,7 +273,7 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
         extra["fp64_mlups"] = round(mlups64, 2)
         extra["speedup_fp32_surrogate_vs_fp64"] = round(mlups / mlups64, 3)
         extra["fp64_kernels"] = {n: {"avg_ms": round(v["avg_ms"], 4)} for n, v in kern64.items()}
-        if mode == LBM_MODE_FUSED:
+        if mode != LBM_MODE_STRICT:
             mlups64s, kern64s, _, _ = fp64_run(LBM_MODE_STRICT)
             extra["fp64_strict_mlups"] = round(mlups64s, 2)
             extra["speedup_fp32_surrogate_vs_fp64_strict"] = round(mlups / mlups64s, 3)
@@ -286,6 +292,164 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
         "roofline": roofline, "cpu_baseline": None, "extra": extra, "host": platform.node(),
     }
     return rec
+
+
+def block_mean(a, nx, ny, nz):
+    """2 x 2 x 2 block means of the interior of a ghost-inclusive field."""
+    a = a[..., 1:-1, 1:-1, 1:-1]
+    lead = a.shape[:-3]
+    return a.reshape(lead + (nx // 2, 2, ny // 2, 2, nz // 2, 2)).mean(axis=(-5, -3, -1))
+
+
+def mixing_c5_bench(args):
+    """The combined large variant: the reacting mixing layer on 768x384x384
+    (113 M cells) in the in-place mode (one species buffer), FP32 shifted
+    populations and the learned chemistry, on one GPU.  Then the FP64 plain
+    run of the 384x192x192 twin (same physical set-up: dx and dt doubled, the
+    tanh thickness and the speeds in lattice units unchanged) for half the
+    steps, and the fine run's fields averaged over 2x2x2 blocks against it:
+    the error of resolution and precision together."""
+    from lbm_amd import cases
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_INPLACE, MixtureLattice
+    nx, ny, nz = 768, 384, 384
+    cells = nx * ny * nz
+    steps = args.steps
+
+    def log(msg):
+        print(f"mixing768: {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    fl = cases.mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
+    log(f"set-up built in {time.perf_counter() - t0:.0f} s")
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, precision=LBM_FP32_SHIFTED,
+                         mode=LBM_MODE_INPLACE)
+    lat.load_case(fl)
+    del fl
+    lat.Init()
+    lat.calculate_moment()
+    log(f"initialised, {lat.device_bytes() / 1e9:.1f} GB on the device")
+    cap_steps = 4
+    cap_stride = max(4, cap_steps * cells // 4_200_000)   # ~4.2 M training states
+    train_iters, batch = 2000, 4096
+    lat.surrogate_capture(cap_steps * cells // cap_stride + 65536, cap_stride)
+    lat.step(cap_steps)
+    lat.surrogate_capture_stop()
+    lat.sync()
+    t1 = time.perf_counter()
+    lat.surrogate_train(train_iters * 3 // 4, batch=batch, lr=2e-3, seed=1)
+    lat.surrogate_train(train_iters // 4, batch=batch, lr=3e-4, seed=2)
+    train_s = time.perf_counter() - t1
+    lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
+    log(f"surrogate trained in {train_s:.1f} s; timing {steps} steps")
+    sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS)
+    mlups = cells * steps / sec / 1e6
+    log(f"{mlups:.1f} MLUPS")
+    sstats = lat.surrogate_stats()
+    diag = lat.diagnostics()
+    dev_bytes = lat.device_bytes()
+    T32 = lat.get_field("T")
+    rho32 = lat.get_field("rho")
+    ra32 = lat.get_species_field("rho")
+    lat.close()
+    fine_steps = cap_steps + args.warmup + steps
+    T_f = block_mean(T32, nx, ny, nz)
+    Y_f = block_mean(ra32 / rho32[None], nx, ny, nz)
+    del T32, rho32, ra32
+
+    tx, ty, tz = nx // 2, ny // 2, nz // 2
+    twin_units = (2 * cases.UNITS_FLAME[0], 2 * cases.UNITS_FLAME[1])
+    ft = cases.mixing_layer(tx, ty, tz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
+    ref = MixtureLattice(tx, ty, tz, *twin_units, reaction=True, precision=LBM_FP64, mode=LBM_MODE_INPLACE)
+    ref.load_case(ft)
+    ref.Init()
+    ref.calculate_moment()
+    twin_steps = fine_steps // 2
+    t2 = time.perf_counter()
+    ref.step(twin_steps)
+    ref.sync()
+    twin_s = time.perf_counter() - t2
+    T64 = ref.get_field("T")[1:-1, 1:-1, 1:-1]
+    Y64 = (ref.get_species_field("rho") / ref.get_field("rho")[None])[:, 1:-1, 1:-1, 1:-1]
+    ref.close()
+    log(f"twin: {twin_steps} FP64 steps in {twin_s:.1f} s")
+
+    dom = "mc_collide_species_inplace"
+    roofline = None
+    if dom in kern:
+        bpl = MC_POP_TRAFFIC[dom] * 4
+        achieved = bpl * cells / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload, dom),
+                    "kernel": dom, "bytes_per_launch": bpl * cells, "avg_ms": round(kern[dom]["avg_ms"], 4)}
+    rel = np.abs(T_f - T64) / T64
+    extra = {"kernels": {n: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]} for n, v in kern.items()},
+             "device_bytes_fp32": dev_bytes, "nan_count": diag["nan_count"],
+             "surrogate": {"train_s": round(train_s, 2), "capture_stride": cap_stride, **sstats},
+             "fallback_rate_over_run": round((sstats["fallback"] + sstats["audit_failures"]) /
+                                             max(sstats["predicted"] + sstats["fallback"], 1), 6),
+             "twin": {"grid": [tx, ty, tz], "precision": "f64", "chemistry": "plain", "steps": twin_steps,
+                      "fine_steps": fine_steps, "units": twin_units, "seconds": round(twin_s, 1)},
+             "error_vs_fp64_twin": {"flame_T_rel": float(f"{abs(T_f.max() - T64.max()) / T64.max():.3e}"),
+                                    "T_max_rel": float(f"{rel.max():.3e}"),
+                                    "T_rms_rel": float(f"{np.sqrt(np.mean(rel * rel)):.3e}"),
+                                    "Y_L2": float(f"{np.sqrt(np.mean((Y_f - Y64) ** 2)):.3e}"),
+                                    "Y_max_abs": float(f"{np.max(np.abs(Y_f - Y64)):.3e}")}}
+    return {
+        "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(sec / steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "reacting_mixing_layer_768x384x384_9sp_fp32_shifted_surrogate", "cells": cells,
+                   "grid": [nx, ny, nz], "mode": "inplace", "parallelism": "none"},
+        "roofline": roofline, "cpu_baseline": None, "extra": extra, "host": platform.node(),
+    }
+
+
+def turbulence_report(args):
+    """The 256^3 decaying-turbulence precision variant: FP32 shifted
+    populations and FP64 plain, both fused, stepped side by side from the same
+    set-up for --steps steps (the configuration's 5000).  Every
+    --report-every steps: kinetic energy and enstrophy of each run (the
+    reference's output.cpp:212-276 sums) and their relative difference, and
+    the FP32 temperature field's error against FP64 (max and RMS relative over
+    the interior).  Stops at the first step either run holds a non-finite
+    cell and reports it.  Progress lines go to stderr; one JSON line to stdout."""
+    from lbm_amd import cases
+    N = 256
+    fl, pr = cases.decaying_turbulence(N, nu=0.005)
+    from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED
+    f32 = make_lattice(fl, pr, LBM_FP32_SHIFTED, LBM_MODE_FUSED)
+    f64 = make_lattice(fl, pr, LBM_FP64, LBM_MODE_FUSED)
+    rows, diverged, done = [], None, 0
+    t0 = time.perf_counter()
+    while done < args.steps:
+        k = min(args.report_every, args.steps - done)
+        f32.step(k)
+        f64.step(k)
+        done += k
+        a, b = f32.diagnostics(), f64.diagnostics()
+        row = {"step": done, "ke_fp64": b["kinetic"], "ens_fp64": b["enstrophy"],
+               "ke_rel": abs(a["kinetic"] - b["kinetic"]) / abs(b["kinetic"]),
+               "ens_rel": abs(a["enstrophy"] - b["enstrophy"]) / abs(b["enstrophy"]),
+               "nan_fp32": a["nan_count"], "nan_fp64": b["nan_count"], "max_mach_fp64": b["max_mach"]}
+        if a["nan_count"] or b["nan_count"]:
+            diverged = done
+        else:
+            T32 = f32.get_field("T")[1:-1, 1:-1, 1:-1]
+            T64 = f64.get_field("T")[1:-1, 1:-1, 1:-1]
+            rel = np.abs(T32 - T64) / np.abs(T64)
+            row["T_max_rel"] = float(rel.max())
+            row["T_rms_rel"] = float(np.sqrt(np.mean(rel * rel)))
+        rows.append(row)
+        print(f"turb256 report: step {done}: " + ", ".join(f"{key} {val:.4g}" for key, val in row.items()
+                                                          if key != "step"), file=sys.stderr, flush=True)
+        if diverged:
+            break
+    wall = time.perf_counter() - t0
+    f32.close()
+    f64.close()
+    return {"workload": "decaying_turbulence_256^3_fp32_vs_fp64", "grid": [N, N, N], "params": pr,
+            "steps_requested": args.steps, "steps_run": done, "diverged_at_or_before": diverged,
+            "sample_every": args.report_every, "wall_s": round(wall, 1), "rows": rows}
 
 
 def main():
@@ -312,6 +476,12 @@ def main():
         sec = time.perf_counter() - t0
         print(json.dumps({"workload": "copy1g", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
                           "copies": args.steps, "GBs": 8 * n * args.steps / sec / 1e9}))
+        return
+    if args.workload == "mixing768":
+        print(json.dumps(mixing_c5_bench(args)))
+        return
+    if args.workload == "turb256_report":
+        print(json.dumps(turbulence_report(args)))
         return
     if args.workload == "mixing_c5share":
         print(json.dumps(mixing_layer_bench(args, 96, 384, 384, "reacting_mixing_layer_c5_share_96x384x384")))

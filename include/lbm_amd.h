@@ -56,7 +56,18 @@ enum lbm_precision {
  * (LBM_POP_FPC) is rebuilt when it is read. */
 enum lbm_mode {
     LBM_MODE_FUSED = 0,
-    LBM_MODE_STRICT = 1   /* one kernel per reference call, explicit f and fpc */
+    LBM_MODE_STRICT = 1,  /* one kernel per reference call, explicit f and fpc */
+    /* With species only: one species population buffer streamed in place by
+     * the AA pattern (Bailey et al. 2009), half the species memory of the
+     * other modes (the 768x384x384 layer fits one 288 GB GPU).  Collide_Species
+     * writes each post-collision population where the next step reads it;
+     * every state is bitwise that of the strict calls in a fresh context (the
+     * first step after lbm_init takes the post-collision value a low species
+     * keeps as the rest state, as a fresh strict context holds it).  Between
+     * lbm_collide_species and lbm_stream the pre-collision species set is gone
+     * (reading LBM_POP_F or uploading populations there is LBM_E_STATE);
+     * TYPE_FS and TYPE_Q neighbours need LBM_MODE_STRICT. */
+    LBM_MODE_INPLACE = 2
 };
 
 /* Host-visible per-cell fields (MIXTURE members, lbm.hpp:56-88). */
@@ -132,6 +143,18 @@ const char* lbm_species_name(int k);
  * otherwise); 0 turns it off.  TYPE_S cells on the ghost layer stay isothermal
  * walls. */
 int lbm_set_conjugate(lbm_ctx* ctx, int on);
+/* The SMOOTHING variant (defines.hpp; LBM::Smoothing, lbm-smoothing.cpp:95-173,
+ * called by Init_smooting and calculate_moment_smoothing, lbm.hpp:154-174,
+ * lbm-main.cpp:133-134,178-179) as a run-time switch: lbm_init runs Init,
+ * Smoothing, calculate_moment and lbm_moments runs calculate_moment,
+ * Smoothing, calculate_moment.  Smoothing moves every species population of
+ * the TYPE_F and TYPE_I cells by alpha (the reference's 0.1) times its
+ * discrete Laplacian (fd_laplace, FD.cpp:40-50).  The reference updates in
+ * place while reading neighbours (order-dependent); this is the race-free
+ * form, every Laplacian from the populations before the call.  Multicomponent
+ * LBM_MODE_STRICT contexts only (LBM_E_UNSUPPORTED otherwise); holds one
+ * extra [27][n] population buffer while on. */
+int lbm_set_smoothing(lbm_ctx* ctx, int on, double alpha);
 
 /* Cell types, uint8 per ghost-inclusive cell (MIXTURE::type, defines.hpp:136-145). */
 int lbm_set_types(lbm_ctx* ctx, const uint8_t* types);

@@ -96,12 +96,33 @@ struct lbm_ctx {
     int* side_cell = nullptr;  // [nside]
     void* fside = nullptr;     // [NS][27][nside]
     int64_t nside = 0;
+    int64_t side_bytes = 0;  // side store and its index planes (counted in `bytes`)
     bool push_ready = false;
     // Collide_Species has pushed this step's post-collision species sets into
     // their streamed slots of buffer 1 and Streaming has not run yet: buffer 0
     // still holds the pre-collision f, the post-collision set lives at the
     // downstream slots of buffer 1 (and in the side store)
     bool pushed = false;
+    // in-place mode (LBM_MODE_INPLACE): species buffer sf[0] only, in layout
+    // aa_swapped; aa_first until the first Collide_Species after lbm_init or
+    // an upload (the kept post-collision values then come from aa_pending or
+    // the rest state); the step's list of low species pairs and their kept
+    // values (capacity low_cap, grown when a step finds more)
+    bool aa_swapped = false;
+    bool aa_first = true;
+    void* aa_pending = nullptr;  // [NS][27][n] uploaded LBM_POP_FPC for the first step
+    int* low_cell = nullptr;
+    uint8_t* low_sp = nullptr;
+    void* low_keep = nullptr;
+    unsigned* low_count = nullptr;
+    unsigned* low_count_host = nullptr;  // pinned
+    int64_t low_cap = 0;
+    int64_t low_bytes = 0;  // counted in `bytes`
+    hipEvent_t ev_save = nullptr;
+    // SMOOTHING variant (strict multicomponent): Jacobi buffer [27][n]
+    bool smoothing = false;
+    double smooth_alpha = 0.1;
+    void* smooth_tmp = nullptr;
     void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
@@ -247,6 +268,16 @@ MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
     v.side_cell = c->side_cell;
     v.fside = static_cast<P*>(c->fside);
     v.nside = c->nside;
+    if (c->cfg.mode == LBM_MODE_INPLACE) {
+        v.fo = v.f = static_cast<P*>(c->sf[0]);
+        v.swapped = c->aa_swapped ? 1 : 0;
+        v.low_cell = c->low_cell;
+        v.low_sp = c->low_sp;
+        v.low_keep = static_cast<P*>(c->low_keep);
+        v.low_count = c->low_count;
+        v.low_cap = c->low_cap;
+        v.pending = static_cast<const P*>(c->aa_pending);
+    }
     return v;
 }
 
@@ -260,6 +291,18 @@ int run_mc(lbm_ctx* c, McKernel k, const char* name, int src, int dst)
     return launch(c, name, [&] { launch_mc<double>(k, make_mview<double>(c, src, dst), c->stream); });
 }
 bool fused(const lbm_ctx* c) { return c->cfg.mode == LBM_MODE_FUSED; }
+
+// SMOOTHING of every species (lbm-smoothing.cpp:95-173) on the streamed
+// populations of buffer 0.
+int run_smoothing(lbm_ctx* c)
+{
+    if (fp32(c))
+        return launch(c, "mc_smoothing",
+                      [&] { launch_mc_smoothing<float>(make_mview<float>(c, 0, 0), c->smooth_tmp, c->smooth_alpha, c->stream); });
+    return launch(c, "mc_smoothing",
+                  [&] { launch_mc_smoothing<double>(make_mview<double>(c, 0, 0), c->smooth_tmp, c->smooth_alpha, c->stream); });
+}
+bool inplace(const lbm_ctx* c) { return c->cfg.mode == LBM_MODE_INPLACE; }
 
 // Runs `fn.template operator()<P>()` with the context's population type.
 #define DISPATCH(c, body)          \
@@ -406,9 +449,12 @@ void release(lbm_ctx* c)
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
                     c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell, c->cjc_cell,
-                    c->side_slot, c->side_cell, c->fside};
+                    c->side_slot, c->side_cell, c->fside, c->aa_pending, c->low_cell, c->low_sp,
+                    c->low_keep,  c->low_count, c->smooth_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->low_count_host) (void)hipHostFree(c->low_count_host);
+    if (c->ev_save) (void)hipEventDestroy(c->ev_save);
     if (c->net_ready) sur::mlp_free(c->net);
     if (c->work.cap) sur::work_free(c->work);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -462,9 +508,14 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         g_create_error = "unknown precision";
         return LBM_E_ARG;
     }
-    if (cfg->mode != LBM_MODE_FUSED && cfg->mode != LBM_MODE_STRICT) {
+    if (cfg->mode != LBM_MODE_FUSED && cfg->mode != LBM_MODE_STRICT && cfg->mode != LBM_MODE_INPLACE) {
         g_create_error = "unknown mode";
         return LBM_E_ARG;
+    }
+    if (cfg->mode == LBM_MODE_INPLACE && cfg->n_species == 0) {
+        g_create_error = "LBM_MODE_INPLACE streams species populations; single-component contexts use "
+                         "LBM_MODE_FUSED or LBM_MODE_STRICT";
+        return LBM_E_UNSUPPORTED;
     }
     if (cfg->n_species == 0 &&
         (!(cfg->gamma > 1.0) || !(cfg->gas_const > 0.0) || !(cfg->prtl > 0.0) || !(cfg->nu >= 0.0))) {
@@ -516,18 +567,29 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return bail("hipMalloc fields", e);
     for (int i = 0; i < LBM_FIELD_COUNT; ++i) c->fld[i] = c->fld_block + (size_t)i * n;
     (void)hipMemsetAsync(c->fld_block, 0, sizeof(double) * n * LBM_FIELD_COUNT, c->stream);
-    if ((e = hipMalloc(&c->qd_block, c->esize * n * 3)) != hipSuccess) return bail("hipMalloc fluxes", e);
-    for (int d = 0; d < 3; ++d) c->qd[d] = static_cast<char*>(c->qd_block) + (size_t)d * n * c->esize;
-    (void)hipMemsetAsync(c->qd_block, 0, c->esize * n * 3, c->stream);
+    if (cfg->n_species == 0) {  // Q_d planes of the single-component fused collide
+        if ((e = hipMalloc(&c->qd_block, c->esize * n * 3)) != hipSuccess) return bail("hipMalloc fluxes", e);
+        for (int d = 0; d < 3; ++d) c->qd[d] = static_cast<char*>(c->qd_block) + (size_t)d * n * c->esize;
+        (void)hipMemsetAsync(c->qd_block, 0, c->esize * n * 3, c->stream);
+        c->bytes += c->esize * n * 3;
+    }
     if ((e = hipMalloc(&c->tau_count, sizeof(unsigned long long))) != hipSuccess) return bail("hipMalloc", e);
     (void)hipMemsetAsync(c->tau_count, 0, sizeof(unsigned long long), c->stream);
     if (cfg->n_species > 0) {
         c->ns = cfg->n_species;
         c->units = chem::make_units(cfg->dx_si, cfg->dt_si);
         const size_t sbytes = (size_t)n * Q * c->ns * c->esize;
-        for (int b = 0; b < 2; ++b) {
+        const int nbuf = cfg->mode == LBM_MODE_INPLACE ? 1 : 2;
+        for (int b = 0; b < nbuf; ++b) {
             if ((e = hipMalloc(&c->sf[b], sbytes)) != hipSuccess) return bail("hipMalloc species populations", e);
             (void)hipMemsetAsync(c->sf[b], 0, sbytes, c->stream);
+        }
+        if (cfg->mode == LBM_MODE_INPLACE) {
+            if ((e = hipMalloc((void**)&c->low_count, sizeof(unsigned))) != hipSuccess) return bail("hipMalloc", e);
+            if ((e = hipHostMalloc((void**)&c->low_count_host, sizeof(unsigned))) != hipSuccess)
+                return bail("hipHostMalloc", e);
+            if ((e = hipEventCreateWithFlags(&c->ev_save, hipEventDisableTiming)) != hipSuccess)
+                return bail("hipEventCreate", e);
         }
         const size_t fb = c->esize * (size_t)n * c->ns * 5;
         if ((e = hipMalloc(&c->sp_block, fb)) != hipSuccess) return bail("hipMalloc species fields", e);
@@ -545,13 +607,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
             if ((e = hipMalloc((void**)&c->sums_part, sizeof(double) * MC_STATS * MC_SUM_BLOCKS)) != hipSuccess)
                 return bail("hipMalloc", e);
         }
-        c->bytes += 2 * sbytes + fb + srcb;
+        c->bytes += nbuf * sbytes + fb + srcb;
     }
     c->diag_npart = diag_partials(c->d);
     if ((e = hipMalloc(&c->diag_part, sizeof(double) * 8 * c->diag_npart)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = hipMalloc(&c->diag_out, sizeof(double) * 8)) != hipSuccess) return bail("hipMalloc", e);
-    c->bytes += 2 * pop_bytes + 2 * n + sizeof(double) * n * LBM_FIELD_COUNT + c->esize * n * 3 +
-               sizeof(double) * (8 * c->diag_npart + 9);
+    c->bytes += 2 * pop_bytes + 2 * n + sizeof(double) * n * LBM_FIELD_COUNT + sizeof(double) * (8 * c->diag_npart + 9);
     // MIXTURE defaults (lbm.hpp:70-83): rho = 1, p = T = 1/3.
     std::vector<double> tmp(n, 1.0);
     (void)hipMemcpyAsync(c->fld[LBM_FIELD_RHO], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
@@ -605,13 +666,13 @@ int push_plan(lbm_ctx* c, const uint8_t* types, const std::vector<int>& tcell, c
                     const uint8_t tu = types[at(i - cx(l), j - cy(l), k - cz(l))];
                     if (tu == TYPE_FS || tu == TYPE_Q)
                         return fail(c, LBM_E_UNSUPPORTED,
-                                    "the multicomponent fused mode handles periodic, wall (TYPE_S, TYPE_A) and "
-                                    "in/outflow neighbours; TYPE_FS/Q neighbours need LBM_MODE_STRICT");
+                                    "the multicomponent fused and in-place modes handle periodic, wall (TYPE_S, "
+                                    "TYPE_A) and in/outflow neighbours; TYPE_FS/Q neighbours need LBM_MODE_STRICT");
                     if (pullable(tu) &&
                         types[at(wrap(i - cx(l), d.nx), wrap(j - cy(l), d.ny), wrap(k - cz(l), d.nz))] != TYPE_F)
                         return fail(c, LBM_E_UNSUPPORTED,
-                                    "a periodic image of a fluid cell's neighbour is not fluid; the fused mode "
-                                    "needs LBM_MODE_STRICT here");
+                                    "a periodic image of a fluid cell's neighbour is not fluid; the fused and "
+                                    "in-place modes need LBM_MODE_STRICT here");
                     if (!pullable(tu)) push = false;
                     const uint8_t tt = types[at(i + cx(l), j + cy(l), k + cz(l))];
                     const int di = wrap(i + cx(l), d.nx), dj = wrap(j + cy(l), d.ny), dk = wrap(k + cz(l), d.nz);
@@ -735,7 +796,7 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
                     }
                 }
     std::vector<int> side;
-    if (multi(c) && fused(c))
+    if (multi(c) && (fused(c) || inplace(c)))
         if (int rc = push_plan(c, types, tcell, icell, wcell, flag, side)) return rc;
     if (!multi(c) && fused(c) && (needs_tms || !fused_ok))
         return fail(c, LBM_E_UNSUPPORTED,
@@ -780,18 +841,26 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
             *p = nullptr;
         }
         c->nside = (int64_t)side.size();
+        c->bytes -= c->side_bytes;
+        c->side_bytes = 0;
         c->push_ready = false;
         c->pushed = false;
-        if (fused(c)) {
+        c->aa_swapped = false;
+        c->aa_first = true;
+        if (fused(c) || inplace(c)) {
             std::vector<int> slot(d.n, -1);
             for (size_t s = 0; s < side.size(); ++s) slot[side[s]] = (int)s;
             HIP_TRY(c, hipMalloc((void**)&c->side_slot, sizeof(int) * d.n));
+            c->side_bytes = sizeof(int) * d.n + (sizeof(int) + Q * c->ns * c->esize) * c->nside;
+            c->bytes += c->side_bytes;
             HIP_TRY(c, hipMemcpy(c->side_slot, slot.data(), sizeof(int) * d.n, hipMemcpyHostToDevice));
             if (c->nside) {
                 const size_t sb = (size_t)c->nside * Q * c->esize;
                 HIP_TRY(c, hipMalloc((void**)&c->side_cell, sizeof(int) * side.size()));
                 HIP_TRY(c, hipMemcpy(c->side_cell, side.data(), sizeof(int) * side.size(), hipMemcpyHostToDevice));
                 HIP_TRY(c, hipMalloc(&c->fside, sb * c->ns));
+                HIP_TRY(c, hipMemsetAsync(c->fside, 0, sb * c->ns, c->stream));
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
             }
         }
     }
@@ -916,6 +985,138 @@ int read_pushed_fpc(lbm_ctx* c, double* host)
 
 }  // namespace
 
+namespace {
+
+// In-place mode: host copies of the single species buffer (values, the FP32
+// shifts added back) and of the side store.
+int aa_download(lbm_ctx* c, std::vector<double>& B, std::vector<double>& S)
+{
+    const int64_t n = c->d.n;
+    const size_t count = (size_t)n * Q * c->ns, nsz = (size_t)c->nside * Q * c->ns;
+    B.resize(count);
+    S.resize(nsz);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (fp32(c)) {
+        std::vector<float> tb(count), ts(nsz);
+        HIP_TRY(c, hipMemcpy(tb.data(), c->sf[0], count * 4, hipMemcpyDeviceToHost));
+        if (nsz) HIP_TRY(c, hipMemcpy(ts.data(), c->fside, nsz * 4, hipMemcpyDeviceToHost));
+        for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl) {
+            for (int64_t i = 0; i < n; ++i) B[pl * n + i] = (double)tb[pl * n + i] + c->mfref[pl];
+            for (int64_t s = 0; s < c->nside; ++s) S[pl * c->nside + s] = (double)ts[pl * c->nside + s] + c->mfref[pl];
+        }
+    } else {
+        HIP_TRY(c, hipMemcpy(B.data(), c->sf[0], count * 8, hipMemcpyDeviceToHost));
+        if (nsz) HIP_TRY(c, hipMemcpy(S.data(), c->fside, nsz * 8, hipMemcpyDeviceToHost));
+    }
+    return LBM_OK;
+}
+
+// In-place mode: the species populations in the natural order (post: the
+// post-collision set, else the incoming one), gathered on the host from the
+// buffer's current layout.  Incoming population l of a fluid cell x sits at
+// (l, x) in layout 0 and at (opp(l), x - e_l) in layout 1 (wrapped through
+// periodic ghosts).  The post-collision population l of a NB_PUSH cell is the
+// incoming population l of wrap(x + e_l); side cells keep theirs in the side
+// store.  Before the first in-place step the post-collision set is the
+// uploaded one, or the rest state.  Other cells read as zero.
+int aa_gather(lbm_ctx* c, bool post, double* host)
+{
+    const Dims& d = c->d;
+    const int64_t n = d.n;
+    const size_t count = (size_t)n * Q * c->ns;
+    if (post && c->aa_first) {
+        if (c->aa_pending) {
+            if (fp32(c)) {
+                std::vector<float> t(count);
+                HIP_TRY(c, hipMemcpy(t.data(), c->aa_pending, count * 4, hipMemcpyDeviceToHost));
+                for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl)
+                    for (int64_t i = 0; i < n; ++i) host[pl * n + i] = (double)t[pl * n + i] + c->mfref[pl];
+            } else {
+                HIP_TRY(c, hipMemcpy(host, c->aa_pending, count * 8, hipMemcpyDeviceToHost));
+            }
+        } else {
+            for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl)
+                for (int64_t i = 0; i < n; ++i) host[pl * n + i] = fp32(c) ? c->mfref[pl] : 0.0;
+        }
+        return LBM_OK;
+    }
+    std::vector<double> B, S;
+    if (int rc = aa_download(c, B, S)) return rc;
+    std::vector<uint8_t> type(n), flag(n);
+    std::vector<int> side_cell(c->nside);
+    HIP_TRY(c, hipMemcpy(type.data(), c->type, n, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(flag.data(), c->nbflag, n, hipMemcpyDeviceToHost));
+    if (c->nside) HIP_TRY(c, hipMemcpy(side_cell.data(), c->side_cell, sizeof(int) * c->nside, hipMemcpyDeviceToHost));
+    std::fill(host, host + count, 0.0);
+    auto at = [&](int i, int j, int k) { return ((int64_t)i * d.ny + j) * d.nz + k; };
+    auto wrap = [](int i, int m) { return i == 0 ? m - 2 : (i == m - 1 ? 1 : i); };
+    // slot of the incoming population l of cell (i, j, k) in the current layout
+    auto in_slot = [&](int i, int j, int k, int l) -> int64_t {
+        if (!c->aa_swapped) return (int64_t)l * n + at(i, j, k);
+        int si = i - cx(l), sj = j - cy(l), sk = k - cz(l);
+        if (type[at(si, sj, sk)] == TYPE_P) {
+            si = wrap(si, d.nx);
+            sj = wrap(sj, d.ny);
+            sk = wrap(sk, d.nz);
+        }
+        return (int64_t)opp(l) * n + at(si, sj, sk);
+    };
+    for (int i = 1; i < d.nx - 1; ++i)
+        for (int j = 1; j < d.ny - 1; ++j)
+            for (int k = 1; k < d.nz - 1; ++k) {
+                const int64_t ci = at(i, j, k);
+                if (type[ci] != TYPE_F) continue;
+                if (post && !(flag[ci] & NB_PUSH)) continue;  // side store below
+                for (int l = 0; l < Q; ++l) {
+                    const int64_t sl = post ? in_slot(wrap(i + cx(l), d.nx), wrap(j + cy(l), d.ny), wrap(k + cz(l), d.nz), l)
+                                            : in_slot(i, j, k, l);
+                    for (int a = 0; a < c->ns; ++a)
+                        host[((size_t)a * Q + l) * n + ci] = B[(size_t)a * Q * n + sl];
+                }
+            }
+    if (post)
+        for (int64_t s = 0; s < c->nside; ++s)
+            for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl) host[pl * n + side_cell[s]] = S[pl * c->nside + s];
+    return LBM_OK;
+}
+
+// In-place mode uploads.  LBM_POP_F: the buffer in layout 0.  LBM_POP_FPC:
+// kept for the first step (the post-collision values a low species keeps)
+// in a transient buffer, and copied into the side store.
+int aa_upload(lbm_ctx* c, bool post, const double* host)
+{
+    const int64_t n = c->d.n;
+    const size_t count = (size_t)n * Q * c->ns;
+    void* dev = c->sf[0];
+    if (post) {
+        if (!c->aa_pending) HIP_TRY(c, hipMalloc(&c->aa_pending, count * c->esize));
+        dev = c->aa_pending;
+    }
+    if (fp32(c)) {
+        std::vector<float> tmp(count);
+        for (size_t pl = 0; pl < (size_t)Q * c->ns; ++pl)
+            for (int64_t i = 0; i < n; ++i) tmp[pl * n + i] = (float)(host[pl * n + i] - c->mfref[pl]);
+        HIP_TRY(c, hipMemcpyAsync(dev, tmp.data(), count * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(dev, host, count * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    c->aa_first = true;
+    if (!post) {
+        c->aa_swapped = false;
+        return LBM_OK;
+    }
+    // side cells take their post-collision sets from the upload (k_mc_side_fill reads v.fo)
+    return DISPATCH(c, launch(c, "mc_side_fill", [&] {
+        MultiView<P> v = make_mview<P>(c, 0, 0);
+        v.fo = static_cast<P*>(c->aa_pending);
+        launch_mc<P>(McKernel::SideFill, v, c->stream);
+    }));
+}
+
+}  // namespace
+
 int lbm_set_populations(lbm_ctx* c, int which, const double* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
@@ -925,7 +1126,8 @@ int lbm_set_populations(lbm_ctx* c, int which, const double* host)
         if (c->pushed)
             return fail(c, LBM_E_STATE,
                         "populations cannot be replaced between lbm_collide_species and lbm_stream in the fused "
-                        "mode (the species are already pushed to their streamed slots)");
+                        "and in-place modes (the species are already pushed to their streamed slots)");
+        if (inplace(c) && (which == LBM_POP_F || which == LBM_POP_FPC)) return aa_upload(c, which == LBM_POP_FPC, host);
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
         void* dev = sp ? c->sf[slot] : c->pg[slot];
@@ -987,6 +1189,13 @@ int lbm_get_populations(lbm_ctx* c, int which, double* host)
     if (multi(c)) {
         const int slot = (which == LBM_POP_F || which == LBM_POP_G) ? 0 : 1;
         const bool sp = (which == LBM_POP_F || which == LBM_POP_FPC);
+        if (inplace(c) && (which == LBM_POP_F || which == LBM_POP_FPC)) {
+            if (which == LBM_POP_F && c->pushed)
+                return fail(c, LBM_E_STATE,
+                            "the in-place mode has overwritten the pre-collision species populations: LBM_POP_F "
+                            "cannot be read between lbm_collide_species and lbm_stream");
+            return aa_gather(c, which == LBM_POP_FPC, host);
+        }
         if (which == LBM_POP_FPC && c->pushed) return read_pushed_fpc(c, host);
         // the fused mode keeps no post-collision species buffer: rebuild it
         // from the streamed populations and the side store
@@ -1095,9 +1304,21 @@ int lbm_init(lbm_ctx* c)
         if (fp32(c))
             if (int rc = set_multi_reference_state(c)) return rc;
         if (int rc = run_mc(c, McKernel::Init, "mc_init", 0, 0)) return rc;
+        if (c->smoothing) {  // Init_smooting (lbm.hpp:170-174): Init, Smoothing, calculate_moment
+            if (int rc = run_smoothing(c)) return rc;
+            if (int rc = run_mc(c, McKernel::Moments, "mc_moments", 0, 0)) return rc;
+        }
         c->fields_valid = true;
         c->push_ready = false;
         c->pushed = false;
+        if (inplace(c)) {
+            // fresh-context semantics: low species keep the rest state in the first step
+            c->aa_swapped = false;
+            c->aa_first = true;
+            if (c->aa_pending) HIP_TRY(c, hipFree(c->aa_pending));
+            c->aa_pending = nullptr;
+            if (c->nside) HIP_TRY(c, hipMemsetAsync(c->fside, 0, (size_t)c->nside * Q * c->ns * c->esize, c->stream));
+        }
         return LBM_OK;
     }
     if (int rc = materialize(c)) return rc;
@@ -1110,12 +1331,84 @@ int lbm_init(lbm_ctx* c)
     return LBM_OK;
 }
 
+namespace {
+
+// (Re)allocates the low species list with room for `cap` entries.
+int aa_grow(lbm_ctx* c, int64_t cap)
+{
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (void** p : {(void**)&c->low_cell, (void**)&c->low_sp, &c->low_keep}) {
+        if (*p) HIP_TRY(c, hipFree(*p));
+        *p = nullptr;
+    }
+    c->low_cap = 0;
+    c->bytes -= c->low_bytes;
+    c->low_bytes = 0;
+    HIP_TRY(c, hipMalloc((void**)&c->low_cell, sizeof(int) * cap));
+    HIP_TRY(c, hipMalloc((void**)&c->low_sp, cap));
+    HIP_TRY(c, hipMalloc(&c->low_keep, c->esize * Q * cap));
+    c->low_cap = cap;
+    c->low_bytes = (sizeof(int) + 1 + c->esize * Q) * cap;
+    c->bytes += c->low_bytes;
+    return LBM_OK;
+}
+
+// In-place mode, first half of Collide_Species: list the low species pairs
+// and save the post-collision values they keep (before the collision
+// overwrites them); the count is read back asynchronously.
+int aa_save(lbm_ctx* c)
+{
+    if (!c->low_cell) {  // first use: room for 1/16 of all (cell, species) pairs
+        const int64_t cap = std::max<int64_t>(4096, c->d.n * c->ns / 16);
+        if (int rc = aa_grow(c, cap)) return rc;
+    }
+    int rc = DISPATCH(c, launch(c, "mc_aa_save", [&] { launch_mc_aa_save<P>(make_mview<P>(c, 0, 0), c->aa_first, c->stream); }));
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->low_count_host, c->low_count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_save, c->stream));
+    return LBM_OK;
+}
+
+// Second half: wait for the count (the chemistry and transport passes are
+// queued behind it), grow the list and save again if it overflowed, then
+// collide.  The layout flips.
+int aa_collide_species(lbm_ctx* c)
+{
+    HIP_TRY(c, hipEventSynchronize(c->ev_save));
+    const int64_t found = *c->low_count_host;
+    if (found > c->low_cap) {
+        if (int rc = aa_grow(c, found + found / 4 + 4096)) return rc;
+        int rc = DISPATCH(c, launch(c, "mc_aa_save", [&] { launch_mc_aa_save<P>(make_mview<P>(c, 0, 0), c->aa_first, c->stream); }));
+        if (rc) return rc;
+    }
+    int rc = DISPATCH(c, launch(c, "mc_collide_species_inplace",
+                                [&] { launch_mc_aa_collide<P>(make_mview<P>(c, 0, 0), c->stream); }));
+    if (rc) return rc;
+    c->aa_swapped = !c->aa_swapped;
+    c->aa_first = false;
+    c->pushed = true;
+    if (c->aa_pending) {  // consumed by the save pass (hipFree waits for it)
+        HIP_TRY(c, hipFree(c->aa_pending));
+        c->aa_pending = nullptr;
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
 int lbm_collide_species(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!multi(c)) return LBM_OK;  // Collide_Species exists only with species (lbm-main.cpp:156-158)
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (inplace(c)) {
+        if (c->pushed)
+            return fail(c, LBM_E_STATE,
+                        "the in-place mode's species are already collided into the next layout: lbm_stream must "
+                        "follow lbm_collide_species");
+        if (int rc = aa_save(c)) return rc;
+    }
     if (c->cfg.reaction) {
         int rc;
         if (c->sur_on) {
@@ -1136,6 +1429,11 @@ int lbm_collide_species(lbm_ctx* c)
         if (rc) return rc;
     }
     int rc = run_mc(c, McKernel::SpeciesTransport, "mc_species_transport", 0, 1);
+    if (inplace(c)) {
+        if (!rc) rc = aa_collide_species(c);
+        ++c->step;
+        return rc;
+    }
     const bool push = fused(c) && c->push_ready;
     if (!rc)
         rc = push ? run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1)
@@ -1172,6 +1470,12 @@ int lbm_stream(lbm_ctx* c)
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
     if (multi(c)) {
+        if (inplace(c)) {  // Collide_Species wrote the species into the next step's layout
+            if (!c->pushed)
+                return fail(c, LBM_E_STATE, "lbm_collide_species must run before lbm_stream in the in-place mode");
+            c->pushed = false;
+            return run_mc(c, McKernel::StreamEnergy, "mc_stream_energy", 1, 0);
+        }
         if (fused(c) && c->push_ready) {  // Collide_Species wrote the streamed species slots of buffer 1
             if (!c->pushed)
                 return fail(c, LBM_E_STATE, "lbm_collide_species must run before lbm_stream in the fused mode");
@@ -1227,6 +1531,26 @@ int lbm_tms_bc(lbm_ctx* c)
     return LBM_OK;
 }
 
+int lbm_set_smoothing(lbm_ctx* c, int on, double alpha)
+{
+    if (!c) return LBM_E_ARG;
+    if (on && !multi(c)) return fail(c, LBM_E_UNSUPPORTED, "SMOOTHING filters species populations (MULTICOMP only)");
+    if (on && !(c->cfg.mode == LBM_MODE_STRICT))
+        return fail(c, LBM_E_UNSUPPORTED,
+                    "SMOOTHING reads the populations of in/outflow ghosts, which only LBM_MODE_STRICT keeps as the "
+                    "reference does");
+    if (on && !(alpha >= 0.0 && std::isfinite(alpha))) return fail(c, LBM_E_ARG, "smoothing alpha must be >= 0");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (on && !c->smooth_tmp) {
+        const size_t bytes = (size_t)c->d.n * Q * c->esize;
+        HIP_TRY(c, hipMalloc(&c->smooth_tmp, bytes));
+        c->bytes += bytes;
+    }
+    c->smoothing = on != 0;
+    c->smooth_alpha = alpha;
+    return LBM_OK;
+}
+
 int lbm_set_conjugate(lbm_ctx* c, int on)
 {
     if (!c) return LBM_E_ARG;
@@ -1240,7 +1564,13 @@ int lbm_moments(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
-    if (multi(c)) return run_mc(c, McKernel::Moments, "mc_moments", 0, 0);
+    if (multi(c)) {
+        if (int rc = run_mc(c, McKernel::Moments, "mc_moments", 0, 0)) return rc;
+        if (!c->smoothing) return LBM_OK;
+        // calculate_moment_smoothing (lbm.hpp:154-157): moments, Smoothing, moments
+        if (int rc = run_smoothing(c)) return rc;
+        return run_mc(c, McKernel::Moments, "mc_moments", 0, 0);
+    }
     if (!fused(c)) {
         if (int rc = run_moments(c, 0)) return rc;
         c->fields_valid = true;

@@ -47,6 +47,20 @@ struct MultiView {
     const int* side_cell;  // [nside]
     P* fside;              // [NS][27][nside]
     int64_t nside;
+    // in-place mode: one species buffer (f == fo) streamed by the AA pattern.
+    // swapped = 0: the incoming population l of cell x sits at slot (l, x);
+    // swapped = 1: at slot (opp(l), x - e_l), the coordinates wrapped when
+    // x - e_l is a periodic ghost.  A step reads one layout and writes the other.
+    int swapped;
+    // the step's list of NB_PUSH (cell, species) pairs at the SPECIES_MIN floor
+    // and, per entry, the 27 post-collision values such a species keeps
+    // (lbm-collision.cpp:512-513), saved before the collision overwrites them
+    int* low_cell;           // [low_cap]
+    uint8_t* low_sp;         // [low_cap]
+    P* low_keep;             // [27][low_cap]
+    unsigned* low_count;     // entries found (the host grows the list when it exceeds low_cap)
+    int64_t low_cap;
+    const P* pending;        // post-collision set [NS][27][n] uploaded for the first in-place step, or null
 };
 
 // counters of the surrogate path (cumulative)
@@ -68,6 +82,25 @@ enum class McKernel { Init, Chemistry, SpeciesTransport, CollideSpecies, Collide
 
 template <typename P>
 void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);
+
+// In-place mode, Collide_Species in the layout v.swapped.  launch_mc_aa_save
+// lists the NB_PUSH (cell, species) pairs at the SPECIES_MIN floor and saves
+// the post-collision values they keep: from their streamed slots, or (first
+// step after Init or an upload) from v.pending, else the rest state (stored
+// 0).  *v.low_count is zeroed first on the stream.  launch_mc_aa_collide then
+// collides every pair (the listed ones from their saved values, side cells
+// into the side store) and writes each result where the other layout keeps it.
+template <typename P>
+void launch_mc_aa_save(const MultiView<P>& v, bool first, hipStream_t s);
+template <typename P>
+void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s);
+// The moment pass and TMS_BC read the species populations in the layout
+// v.swapped; the energy populations stream as in the fused mode.
+
+// SMOOTHING (lbm-smoothing.cpp:95-173) of every species, race-free, in the
+// strict mode's layout; tmp holds [27][n] populations of the storage type.
+template <typename P>
+void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStream_t s);
 
 // NSCBC sigma of the partially non-reflecting outflow (NSCBC_SIGMA_OUT, defines.hpp:100-102)
 constexpr double NSCBC_SIGMA_OUT = 0.25;

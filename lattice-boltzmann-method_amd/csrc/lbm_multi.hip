@@ -118,6 +118,62 @@ __device__ inline bool pullable(uint8_t t)
 }
 __device__ inline int wrap1(int i, int n) { return i == 0 ? n - 2 : (i == n - 1 ? 1 : i); }
 
+// The wrapped coordinates x - 1, x, x + 1 of a cell along each axis.
+struct Wrap3 {
+    int i[3], j[3], k[3];
+    __device__ Wrap3(const Dims& d, const Ijk& x)
+        : i{wrap1(x.i - 1, d.nx), x.i, wrap1(x.i + 1, d.nx)},
+          j{wrap1(x.j - 1, d.ny), x.j, wrap1(x.j + 1, d.ny)},
+          k{wrap1(x.k - 1, d.nz), x.k, wrap1(x.k + 1, d.nz)}
+    {
+    }
+    // cell x + s e_l (s = +1 or -1), wrapped
+    __device__ int at(const Dims& d, int l, int s) const
+    {
+        return cell(d, i[s * cx(l) + 1], j[s * cy(l) + 1], k[s * cz(l) + 1]);
+    }
+};
+
+// In-place mode: the cell whose slot opp(l) holds the incoming population l
+// of cell c in layout 1, c - e_l, wrapped when that cell is a periodic ghost
+// (walls and in/outflow ghosts keep their own slots).
+template <typename P>
+__device__ inline int aa_source(const MultiView<P>& v, int c, int l)
+{
+    const Ijk x = ijk(v.d, c);
+    const int i = x.i - cx(l), j = x.j - cy(l), k = x.k - cz(l);
+    const int s = cell(v.d, i, j, k);
+    if (v.type[s] != TYPE_P) return s;
+    return cell(v.d, wrap1(i, v.d.nx), wrap1(j, v.d.ny), wrap1(k, v.d.nz));
+}
+
+// Slot of the incoming species population (a, l) of cell c in the view's
+// layout (always layout 0 outside the in-place mode).
+template <typename P>
+__device__ inline int64_t in_slot(const MultiView<P>& v, int a, int l, int c)
+{
+    const int64_t n = v.d.n;
+    if (!v.swapped) return ((int64_t)a * Q + l) * n + c;
+    return ((int64_t)a * Q + opp(l)) * n + aa_source(v, c, l);
+}
+
+// Incoming species population (a, l) of cell c (buffer v.f, view's layout).
+// The FP32 rest-state shifts are symmetric in l <-> opp(l), so a stored value
+// means the same in either slot.
+template <typename P>
+__device__ inline double ldf_in(const MultiView<P>& v, int a, int l, int c)
+{
+    const double x = (double)v.f[in_slot(v, a, l, c)];
+    if constexpr (sizeof(P) == 4) return x + fshift(v, a, l);
+    else return x;
+}
+template <typename P>
+__device__ inline void stf_in(const MultiView<P>& v, int a, int l, int c, double x)
+{
+    if constexpr (sizeof(P) == 4) x -= fshift(v, a, l);
+    v.f[in_slot(v, a, l, c)] = (P)x;
+}
+
 // Writes the post-collision value pv of (c, l) of one species (A, B: the
 // species' planes in the source and destination buffers) to every slot the
 // pull would fill from it, and keeps the slot (c, l) of a cell whose
@@ -644,16 +700,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc_species_transport(MultiView<P> v)
         for (int d = 0; d < 3; ++d) v.tsp[((1 + d) * NS + a) * n + c] = P(acc[a][d] * 1.0 / ra[a]);
 }
 
-// grid: (xcd_grid(n), NS); species a = blockIdx.y.  PUSH: the fused mode's
-// form, each post-collision population written to its streamed slot.
-template <typename P, bool PUSH>
-__global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
+// Collide_Species of species a at cell c (lbm-collision.cpp:452-520) from
+// its incoming populations fl (storage form).  emit(l, value) receives each
+// post-collision population in storage form; keep(l) is called instead where
+// the reference leaves the post-collision value as it was (a species at the
+// SPECIES_MIN floor without a positive chemistry source, :512-513).
+template <typename P, typename Emit, typename Keep>
+__device__ __forceinline__ void species_collision(const MultiView<P>& v, int c, int a, const Ijk& x, const P fl[Q],
+                                                  Emit emit, Keep keep)
 {
-    const int c = xcd_block() * BLOCK + threadIdx.x;
-    const int a = blockIdx.y;
-    if (c >= v.d.n || v.type[c] != TYPE_F) return;
     const int64_t n = v.d.n;
-    const Ijk x = ijk(v.d, c);
     double* const* Fd = v.fld;
     const double T = Fd[LBM_FIELD_T][c];
     const double umix[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
@@ -664,11 +720,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
     const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
     const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
     const double* Tf = Fd[LBM_FIELD_T];
-    P fl[Q];
     double mom[3] = {0, 0, 0};
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        fl[l] = spop(v.f, a, l, n, c);
         const double f = (double)fl[l] + fshift(v, a, l);
         if (cx(l)) mom[0] += cx(l) * f;
         if (cy(l)) mom[1] += cy(l) * f;
@@ -696,15 +750,32 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
     feq_axes<double>(Fu, udu, theta, corr);
     feq_axes<double>(Fr, umix, theta, zero);
     const bool low = ra <= SPECIES_MIN;
-    if constexpr (!PUSH) {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            const double fe = feq_dir<double>(Fe, l, ra);
-            const double fdu = feq_dir<double>(Fu, l, ra);
-            const double fr = feq_dir<double>(Fr, l, src);
-            if (low && fr <= 0.0) continue;  // post-collision value left as it was
-            spop(v.fo, a, l, n, c) = species_post(v, a, l, fl[l], om, fe, fdu, fr);
-        }
+    for (int l = 0; l < Q; ++l) {
+        const double fe = feq_dir<double>(Fe, l, ra);
+        const double fdu = feq_dir<double>(Fu, l, ra);
+        const double fr = feq_dir<double>(Fr, l, src);
+        if (low && fr <= 0.0) keep(l);  // post-collision value left as it was
+        else emit(l, species_post(v, a, l, fl[l], om, fe, fdu, fr));
+    }
+}
+
+// grid: (xcd_grid(n), NS); species a = blockIdx.y.  PUSH: the fused mode's
+// form, each post-collision population written to its streamed slot.
+template <typename P, bool PUSH>
+__global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
+{
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    const int a = blockIdx.y;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    P fl[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fl[l] = spop(v.f, a, l, n, c);
+    auto none = [](int) {};
+    if constexpr (!PUSH) {
+        species_collision(v, c, a, x, fl, [&](int l, P pv) { spop(v.fo, a, l, n, c) = pv; }, none);
     } else {
         // A cell with NB_PUSH streams every population to a fluid cell that
         // TMS_BC leaves alone: the post-collision value a low species keeps
@@ -713,28 +784,178 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
         // then streams them.
         if (!(v.nbflag[c] & NB_PUSH)) {
             P* side = v.fside + (int64_t)a * Q * v.nside + v.side_slot[c];
-#pragma unroll
-            for (int l = 0; l < Q; ++l) {
-                const double fe = feq_dir<double>(Fe, l, ra);
-                const double fdu = feq_dir<double>(Fu, l, ra);
-                const double fr = feq_dir<double>(Fr, l, src);
-                if (low && fr <= 0.0) continue;
-                side[(int64_t)l * v.nside] = species_post(v, a, l, fl[l], om, fe, fdu, fr);
-            }
+            species_collision(v, c, a, x, fl, [&](int l, P pv) { side[(int64_t)l * v.nside] = pv; }, none);
             return;
         }
         const P* A = v.f + (int64_t)a * Q * n;
         P* B = v.fo + (int64_t)a * Q * n;
-        const int ti[3] = {wrap1(x.i - 1, v.d.nx), x.i, wrap1(x.i + 1, v.d.nx)};
-        const int tj[3] = {wrap1(x.j - 1, v.d.ny), x.j, wrap1(x.j + 1, v.d.ny)};
-        const int tk[3] = {wrap1(x.k - 1, v.d.nz), x.k, wrap1(x.k + 1, v.d.nz)};
+        const Wrap3 w(v.d, x);
+        species_collision(
+            v, c, a, x, fl, [&](int l, P pv) { B[(int64_t)l * n + w.at(v.d, l, 1)] = pv; },
+            [&](int l) {
+                const int64_t dst = (int64_t)l * n + w.at(v.d, l, 1);
+                B[dst] = A[dst];
+            });
+    }
+}
+
+// ------------------------------------------- in-place mode (AA pattern)
+// One species buffer.  A step that reads layout SW writes layout 1 - SW:
+//   SW = 0: a plain (NB_PUSH) cell reads its own slots (l, x) and writes its
+//           post-collision population l to (opp(l), x), where layout 1 keeps
+//           the incoming population l of x + e_l;
+//   SW = 1: it reads (opp(l), x - e_l) and writes (l, x + e_l), coordinates
+//           wrapped through periodic ghosts.
+// Each plain cell writes exactly the slots it reads, so the cells of a step
+// touch disjoint sets.  Side cells (near walls, in/outflow faces and TMS_BC
+// cells) collide into the side store and k_mc_side_push_aa applies the
+// strict stream's rules forwards from there.  A low species of a plain cell
+// keeps last step's post-collision value, which sits in a slot another cell
+// overwrites this step: k_mc_aa_save copies it out first and the listed pairs
+// collide in k_mc_collide_species_low.
+
+// The post-collision value a plain cell's population l kept from last step:
+// the incoming population l of x + e_l in layout SW.
+template <int SW>
+__device__ inline int64_t aa_kept_slot(const Dims& d, const Wrap3& w, int c, int l)
+{
+    return SW ? (int64_t)opp(l) * d.n + c : (int64_t)l * d.n + w.at(d, l, 1);
+}
+
+// Incoming populations of a plain cell in layout SW (F: the species' planes)
+// and the slot its post-collision population l goes to.
+template <typename P, int SW>
+__device__ inline void aa_plain_load(const Dims& d, const P* F, const Wrap3& w, int c, P fl[Q])
+{
 #pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            const double fe = feq_dir<double>(Fe, l, ra);
-            const double fdu = feq_dir<double>(Fu, l, ra);
-            const double fr = feq_dir<double>(Fr, l, src);
-            const int64_t dst = (int64_t)l * n + cell(v.d, ti[cx(l) + 1], tj[cy(l) + 1], tk[cz(l) + 1]);
-            B[dst] = (low && fr <= 0.0) ? A[dst] : species_post(v, a, l, fl[l], om, fe, fdu, fr);
+    for (int l = 0; l < Q; ++l) fl[l] = SW ? F[(int64_t)opp(l) * d.n + w.at(d, l, -1)] : F[(int64_t)l * d.n + c];
+}
+template <int SW>
+__device__ inline int64_t aa_plain_dst(const Dims& d, const Wrap3& w, int c, int l)
+{
+    return SW ? (int64_t)l * d.n + w.at(d, l, 1) : (int64_t)opp(l) * d.n + c;
+}
+
+template <typename P, int SW>
+__global__ __launch_bounds__(BLOCK) void k_mc_aa_save(MultiView<P> v, int first)
+{
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    const int a = blockIdx.y;
+    const int64_t n = v.d.n;
+    const bool listed = c < n && v.type[c] == TYPE_F && (v.nbflag[c] & NB_PUSH) && v.rhoa[a * n + c] <= SPECIES_MIN;
+    // wavefront-aggregated append
+    const unsigned long long m = __ballot(listed);
+    if (!m) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+    unsigned slot0 = 0;
+    if (lane == leader) slot0 = atomicAdd(v.low_count, (unsigned)__popcll(m));
+    slot0 = __shfl(slot0, leader);
+    if (!listed) return;
+    const int64_t e = slot0 + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    if (e >= v.low_cap) return;
+    v.low_cell[e] = c;
+    v.low_sp[e] = (uint8_t)a;
+    const P* F = v.f + (int64_t)a * Q * n;
+    const Wrap3 w(v.d, ijk(v.d, c));
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        P kept;
+        if (first) kept = v.pending ? v.pending[((int64_t)a * Q + l) * n + c] : P(0);
+        else kept = F[aa_kept_slot<SW>(v.d, w, c, l)];
+        v.low_keep[(int64_t)l * v.low_cap + e] = kept;
+    }
+}
+
+// grid: (xcd_grid(n), NS).  Plain cells (not listed) and side cells.
+template <typename P, int SW>
+__global__ __launch_bounds__(BLOCK) void k_mc_collide_species_aa(MultiView<P> v)
+{
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    const int a = blockIdx.y;
+    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    const int64_t n = v.d.n;
+    const bool plain = v.nbflag[c] & NB_PUSH;
+    if (plain && v.rhoa[a * n + c] <= SPECIES_MIN) return;  // listed: k_mc_collide_species_low
+    const Ijk x = ijk(v.d, c);
+    P* F = v.f + (int64_t)a * Q * n;
+    P fl[Q];
+    auto none = [](int) {};
+    if (plain) {
+        const Wrap3 w(v.d, x);
+        aa_plain_load<P, SW>(v.d, F, w, c, fl);
+        species_collision(v, c, a, x, fl, [&](int l, P pv) { F[aa_plain_dst<SW>(v.d, w, c, l)] = pv; }, none);
+        return;
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) fl[l] = v.f[in_slot(v, a, l, c)];
+    P* side = v.fside + (int64_t)a * Q * v.nside + v.side_slot[c];
+    species_collision(v, c, a, x, fl, [&](int l, P pv) { side[(int64_t)l * v.nside] = pv; }, none);
+}
+
+// The listed (cell, species) pairs: grid-stride over the list.
+template <typename P, int SW>
+__global__ __launch_bounds__(BLOCK) void k_mc_collide_species_low(MultiView<P> v)
+{
+    const int64_t total = min((int64_t)*v.low_count, v.low_cap);
+    const int64_t n = v.d.n;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < total; e += (int64_t)gridDim.x * BLOCK) {
+        const int c = v.low_cell[e];
+        const int a = v.low_sp[e];
+        const Ijk x = ijk(v.d, c);
+        const Wrap3 w(v.d, x);
+        P* F = v.f + (int64_t)a * Q * n;
+        P fl[Q];
+        aa_plain_load<P, SW>(v.d, F, w, c, fl);
+        species_collision(
+            v, c, a, x, fl, [&](int l, P pv) { F[aa_plain_dst<SW>(v.d, w, c, l)] = pv; },
+            [&](int l) { F[aa_plain_dst<SW>(v.d, w, c, l)] = v.low_keep[(int64_t)l * v.low_cap + e]; });
+    }
+}
+
+// Side cells: the strict stream's rules (lbm-streaming.cpp:3-119) applied
+// forwards from the side store into layout 1 - SW.  Population m of cell x
+// goes to the incoming slot (m, z) of z = wrap(x + e_m) when z is fluid and
+// pulls through a plain move; bounces back into the incoming slot (opp(m), x)
+// at a TYPE_S / TYPE_A neighbour; an in/outflow neighbour takes nothing.  A
+// population of x whose upstream cell is an in/outflow ghost keeps its value.
+// grid: (ceil(nside / BLOCK), NS).
+template <typename P, int SW>
+__global__ __launch_bounds__(BLOCK) void k_mc_side_push_aa(MultiView<P> v)
+{
+    const int64_t s = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= v.nside) return;
+    const int c = v.side_cell[s];
+    const int a = blockIdx.y;
+    const int64_t n = v.d.n;
+    const Ijk x = ijk(v.d, c);
+    const P* S = v.fside + (int64_t)a * Q * v.nside;
+    P* F = v.f + (int64_t)a * Q * n;
+    for (int m = 0; m < Q; ++m) {
+        const P pv = S[(int64_t)m * v.nside + s];
+        const int t = cell(v.d, x.i + cx(m), x.j + cy(m), x.k + cz(m));
+        const uint8_t tt = v.type[t];
+        if (tt == TYPE_S || tt == TYPE_A) {
+            // the incoming population opp(m) of x: slot (m, t) in layout 1, (opp(m), x) in layout 0
+            if (SW) F[(int64_t)opp(m) * n + c] = pv;
+            else F[(int64_t)m * n + t] = pv;
+        } else if (pullable(tt)) {
+            const int di = wrap1(x.i + cx(m), v.d.nx), dj = wrap1(x.j + cy(m), v.d.ny),
+                      dk = wrap1(x.k + cz(m), v.d.nz);
+            const int z = cell(v.d, di, dj, dk);
+            if (v.type[z] == TYPE_F && pullable(v.type[cell(v.d, di - cx(m), dj - cy(m), dk - cz(m))])) {
+                // the incoming population m of z: slot (opp(m), x) in layout 1, (m, z) in layout 0
+                if (SW) F[(int64_t)m * n + z] = pv;
+                else F[(int64_t)opp(m) * n + c] = pv;
+            }
+        }
+    }
+    for (int l = 0; l < Q; ++l) {
+        const int y = cell(v.d, x.i - cx(l), x.j - cy(l), x.k - cz(l));
+        const uint8_t ty = v.type[y];
+        if (ty == TYPE_O || ty == TYPE_O_C || ty == TYPE_I || ty == TYPE_I_C) {
+            // incoming population l of x: slot (l, x) in layout 0, (opp(l), y) in layout 1
+            if (SW) F[(int64_t)l * n + c] = F[(int64_t)opp(l) * n + y];
+            else F[(int64_t)opp(l) * n + y] = F[(int64_t)l * n + c];
         }
     }
 }
@@ -1109,12 +1330,24 @@ __device__ inline void cholesky_solve3_rcp(float L[NTRI], float B[NS][3])
 // waves per SIMD without scratch: its per-species sums go through LDS, the
 // Maxwell-Stefan solve is binary32 and the species planes are written before
 // the temperature iteration (2.10 -> 1.49 ms on the 256x128x128 layer).
-template <typename P>
+// SW: the in-place mode's layout 1 (species populations read through their
+// sources, with the wrapped neighbour coordinates when every neighbour is
+// fluid or periodic).
+template <typename P, int SW>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P) == 8 ? 1 : 2, sizeof(P) == 8 ? 1 : 8))) void k_mc_moments(MultiView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= v.d.n || v.type[c] != TYPE_F) return;
     const int64_t n = v.d.n;
+    const bool simple = SW && (v.nbflag[c] & NB_SIMPLE);
+    const Wrap3 w(v.d, ijk(v.d, c));
+    auto species_f = [&](int a, int l) -> double {
+        if constexpr (!SW) return ldf(v, v.f, a, l, c);
+        const int src = simple ? w.at(v.d, l, -1) : aa_source(v, c, l);
+        const double x = (double)v.f[((int64_t)a * Q + opp(l)) * n + src];
+        if constexpr (sizeof(P) == 4) return x + fshift(v, a, l);
+        else return x;
+    };
     double* const* Fd = v.fld;
     // Cantera state of the previous moments (old T, rho, composition)
     const double rho_old = Fd[LBM_FIELD_RHO][c];
@@ -1155,7 +1388,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
         double r = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
-            const double f = ldf(v, v.f, a, l, c);
+            const double f = species_f(a, l);
             r += f;
             if (cx(l)) mx += cx(l) * f;
             if (cy(l)) my += cy(l) * f;
@@ -1337,7 +1570,7 @@ __device__ __noinline__ void mc_moment_point(const MultiView<P>& v, int c)
     for (int a = 0; a < NS; ++a) {
         double r = 0, mx = 0, my = 0, mz = 0;
         for (int l = 0; l < Q; ++l) {
-            const double f = ldf(v, v.f, a, l, c);
+            const double f = ldf_in(v, a, l, c);
             r += f;
             if (cx(l)) mx += cx(l) * f;
             if (cy(l)) my += cy(l) * f;
@@ -1592,7 +1825,7 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
         double r = 0, m0 = 0, m1 = 0, m2 = 0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out[a]) : ldf(v, v.f, a, q, c);
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out[a]) : ldf_in(v, a, q, c);
             r += f;
             if (cx(q)) m0 += cx(q) * f;
             if (cy(q)) m1 += cy(q) * f;
@@ -1642,8 +1875,8 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
         for (int q = 0; q < Q; ++q) {
             const double fo = feq_dir<double>(Fo, q, ra_out[a]);
             const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
-            const double fin = ldf(v, v.f, a, q, c);
-            stf(v, v.f, a, q, c, (unknown >> q & 1u) ? 2 * fo - flc : fo + fin - flc);
+            const double fin = ldf_in(v, a, q, c);
+            stf_in(v, a, q, c, (unknown >> q & 1u) ? 2 * fo - flc : fo + fin - flc);
         }
     }
     const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
@@ -1778,7 +2011,7 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
         double r = 0, m0 = 0, m1 = 0, m2 = 0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fi, q, ra_in[a]) : ldf(v, v.f, a, q, c);
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fi, q, ra_in[a]) : ldf_in(v, a, q, c);
             r += f;
             if (cx(q)) m0 += cx(q) * f;
             if (cy(q)) m1 += cy(q) * f;
@@ -1831,8 +2064,8 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
             const double fi = feq_dir<double>(Fi, q, ra_in[a]);
             const double ft = feq_dir<double>(Fi, q, ra_loc[a]);
             const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
-            const double fin = ldf(v, v.f, a, q, c);
-            stf(v, v.f, a, q, c, (unknown >> q & 1u) ? ft + fi - flc : ft + fin - flc);
+            const double fin = ldf_in(v, a, q, c);
+            stf_in(v, a, q, c, (unknown >> q & 1u) ? ft + fi - flc : ft + fin - flc);
         }
     }
     const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
@@ -1878,7 +2111,7 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
     double ra_bb[NS], rho_bb = 0;
     for (int a = 0; a < NS; ++a) {
         double r = 0;
-        for (int q = 0; q < Q; ++q) r += ldf(v, v.f, a, q, c);
+        for (int q = 0; q < Q; ++q) r += ldf_in(v, a, q, c);
         ra_bb[a] = r;
         rho_bb += r;
     }
@@ -1931,7 +2164,7 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
         double r = 0, m0 = 0, m1 = 0, m2 = 0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const double f = (unknown >> q & 1u) ? feq_dir<double>(Ft, q, ra_bb[a]) : ldf(v, v.f, a, q, c);
+            const double f = (unknown >> q & 1u) ? feq_dir<double>(Ft, q, ra_bb[a]) : ldf_in(v, a, q, c);
             r += f;
             if (cx(q)) m0 += cx(q) * f;
             if (cy(q)) m1 += cy(q) * f;
@@ -1980,8 +2213,8 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
         for (int q = 0; q < Q; ++q) {
             const double ft = feq_dir<double>(Ft, q, ra_bb[a]);
             const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
-            const double fin = ldf(v, v.f, a, q, c);
-            stf(v, v.f, a, q, c, (unknown >> q & 1u) ? 2 * ft - flc : ft + fin - flc);
+            const double fin = ldf_in(v, a, q, c);
+            stf_in(v, a, q, c, (unknown >> q & 1u) ? 2 * ft - flc : ft + fin - flc);
         }
     }
     const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
@@ -1998,6 +2231,59 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
 // Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
 // the mean state the FP32 deviations are taken from, and the minima of X_a
 // over the TYPE_F cells and the inflow ghosts (TYPE_I, TYPE_I_C).
+// ------------------------------------------------------------- SMOOTHING
+// LBM::Smoothing (lbm-smoothing.cpp:95-173) of species a, race-free: every
+// population l of a TYPE_F or TYPE_I cell becomes f + alpha * lap(f) with
+// fd_laplace (FD.cpp:40-50): per axis (r - 2c) + l, 0 when either neighbour
+// is TYPE_A / TYPE_FS / TYPE_S / TYPE_P; an axis whose neighbours fall
+// outside the array takes them as 0 of TYPE_F.  The results go to tmp
+// [27][n] (Jacobi: the Laplacians read the populations before the call) and
+// k_mc_smooth_store copies them back.  Layout 0 only (strict mode).
+__device__ inline bool smooth_dead(uint8_t t) { return t == TYPE_A || t == TYPE_FS || t == TYPE_S || t == TYPE_P; }
+
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_smooth(MultiView<P> v, int a, P* tmp, double alpha)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n) return;
+    const uint8_t t = v.type[c];
+    if (t != TYPE_F && t != TYPE_I) return;
+    const Ijk x = ijk(v.d, c);
+    const int pos[3] = {x.i, x.j, x.k}, ext[3] = {v.d.nx, v.d.ny, v.d.nz};
+    const int stride[3] = {v.d.ny * v.d.nz, v.d.nz, 1};
+    for (int l = 0; l < Q; ++l) {
+        const double fc = ldf(v, v.f, a, l, c);
+        double lap = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double fl = 0.0, fr = 0.0;
+            uint8_t tl = TYPE_F, tr = TYPE_F;
+            if (pos[d] - 1 >= 0 && pos[d] + 1 < ext[d]) {
+                fl = ldf(v, v.f, a, l, c - stride[d]);
+                fr = ldf(v, v.f, a, l, c + stride[d]);
+                tl = v.type[c - stride[d]];
+                tr = v.type[c + stride[d]];
+            }
+            const double d2 = (smooth_dead(tl) || smooth_dead(tr)) ? 0.0 : (fr - 2.0 * fc + fl) / (1 * 1);
+            lap = d == 0 ? d2 : lap + d2;
+        }
+        double r = fc + alpha * lap;
+        if constexpr (sizeof(P) == 4) r -= fshift(v, a, l);
+        tmp[(int64_t)l * v.d.n + c] = (P)r;
+    }
+}
+
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_smooth_store(MultiView<P> v, int a, const P* tmp)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= v.d.n) return;
+    const uint8_t t = v.type[c];
+    if (t != TYPE_F && t != TYPE_I) return;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) spop(v.f, a, l, v.d.n, c) = tmp[(int64_t)l * v.d.n + c];
+}
+
 template <typename S>
 __global__ __launch_bounds__(BLOCK) void k_mc_state_sums(const uint8_t* type, const double* T, const double* p,
                                                          const S* X, int64_t n, double* part)
@@ -2085,7 +2371,8 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
         hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS + 1), dim3(BLOCK), 0, s, v, 0);
         break;
     case McKernel::Moments:
-        hipLaunchKernelGGL(k_mc_moments<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        if (v.swapped) hipLaunchKernelGGL((k_mc_moments<P, 1>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        else hipLaunchKernelGGL((k_mc_moments<P, 0>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
         break;
     }
 }
@@ -2118,6 +2405,40 @@ static int cu_count()
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
 template <typename P>
+void launch_mc_aa_save(const MultiView<P>& v, bool first, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, v.low_count);
+    const dim3 g(xcd_grid(v.d.n, BLOCK), NS);
+    if (v.swapped) hipLaunchKernelGGL((k_mc_aa_save<P, 1>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
+    else hipLaunchKernelGGL((k_mc_aa_save<P, 0>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
+}
+
+template <typename P>
+void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStream_t s)
+{
+    for (int a = 0; a < NS; ++a) {
+        hipLaunchKernelGGL(k_mc_smooth<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, a, static_cast<P*>(tmp), alpha);
+        hipLaunchKernelGGL(k_mc_smooth_store<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, a, static_cast<const P*>(tmp));
+    }
+}
+
+template <typename P>
+void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s)
+{
+    const dim3 g(xcd_grid(v.d.n, BLOCK), NS), gl(4 * cu_count()), gs((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS);
+    if (v.swapped) {
+        hipLaunchKernelGGL((k_mc_collide_species_aa<P, 1>), g, dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_mc_collide_species_low<P, 1>), gl, dim3(BLOCK), 0, s, v);
+        if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 1>), gs, dim3(BLOCK), 0, s, v);
+    } else {
+        hipLaunchKernelGGL((k_mc_collide_species_aa<P, 0>), g, dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_mc_collide_species_low<P, 0>), gl, dim3(BLOCK), 0, s, v);
+        if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 0>), gs, dim3(BLOCK), 0, s, v);
+    }
+}
+
+
+template <typename P>
 void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, uint16_t* X, float* Y, uint8_t* flag,
                              int64_t rows, unsigned* list, unsigned* count, hipStream_t s)
 {
@@ -2135,6 +2456,12 @@ void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsi
 
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
 template void launch_mc<float>(McKernel, const MultiView<float>&, hipStream_t);
+template void launch_mc_smoothing<double>(const MultiView<double>&, void*, double, hipStream_t);
+template void launch_mc_smoothing<float>(const MultiView<float>&, void*, double, hipStream_t);
+template void launch_mc_aa_save<double>(const MultiView<double>&, bool, hipStream_t);
+template void launch_mc_aa_save<float>(const MultiView<float>&, bool, hipStream_t);
+template void launch_mc_aa_collide<double>(const MultiView<double>&, hipStream_t);
+template void launch_mc_aa_collide<float>(const MultiView<float>&, hipStream_t);
 template void launch_mc_tms<float>(const MultiView<float>&, const int*, const int*, int, hipStream_t);
 template void launch_mc_sur_chemistry<float>(const MultiView<float>&, const sur::Mlp&, uint16_t*, float*,
                                              uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);

@@ -3,7 +3,9 @@
 // the reference keeps its shape: construct, set types and primitive fields,
 // run(nstep, tout); LBM::loop keeps the reference's five step calls
 // (lbm-main.cpp:149-196).  Per-cell access lb.mixture[i][j][k].x becomes
-// bulk field access (field(LBM_FIELD_U)[idx(i, j, k)]) followed by upload().
+// bulk field access (field(LBM_FIELD_U)[idx(i, j, k)]) followed by upload();
+// lb.species[a][i][j][k].X of the MULTICOMP build becomes
+// species_field(LBM_SP_X)[a * n + idx(i, j, k)].
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -43,6 +45,24 @@ public:
         fields_[LBM_FIELD_P].assign(n, 1.0 / 3.0);
         fields_[LBM_FIELD_T].assign(n, 1.0 / 3.0);
     }
+    // LBM(Nx, Ny, Nz, species) of the MULTICOMP build (lbm-main.cpp:38-118)
+    // with units.set_m_kg_s(dx_si, dt_si) (units.hpp:32-44) and the REACTION
+    // switch; the species are the library mechanism's, in its order
+    // (lbm_mechanism, lbm_species_name).  Mixture defaults as LBM() (lbm.hpp:70-83).
+    LBM(int Nx, int Ny, int Nz, double dx_si, double dt_si, bool reaction, int mode = LBM_MODE_FUSED,
+        int precision = LBM_FP64, int device = 0)
+        : LBM(Nx, Ny, Nz, 0.0, mode, precision, device)
+    {
+        int ns = 0;
+        lbm_mechanism(&ns);
+        cfg_.n_species = ns;
+        cfg_.dx_si = dx_si;
+        cfg_.dt_si = dt_si;
+        cfg_.reaction = reaction ? 1 : 0;
+        for (int a = 0; a < ns; ++a) species_names.push_back(lbm_species_name(a));
+        const size_t n = (size_t)this->Nx * this->Ny * this->Nz;
+        species_.assign(5, std::vector<double>(n * ns, 0.0));
+    }
     ~LBM()
     {
         if (ctx_) lbm_destroy(ctx_);
@@ -66,13 +86,25 @@ public:
     void set_gasconst(double v) { cfg_.gas_const = v; }
     void set_gamma(double v) { cfg_.gamma = v; }
     void set_prtl(double v) { cfg_.prtl = v; }
+    // rest state of the FP32 shifted populations (no reference counterpart:
+    // the reference stores FP64 values)
+    void set_reference_state(double rho, double T)
+    {
+        cfg_.ref_rho = rho;
+        cfg_.ref_T = T;
+    }
     // the reference's #define CONJUGATE (defines.hpp:62-63) as a run-time switch;
     // takes effect at the next upload()
     void set_conjugate(bool on) { conjugate_ = on; }
     int get_step() const { return step; }
 
     size_t idx(int i, int j, int k) const { return ((size_t)i * Ny + j) * Nz + k; }
+    size_t ncell() const { return (size_t)Nx * Ny * Nz; }
     std::vector<double>& field(int f) { return fields_[f]; }
+    // species planes [NS][n] of enum lbm_species_field (MULTICOMP only)
+    std::vector<double>& species_field(int f) { return species_.at(f); }
+    int n_species() const { return cfg_.n_species; }
+    std::vector<std::string> species_names;
     std::vector<uint8_t> type;
 
     // ---- host <-> device
@@ -83,10 +115,21 @@ public:
         check(lbm_set_types(ctx_, type.data()), "lbm_set_types");
         const int prim[] = {LBM_FIELD_RHO, LBM_FIELD_U, LBM_FIELD_V, LBM_FIELD_W, LBM_FIELD_P, LBM_FIELD_T};
         for (int f : prim) check(lbm_set_field(ctx_, f, fields_[f].data()), "lbm_set_field");
+        if (cfg_.n_species) check(lbm_set_species_field(ctx_, LBM_SP_X, species_[LBM_SP_X].data()), "lbm_set_species_field");
     }
     void download()
     {
         for (int f = 0; f < LBM_FIELD_COUNT; ++f) check(lbm_get_field(ctx_, f, fields_[f].data()), "lbm_get_field");
+        for (int f = 0; f < (cfg_.n_species ? 5 : 0); ++f)
+            check(lbm_get_species_field(ctx_, f, species_[f].data()), "lbm_get_species_field");
+    }
+    // a population set of enum lbm_pop, [27][n] (with species [NS][27][n])
+    std::vector<double> populations(int which)
+    {
+        const bool sp = cfg_.n_species && (which == LBM_POP_F || which == LBM_POP_FPC);
+        std::vector<double> out(ncell() * 27 * (sp ? cfg_.n_species : 1));
+        check(lbm_get_populations(ctx_, which, out.data()), "lbm_get_populations");
+        return out;
     }
     lbm_diag diagnostics()
     {
@@ -163,6 +206,7 @@ private:
     lbm_ctx* ctx_ = nullptr;
     bool conjugate_ = false;
     std::vector<std::vector<double>> fields_;
+    std::vector<std::vector<double>> species_;
     int Nx, Ny, Nz;
     int step = 0;
 };

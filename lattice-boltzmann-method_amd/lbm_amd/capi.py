@@ -23,7 +23,7 @@ MECHANISMS = tuple(LIB_NAMES)
 LIB_PATH = os.path.join(_PKG_DIR, LIB_NAMES["h2air"])
 
 LBM_FP64, LBM_FP32_SHIFTED = 0, 1
-LBM_MODE_FUSED, LBM_MODE_STRICT = 0, 1
+LBM_MODE_FUSED, LBM_MODE_STRICT, LBM_MODE_INPLACE = 0, 1, 2
 POP_F, POP_FPC, POP_G, POP_GPC = 0, 1, 2, 3
 
 FIELDS = ["rho", "u", "v", "w", "p", "T", "rhoe", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "qx", "qy", "qz", "HRR"]
@@ -39,7 +39,7 @@ EXPORTS = [
     "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
     "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
     "lbm_surrogate_forward", "lbm_surrogate_gradient", "lbm_write_vtk", "lbm_write_restart", "lbm_read_restart",
-    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate",
+    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate", "lbm_set_smoothing",
 ]
 
 LBM_VTK_SI, LBM_VTK_MASS_FRACTIONS = 1, 2
@@ -154,6 +154,7 @@ def load_library(path=None, mechanism="h2air"):
     L.lbm_surrogate_forward.argtypes = [vp, fp, i64, fp]
     L.lbm_surrogate_gradient.argtypes = [vp, fp, fp, i64, fp, fp]
     L.lbm_set_conjugate.argtypes = [vp, i32]
+    L.lbm_set_smoothing.argtypes = [vp, i32, ctypes.c_double]
     L.lbm_mechanism.argtypes = [ctypes.POINTER(i32)]
     L.lbm_mechanism.restype = ctypes.c_char_p
     L.lbm_species_name.argtypes = [i32]
@@ -301,6 +302,13 @@ class Lattice:
         """The CONJUGATE variant (defines.hpp:62-63): interior TYPE_S cells
         conduct heat and couple to the gas through TMS_BC (strict mode)."""
         self._check(self._L.lbm_set_conjugate(self._h, 1 if on else 0), "set_conjugate")
+
+    def set_smoothing(self, on=True, alpha=0.1):
+        """The SMOOTHING variant (lbm-smoothing.cpp:95-173; Init_smooting and
+        calculate_moment_smoothing of lbm.hpp:154-174): species populations
+        filtered by alpha times their Laplacian around Init and each moment
+        pass (strict multicomponent contexts)."""
+        self._check(self._L.lbm_set_smoothing(self._h, 1 if on else 0, float(alpha)), "set_smoothing")
 
     # -- the reference's step calls
     def Init(self):

@@ -513,6 +513,53 @@ class MultiState:
         self.T[c] = U_.temp(T_si).astype(np.float64)
         self.p[c] = U_.p(m.pressure(T_si, rho_si, Y)).astype(np.float64)
 
+    # --------------------------------------------------------- SMOOTHING
+    def smoothing(self, alpha=0.1):
+        """LBM::Smoothing (lbm-smoothing.cpp:95-173): every species population
+        of each TYPE_F and TYPE_I cell moves by alpha times its discrete
+        Laplacian, fd_laplace (FD.cpp:40-50) with fd_central_2der (FD.cpp:
+        25-37): per axis (r - 2c + l) / dx^2 (dx = 1), 0 when either neighbour
+        is TYPE_A, TYPE_FS, TYPE_S or TYPE_P.  An axis without both
+        neighbours in the array (a cell on the array's edge) takes 0 for
+        their values and TYPE_F (0) for their types, so its term is -2c.
+        The reference updates f in place while its loop reads neighbours, an
+        order-dependent (and, under OpenMP, racy) result; this restatement is
+        the race-free (Jacobi) form: every Laplacian reads the populations as
+        they were before the call."""
+        nx, ny, nz = self.dims
+        typ = self.type.reshape(self.dims)
+        f = self.f.reshape((self.ns, 27) + self.dims)
+        out = f.copy()
+        zero_types = np.isin(typ, (TYPE_A, TYPE_FS, TYPE_S, TYPE_P))
+        lap = np.zeros_like(f)
+        for ax, m in enumerate((nx, ny, nz)):
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            mid = [slice(None)] * 3
+            lo[ax], mid[ax], hi[ax] = slice(0, m - 2), slice(1, m - 1), slice(2, m)
+            d2 = np.zeros_like(f)
+            sl = (Ellipsis,) + tuple(mid)
+            d2[sl] = (f[(Ellipsis,) + tuple(hi)] - 2.0 * f[sl]) + f[(Ellipsis,) + tuple(lo)]
+            dead = np.zeros(self.dims, bool)
+            dead[tuple(mid)] = zero_types[tuple(lo)] | zero_types[tuple(hi)]
+            d2[..., dead] = 0.0
+            # array edges: both neighbour values 0, types TYPE_F
+            for e in (0, m - 1):
+                edge = [slice(None)] * 3
+                edge[ax] = e
+                d2[(Ellipsis,) + tuple(edge)] = 0.0 - 2.0 * f[(Ellipsis,) + tuple(edge)] + 0.0
+            lap = lap + d2 if ax else d2
+        # fd_laplace sums the three axes in the order x, y, z
+        cells = (typ == TYPE_F) | (typ == TYPE_I)
+        out[..., cells] = f[..., cells] + alpha * lap[..., cells]
+        self.f = out.reshape(self.ns, 27, -1)
+
+    def calculate_moment_smoothing(self, alpha=0.1):
+        """lbm.hpp:154-157: moments, Smoothing, moments."""
+        self.moments()
+        self.smoothing(alpha)
+        self.moments()
+
     def step(self, reaction=True):
         self.collide_species(reaction)
         self.collide()

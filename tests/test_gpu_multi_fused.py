@@ -21,55 +21,13 @@ from lbm_amd import cases
 from lbm_amd.capi import (LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_STRICT, POP_F, POP_FPC, POP_G,
                           POP_GPC, LbmError, MixtureLattice)
 from lbm_amd.cases import TYPE_A, TYPE_FS
+from mode_compare import assert_same, compare as _compare, make, snapshot
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = ("rho", "u", "v", "w", "T", "p", "rhoe", "HRR")
-SPECIES_FIELDS = ("X", "rho", "u", "v", "w")
-
-
-def make(fl, mode, precision=LBM_FP64, reaction=True):
-    nx, ny, nz = (s - 2 for s in fl["type"].shape)
-    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=reaction, mode=mode, precision=precision)
-    lat.load_case(fl)
-    lat.Init()
-    lat.calculate_moment()
-    return lat
-
-
-def snapshot(lat, F):
-    ns = len(lat.SPECIES)
-    out = {}
-    for k in FIELDS:
-        out[k] = lat.get_field(k).ravel()[F]
-    for k in SPECIES_FIELDS:
-        out["sp_" + k] = lat.get_species_field(k).reshape(ns, -1)[:, F]
-    for which, nm in ((POP_F, "f"), (POP_FPC, "fpc")):
-        out[nm] = lat.get_populations(which).reshape(ns, 27, -1)[:, :, F]
-    for which, nm in ((POP_G, "g"), (POP_GPC, "gpc")):
-        out[nm] = lat.get_populations(which).reshape(27, -1)[:, F]
-    return out
-
-
-def assert_same(a, b, where):
-    for k in a:
-        if not np.array_equal(a[k], b[k], equal_nan=True):
-            d = np.abs(a[k] - b[k])
-            raise AssertionError(f"{where}: {k} differs (max |diff| {np.nanmax(d):.3e}, "
-                                 f"{int(np.count_nonzero(d))} values)")
-
 
 def compare(fl, steps, precision=LBM_FP64, reaction=True, tag=""):
-    print(f"fused vs strict {tag} grid {fl['type'].shape} steps {steps} precision {precision}", flush=True)
-    F = fl["type"].ravel() == 0
-    strict = make(fl, LBM_MODE_STRICT, precision, reaction)
-    fused = make(fl, LBM_MODE_FUSED, precision, reaction)
-    for step in range(1, steps + 1):
-        strict.step(1)
-        fused.step(1)
-        assert_same(snapshot(fused, F), snapshot(strict, F), f"{tag} step {step}")
-    strict.close()
-    fused.close()
+    _compare(fl, steps, LBM_MODE_FUSED, precision, reaction, tag)
 
 
 @pytest.mark.parametrize("reaction", [False, True])
