@@ -109,7 +109,9 @@ typedef struct {
     double energy;       /* sum rhoe */
     double kinetic;      /* sum rho |u|^2 (no 1/2, as output.cpp:232-233) */
     double enstrophy;    /* output.cpp:235-262 */
-    double max_mach;     /* max |u| / sqrt(gamma p / rho) (gamma of the config, 1.4 if unset) */
+    double max_mach;     /* max |u| / sqrt(gamma p / rho): gamma of the config (1.4 if unset) for a
+                            single component; with species each cell's frozen mixture gamma
+                            c_p / (c_p - R / W) at its T and composition */
     int64_t nan_count;   /* TYPE_F cells with non-finite rho or T */
     int64_t tau_warnings;/* collisions with 1/omega <= 0.5 so far (lbm-collision.cpp:33-34 prints instead) */
     int64_t fallback_count; /* cell updates whose chemistry the plain integrator redid

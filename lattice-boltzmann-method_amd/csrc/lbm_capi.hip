@@ -1605,8 +1605,14 @@ int lbm_diagnostics(lbm_ctx* c, lbm_diag* out)
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (int rc = materialize(c)) return rc;
     if (int rc = launch(c, "diagnostics", [&] {
-            launch_diag(c->d, c->type, c->fld, c->cfg.gamma > 1.0 ? c->cfg.gamma : 1.4, c->diag_part,
-                        c->diag_npart, c->diag_out, c->stream);
+            launch_diag_partials(c->d, c->type, c->fld, c->cfg.gamma > 1.0 ? c->cfg.gamma : 1.4, c->diag_part,
+                                 c->stream);
+            // with species: the Mach number with each cell's mixture gamma
+            if (multi(c)) {
+                if (fp32(c)) launch_mc_diag_mach<float>(make_mview<float>(c, 0, 0), c->diag_part, c->stream);
+                else launch_mc_diag_mach<double>(make_mview<double>(c, 0, 0), c->diag_part, c->stream);
+            }
+            launch_diag_final(c->diag_part, c->diag_npart, c->diag_out, c->stream);
         }))
         return rc;
     double h[8] = {};

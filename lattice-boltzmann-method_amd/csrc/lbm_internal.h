@@ -80,5 +80,11 @@ template <typename P> void launch_tms_conj(const SingleView<P>& v, const int* ce
 void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
                  double* out8, hipStream_t s);
 int diag_partials(const Dims& d);
+// The two halves of launch_diag: per-block partials [npart][6] (sums of
+// rho, rho e, rho |u|^2, rho |omega|^2, non-finite cells; max Mach), then the
+// fixed-order final reduction into out8[0..5].  One block per 256 cells.
+void launch_diag_partials(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials,
+                          hipStream_t s);
+void launch_diag_final(const double* partials, int npart, double* out8, hipStream_t s);
 
 }  // namespace lbm

@@ -102,6 +102,11 @@ void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s);
 template <typename P>
 void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStream_t s);
 
+// Max Mach number with each cell's mixture gamma into slot 5 of the
+// diagnostics partials of launch_diag_partials (same block decomposition).
+template <typename P>
+void launch_mc_diag_mach(const MultiView<P>& v, double* partials, hipStream_t s);
+
 // NSCBC sigma of the partially non-reflecting outflow (NSCBC_SIGMA_OUT, defines.hpp:100-102)
 constexpr double NSCBC_SIGMA_OUT = 0.25;
 

@@ -2231,6 +2231,42 @@ __global__ __launch_bounds__(64) void k_mc_tms_wall(MultiView<P> v, const int* c
 // Per-block sums over TYPE_F cells of 1, T, p and X_a (lattice units), for
 // the mean state the FP32 deviations are taken from, and the minima of X_a
 // over the TYPE_F cells and the inflow ghosts (TYPE_I, TYPE_I_C).
+// ------------------------------------------------------------ diagnostics
+// Max Mach number of the multicomponent state with each cell's frozen
+// gamma = c_p / (c_p - R_univ / W_mix) at its T and composition (the sound
+// speed of impose_nscbc.cpp:358-369): |u| / sqrt(gamma p / rho).  Overwrites
+// slot 5 of the diagnostics partials, one block per 256 cells as k_diag.
+template <typename P>
+__global__ __launch_bounds__(BLOCK) void k_mc_diag_mach(MultiView<P> v, double* part)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    double mach = 0.0;
+    if (c < v.d.n && v.type[c] == TYPE_F) {
+        const int64_t n = v.d.n;
+        double* const* Fd = v.fld;
+        const double rho = Fd[LBM_FIELD_RHO][c];
+        double Y[NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) Y[a] = (double)v.rhoa[a * n + c] / rho;
+        normalize(Y);
+        const double Tsi = v.u.si_temp(Fd[LBM_FIELD_T][c]);
+        const double cp = cp_mass(Tsi, Y);
+        const double gam = cp / (cp - R_UNIV / mean_W(Y));
+        const double u = Fd[LBM_FIELD_U][c], w = Fd[LBM_FIELD_W][c], vv = Fd[LBM_FIELD_V][c];
+        const double m = sqrt(u * u + vv * vv + w * w) / sqrt(gam * Fd[LBM_FIELD_P][c] / rho);
+        mach = isfinite(m) ? m : 0.0;
+    }
+    for (int o = 32; o > 0; o >>= 1) mach = fmax(mach, __shfl_down(mach, o, 64));
+    __shared__ double sm[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mach;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sm[0];
+        for (int k = 1; k < BLOCK / 64; ++k) m = fmax(m, sm[k]);
+        part[(size_t)blockIdx.x * 6 + 5] = m;
+    }
+}
+
 // ------------------------------------------------------------- SMOOTHING
 // LBM::Smoothing (lbm-smoothing.cpp:95-173) of species a, race-free: every
 // population l of a TYPE_F or TYPE_I cell becomes f + alpha * lap(f) with
@@ -2414,6 +2450,12 @@ void launch_mc_aa_save(const MultiView<P>& v, bool first, hipStream_t s)
 }
 
 template <typename P>
+void launch_mc_diag_mach(const MultiView<P>& v, double* partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mc_diag_mach<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, partials);
+}
+
+template <typename P>
 void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStream_t s)
 {
     for (int a = 0; a < NS; ++a) {
@@ -2456,6 +2498,8 @@ void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsi
 
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);
 template void launch_mc<float>(McKernel, const MultiView<float>&, hipStream_t);
+template void launch_mc_diag_mach<double>(const MultiView<double>&, double*, hipStream_t);
+template void launch_mc_diag_mach<float>(const MultiView<float>&, double*, hipStream_t);
 template void launch_mc_smoothing<double>(const MultiView<double>&, void*, double, hipStream_t);
 template void launch_mc_smoothing<float>(const MultiView<float>&, void*, double, hipStream_t);
 template void launch_mc_aa_save<double>(const MultiView<double>&, bool, hipStream_t);

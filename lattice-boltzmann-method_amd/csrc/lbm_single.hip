@@ -917,13 +917,24 @@ void launch_tms(const SingleView<P>& v, const int* cells, int count, hipStream_t
 
 int diag_partials(const Dims& d) { return (int)((d.n + BLOCK - 1) / BLOCK); }
 
-void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
-                 double* out8, hipStream_t s)
+void launch_diag_partials(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials,
+                          hipStream_t s)
 {
     hipLaunchKernelGGL(k_diag, grid_of(d.n), dim3(BLOCK), 0, s, d, type, fld[LBM_FIELD_RHO], fld[LBM_FIELD_U],
                        fld[LBM_FIELD_V], fld[LBM_FIELD_W], fld[LBM_FIELD_T], fld[LBM_FIELD_RHOE], fld[LBM_FIELD_P],
                        gamma, partials);
+}
+
+void launch_diag_final(const double* partials, int npart, double* out8, hipStream_t s)
+{
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(BLOCK), 0, s, partials, npart, out8);
+}
+
+void launch_diag(const Dims& d, const uint8_t* type, double* const* fld, double gamma, double* partials, int npart,
+                 double* out8, hipStream_t s)
+{
+    launch_diag_partials(d, type, fld, gamma, partials, s);
+    launch_diag_final(partials, npart, out8, s);
 }
 
 #define LBM_INSTANTIATE(P)                                                                 \

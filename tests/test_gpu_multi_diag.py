@@ -1,0 +1,68 @@
+"""Multicomponent diagnostics (lbm_diagnostics) against the oracle, through
+the C ABI.
+
+After five steps of a periodic reacting box on the GPU and in the oracle
+(oracle/multicomp.py), the kinetic energy, enstrophy, mass and energy of
+output.cpp:212-276 are evaluated by the oracle's C restatement
+(orc_diag_single) on the oracle's fields and compared with the device
+reduction; the max Mach number uses each cell's frozen mixture gamma
+c_p / (c_p - R / W) from the oracle's NASA7 thermo (oracle/chem.py), not the
+configuration's 1.4.  Bar: 1e-10 relative (the fields agree to the FP64 parity
+bar, the sums are reduced in another order).
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import cases
+from lbm_amd.capi import LBM_FP64, LBM_MODE_STRICT, MixtureLattice
+from oracle.binding import SingleState
+from oracle.chem import Mechanism
+from oracle.multicomp import MultiState, Units
+
+pytestmark = pytest.mark.gpu
+
+SEEDED = dict(cases.X_UNBURNT_PHI1, H=1e-4, O=1e-4, OH=1e-4, H2O=1e-3, HO2=1e-4, H2O2=1e-4)
+
+
+def test_multicomponent_diagnostics_match_oracle():
+    fl = cases.reacting_box(8, 6, 5, T_K=1500.0, amp_T=0.2, amp_u=0.03, X=SEEDED)
+    nx, ny, nz = (s - 2 for s in fl["type"].shape)
+    lat = MixtureLattice(nx, ny, nz, *cases.UNITS_FLAME, reaction=True, mode=LBM_MODE_STRICT, precision=LBM_FP64)
+    units = Units(*cases.UNITS_FLAME)
+    s = MultiState(nx + 2, ny + 2, nz + 2, units)
+    try:
+        lat.load_case(fl)
+        s.type[:] = fl["type"].ravel()
+        for k in ("rho", "u", "v", "w", "p", "T"):
+            getattr(s, k)[:] = fl[k].ravel()
+        s.X[:] = fl["X"].reshape(9, -1)
+        lat.Init()
+        s.init()
+        lat.calculate_moment()
+        s.moments()
+        for _ in range(5):
+            lat.step(1)
+            s.step(True)
+        d = lat.diagnostics()
+    finally:
+        lat.close()
+    # output.cpp's sums on the oracle's fields, by the oracle's C restatement
+    c = SingleState(nx + 2, ny + 2, nz + 2, 1.0, 1.4, 0.7, 0.01)
+    for k in ("type", "rho", "u", "v", "w", "rhoe"):
+        getattr(c, k)[...] = getattr(s, k).reshape(c.shape)
+    ke, ens, mass, energy = c.diag()
+    np.testing.assert_allclose([d["kinetic"], d["enstrophy"], d["mass"], d["energy"]], [ke, ens, mass, energy],
+                               rtol=1e-10)
+    F = s.type == 0
+    mech = Mechanism()
+    Y = (s.rho_a[:, F] / s.rho[F]).T
+    Y = Y / Y.sum(axis=1, keepdims=True)
+    Tsi = s.T[F] * cases.LATTICE_K
+    cp = mech.cp_mass(Tsi, Y)
+    gam = cp / (cp - mech.R / mech.W_mean(Y))
+    u2 = s.u[F] ** 2 + s.v[F] ** 2 + s.w[F] ** 2
+    mach = np.max(np.sqrt(u2) / np.sqrt(gam * s.p[F] / s.rho[F]))
+    assert abs(d["max_mach"] / mach - 1.0) < 1e-10
+    # the configuration's 1.4 would give a different number here
+    assert abs(np.max(np.sqrt(u2) / np.sqrt(1.4 * s.p[F] / s.rho[F])) / mach - 1.0) > 1e-3
+    assert d["nan_count"] == 0

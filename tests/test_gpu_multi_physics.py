@@ -250,7 +250,9 @@ def test_gpu_wave_columns_match_the_oracle_step_by_step(kind, X):
             s.step(reaction=False)
             for k in ("rho", "u", "v", "T", "p"):
                 a, b = lat.get_field(k).ravel()[F], getattr(s, k)[F]
-                scale = max(np.abs(b).max(), 1e-300) if k not in ("u", "v") else max(np.abs(b).max(), 1e-12)
+                # velocities on the scale max(|u|, 1e-6): components that vanish
+                # up to round-off (1e-18 of populations ~0.1) have no relative error
+                scale = max(np.abs(b).max(), 1e-300) if k not in ("u", "v") else max(np.abs(b).max(), 1e-6)
                 assert np.abs(a - b).max() / scale < 1e-10, f"step {step}: {k}"
             ra = lat.get_species_field("rho").reshape(NS, -1)[:, F]
             assert np.abs(ra - s.rho_a[:, F]).max() / s.rho_a[:, F].max() < 1e-10, f"step {step}: rho_a"
