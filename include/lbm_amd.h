@@ -288,6 +288,14 @@ int lbm_surrogate_set_norm(lbm_ctx* ctx, const float* host96);
 /* Network outputs Y [m][16] for normalised inputs X [m][16] with the in-step
  * inference kernel. */
 int lbm_surrogate_forward(lbm_ctx* ctx, const float* X, int64_t m, float* Y);
+/* No counterpart in the reference.  The same network pass for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc): the rows are staged in a fixed per-device array of 65536 input
+ * and 65536 output rows (8 MiB, held until the process ends) instead of
+ * buffers allocated per call; m > 65536 is LBM_E_ARG, and a context without a
+ * network yet (lbm_surrogate_init / _set_params / _train) is LBM_E_STATE.
+ * Calls that overlap on one GPU share that array and race. */
+int lbm_surrogate_forward_noalloc(lbm_ctx* ctx, const float* X, int64_t m, float* Y);
 /* Mean squared error over the 10 outputs of m rows (m multiple of 16) and its
  * gradient in the parameter layout, from the training kernels. */
 int lbm_surrogate_gradient(lbm_ctx* ctx, const float* X, const float* T, int64_t m, float* grad, float* loss);

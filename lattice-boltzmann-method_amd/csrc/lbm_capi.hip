@@ -6,6 +6,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1985,6 +1986,62 @@ int lbm_surrogate_forward(lbm_ctx* c, const float* X, int64_t m, float* Y)
     (void)hipFree(dx);
     if (dy) (void)hipFree(dy);
     if (e != hipSuccess) return fail(c, LBM_E_HIP, std::string("surrogate forward: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Allocation-free network pass: the device staging of its host rows lives in
+// `scratch`, SUR_NOALLOC_ROWS input rows then as many output rows.  Each
+// device holds its own copy; its address is resolved once per device with the
+// device current and kept in noalloc_ptr (host code never passes the symbol
+// itself).  Concurrent calls on one device share the array and race.
+constexpr int64_t SUR_NOALLOC_ROWS = 65536;
+static __device__ __align__(256) float scratch[2 * 16 * SUR_NOALLOC_ROWS];
+constexpr int MAX_DEVICES = 64;
+std::mutex noalloc_mutex;
+float* noalloc_ptr[MAX_DEVICES] = {};
+
+int noalloc_scratch(lbm_ctx* c, float** out)
+{
+    const int dev = c->cfg.device;
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(c, LBM_E_ARG, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_ptr[dev]) {
+        HIP_TRY(c, hipSetDevice(dev));
+        float* p = nullptr;
+        HIP_TRY(c, hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_surrogate_forward_noalloc(lbm_ctx* c, const float* X, int64_t m, float* Y)
+{
+    if (!c || !X || !Y) return fail(c, LBM_E_ARG, "null argument");
+    if (m < 1 || m > SUR_NOALLOC_ROWS)
+        return fail(c, LBM_E_ARG, "m out of range [1, " + std::to_string(SUR_NOALLOC_ROWS) +
+                                      "] (the allocation-free pass stages its rows in a fixed device array)");
+    if (!c->net_ready)
+        return fail(c, LBM_E_STATE, "no network yet: lbm_surrogate_init, _set_params or _train allocates it");
+    float* base = nullptr;
+    if (int rc = noalloc_scratch(c, &base)) return rc;
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    float* dx = base;
+    float* dy = base + 16 * SUR_NOALLOC_ROWS;
+    const size_t bytes = sizeof(float) * 16 * (size_t)m;
+    HIP_TRY(c, hipMemcpyAsync(dx, X, bytes, hipMemcpyHostToDevice, c->stream));
+    sur::mlp_infer(c->net, dx, (int)m, dy, c->stream);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(Y, dy, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
 
