@@ -280,6 +280,13 @@ int lbm_surrogate_get_capture(lbm_ctx* ctx, float* host, int64_t max_rows, int64
 int lbm_surrogate_train(lbm_ctx* ctx, int iters, int batch, float lr, uint64_t seed, float* losses);
 /* Switches Collide_Species between the network (on != 0) and the plain integrator. */
 int lbm_surrogate_enable(lbm_ctx* ctx, int on, int audit_stride, double audit_tol);
+/* No counterpart in the reference.  Widens the training envelope to every
+ * captured row (keeping the normalisation the network was trained with) and
+ * trains `iters` more steps on all of them.  While lbm_surrogate_capture is on
+ * with the surrogate in the step, the cells the plain integrator computes
+ * (envelope fallbacks and audits) are captured, so a refresh teaches the
+ * network the states it fell back on. */
+int lbm_surrogate_refresh(lbm_ctx* ctx, int iters, int batch, float lr, uint64_t seed, float* losses);
 int lbm_surrogate_read_stats(lbm_ctx* ctx, lbm_surrogate_stats* out);
 int lbm_surrogate_get_params(lbm_ctx* ctx, float* host);
 int lbm_surrogate_set_params(lbm_ctx* ctx, const float* host);

@@ -1865,6 +1865,20 @@ int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t see
     return LBM_OK;
 }
 
+int lbm_surrogate_refresh(lbm_ctx* c, int iters, int batch, float lr, uint64_t seed, float* losses)
+{
+    if (!c) return LBM_E_ARG;
+    if (!c->norm_ready) return fail(c, LBM_E_STATE, "no trained network to refresh (lbm_surrogate_train first)");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    unsigned long long have = 0;
+    if (c->cap_count) HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
+    const int64_t rows = std::min<int64_t>((int64_t)have, c->cap_max);
+    if (rows < 1) return fail(c, LBM_E_STATE, "no captured rows");
+    sur::capture_stats(c->cap_buf, rows, c->stats_part, c->sur_stats, c->stream, true);
+    return lbm_surrogate_train(c, iters, batch, lr, seed, losses);
+}
+
 int lbm_surrogate_enable(lbm_ctx* c, int on, int audit_stride, double audit_tol)
 {
     if (!c) return LBM_E_ARG;

@@ -340,6 +340,30 @@ __device__ inline void store_chem(const MultiView<P>& v, int c, const double src
     if (v.tchem) v.tchem[c] = T;
 }
 
+// Appends one (state, result) row of the plain integrator to the surrogate's
+// training set (raw features T, p, Y_k, dt; targets dT, dY_k), while it has room.
+template <typename P>
+__device__ inline void capture_row(const MultiView<P>& v, const ChemState& s, const double src[NS], double Tcur)
+{
+    const unsigned long long slot = atomicAdd(v.cap_count, 1ull);
+    if ((long long)slot >= v.cap_max) return;
+    float* row = v.cap_buf + slot * sur::CAP_STRIDE;
+    row[0] = (float)s.Tsi;
+    row[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
+#pragma unroll
+    for (int a = 0; a < NS; ++a) row[2 + a] = (float)s.Y0[a];
+    row[sur::NIN - 1] = v.u.s;
+#pragma unroll
+    for (int q = sur::NIN; q < 16; ++q) row[q] = 0.f;
+    // dT from the exact temperature of the initial state (Tsi carries the
+    // binary32 rounding of the lattice-unit crossing)
+    row[16] = (float)(Tcur - T_from_UV(s.usi, s.Y0, s.Tsi));
+#pragma unroll
+    for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / s.rho);
+#pragma unroll
+    for (int q = 16 + sur::NOUT; q < 32; ++q) row[q] = 0.f;
+}
+
 // Plain chemistry of every fluid cell; optionally samples (state, result)
 // rows for the surrogate's training set.
 template <typename P>
@@ -352,26 +376,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chemistry(MultiView<P> v)
     double src[NS], Tcur;
     plain_chemistry(v, s, src, &Tcur);
     store_chem(v, c, src, Tcur);
-    if (v.cap_buf && mix_hash(c, v.step) % (unsigned)v.cap_stride == 0) {
-        const unsigned long long slot = atomicAdd(v.cap_count, 1ull);
-        if ((long long)slot < v.cap_max) {
-            float* row = v.cap_buf + slot * sur::CAP_STRIDE;
-            row[0] = (float)s.Tsi;
-            row[1] = (float)pressure(s.Tsi, s.rho_si, s.Y0);
-#pragma unroll
-            for (int a = 0; a < NS; ++a) row[2 + a] = (float)s.Y0[a];
-            row[sur::NIN - 1] = v.u.s;
-#pragma unroll
-            for (int q = sur::NIN; q < 16; ++q) row[q] = 0.f;
-            // dT from the exact temperature of the initial state (Tsi carries
-            // the binary32 rounding of the lattice-unit crossing)
-            row[16] = (float)(Tcur - T_from_UV(s.usi, s.Y0, s.Tsi));
-#pragma unroll
-            for (int a = 0; a < NS; ++a) row[17 + a] = (float)(src[a] / s.rho);
-#pragma unroll
-            for (int q = 16 + sur::NOUT; q < 32; ++q) row[q] = 0.f;
-        }
-    }
+    if (v.cap_buf && mix_hash(c, v.step) % (unsigned)v.cap_stride == 0) capture_row(v, s, src, Tcur);
 }
 
 // Surrogate chemistry in three passes, so that the FP64 feature and decode
@@ -499,6 +504,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const un
         chem_state(v, c, true, s);
         double src[NS], Tp;
         plain_chemistry(v, s, src, &Tp);
+        // while capturing, the states the network could not take (and the
+        // audited ones) join the training set
+        if (v.cap_buf) capture_row(v, s, src, Tp);
         if (!(ent & CHEM_AUDIT)) {
             store_chem(v, c, src, Tp);
             continue;

@@ -61,9 +61,11 @@ void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s);
 // FP32 outputs Y [rows][16], weights staged in LDS.
 void mlp_infer_bf16(const Mlp& net, const uint16_t* X, int64_t rows, float* Y, hipStream_t s);
 // Column statistics of `rows` capture rows into `stats` [SS_SIZE] (device);
-// `part` holds STATS_BLOCKS * 32 * 4 doubles.
+// `part` holds STATS_BLOCKS * 32 * 4 doubles.  envelope_only: only the
+// envelope (SS_LO, SS_HI) is rewritten, the normalisation kept.
 constexpr int STATS_BLOCKS = 256;
-void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s);
+void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s,
+                   bool envelope_only = false);
 // Training batch of B rows drawn (hash of seed, iteration, row) from the
 // capture: BF16 normalised features [B][16] and FP32 normalised targets [B][16].
 void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, int it, int B, uint16_t* X,

@@ -307,7 +307,9 @@ __global__ __launch_bounds__(256) void k_cap_stats(const float* cap, int64_t row
 // the envelope is the captured [min, max] widened by 10 % of its width plus
 // 1e-3 of the mean.  A target column with no spread (an inert species) gets
 // scale 0: it is not learned and decodes to its mean.
-__global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, float* st)
+// envelope_only: keep the normalisation (the network's input and output
+// scaling) and reset only the training envelope (SS_LO, SS_HI).
+__global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, float* st, int envelope_only)
 {
     const int q = threadIdx.x;
     if (q >= 32) return;
@@ -327,11 +329,13 @@ __global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, fl
         sd = fmax(sd, 1e-3 * fabs(mean));
         if (!(sd > 1e-30)) sd = 1.0;
         const double margin = 0.1 * (hi - lo) + 1e-3 * fabs(mean);
-        st[SS_FMEAN + q] = used ? (float)mean : 0.f;
-        st[SS_FSTD + q] = used ? (float)sd : 1.f;
+        if (!envelope_only) {
+            st[SS_FMEAN + q] = used ? (float)mean : 0.f;
+            st[SS_FSTD + q] = used ? (float)sd : 1.f;
+        }
         st[SS_LO + q] = used ? (float)(lo - margin) : 0.f;
         st[SS_HI + q] = used ? (float)(hi + margin) : 0.f;
-    } else {
+    } else if (!envelope_only) {
         if (!(sd > 1e-6 * fabs(mean)) || !(sd > 1e-30)) sd = 0.0;
         const int t = q - 16;
         const bool used = t < NOUT;
@@ -340,10 +344,11 @@ __global__ void k_cap_stats_final(const double* part, int nblk, int64_t rows, fl
     }
 }
 
-void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s)
+void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s, bool envelope_only)
 {
     hipLaunchKernelGGL(k_cap_stats, dim3(STATS_BLOCKS), dim3(256), 0, s, cap, rows, part);
-    hipLaunchKernelGGL(k_cap_stats_final, dim3(1), dim3(64), 0, s, part, STATS_BLOCKS, rows, stats);
+    hipLaunchKernelGGL(k_cap_stats_final, dim3(1), dim3(64), 0, s, part, STATS_BLOCKS, rows, stats,
+                       envelope_only ? 1 : 0);
 }
 
 __device__ inline uint64_t splitmix(uint64_t z)

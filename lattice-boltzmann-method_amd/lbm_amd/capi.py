@@ -39,7 +39,7 @@ EXPORTS = [
     "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
     "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
     "lbm_surrogate_forward", "lbm_surrogate_gradient", "lbm_write_vtk", "lbm_write_restart", "lbm_read_restart",
-    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate", "lbm_set_smoothing",
+    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate", "lbm_set_smoothing", "lbm_surrogate_refresh",
 ]
 
 LBM_VTK_SI, LBM_VTK_MASS_FRACTIONS = 1, 2
@@ -143,6 +143,7 @@ def load_library(path=None, mechanism="h2air"):
     L.lbm_surrogate_capture.argtypes = [vp, i64, i32]
     L.lbm_surrogate_get_capture.argtypes = [vp, fp, i64, ctypes.POINTER(i64)]
     L.lbm_surrogate_train.argtypes = [vp, i32, i32, ctypes.c_float, u64, fp]
+    L.lbm_surrogate_refresh.argtypes = [vp, i32, i32, ctypes.c_float, u64, fp]
     L.lbm_surrogate_enable.argtypes = [vp, i32, i32, ctypes.c_double]
     L.lbm_surrogate_read_stats.argtypes = [vp, ctypes.POINTER(SurrogateStats)]
     for fn in ("lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm",
@@ -479,6 +480,45 @@ class MixtureLattice(Lattice):
         self._check(self._L.lbm_surrogate_train(self._h, int(iters), int(batch), float(lr), int(seed), _fp(losses)),
                     "surrogate_train")
         return losses[:iters]
+
+    def surrogate_refresh(self, iters, batch=4096, lr=3e-4, seed=3):
+        """Widens the envelope to every captured row (the fallbacks and audits
+        captured in the step included) and trains `iters` more steps."""
+        losses = np.zeros(max(iters, 1), np.float32)
+        self._check(self._L.lbm_surrogate_refresh(self._h, int(iters), int(batch), float(lr), int(seed),
+                                                  _fp(losses)), "surrogate_refresh")
+        return losses[:iters]
+
+    def step_with_refresh(self, nsteps, window=25, max_rate=0.005, iters=300, batch=4096, lr=3e-4):
+        """`nsteps` steps in windows of `window`; after a window in which more
+        than `max_rate` of the chemistry updates fell back to (or were
+        overruled by) the plain integrator, the network is refreshed on the
+        captured rows (lbm_surrogate_refresh; capture must be on).  Returns
+        (step seconds, refresh seconds, refresh count), each timed between
+        device synchronisations."""
+        import time
+        step_s = refresh_s = 0.0
+        refreshes = 0
+        done = 0
+        prev = self.surrogate_stats()
+        while done < nsteps:
+            k = min(window, nsteps - done)
+            self.sync()
+            t0 = time.perf_counter()
+            self.step(k)
+            self.sync()
+            step_s += time.perf_counter() - t0
+            done += k
+            st = self.surrogate_stats()
+            bad = (st["fallback"] - prev["fallback"]) + (st["audit_failures"] - prev["audit_failures"])
+            upd = (st["predicted"] - prev["predicted"]) + (st["fallback"] - prev["fallback"])
+            prev = st
+            if done < nsteps and upd > 0 and bad > max_rate * upd:
+                t0 = time.perf_counter()
+                self.surrogate_refresh(iters, batch=batch, lr=lr, seed=1000 + refreshes)
+                refresh_s += time.perf_counter() - t0
+                refreshes += 1
+        return step_s, refresh_s, refreshes
 
     def surrogate_enable(self, on=True, audit_stride=64, audit_tol=0.5):
         self._check(self._L.lbm_surrogate_enable(self._h, 1 if on else 0, int(audit_stride), float(audit_tol)),
