@@ -43,6 +43,10 @@ for _p in (REPO, PKG):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# surrogate maintenance in the multicomponent runs: refresh after a window of
+# steps in which more than REFRESH_RATE of the chemistry updates went to the
+# plain integrator
+REFRESH_WINDOW, REFRESH_RATE, REFRESH_ITERS = 25, 0.005, 200
 
 
 def parse():
@@ -105,14 +109,16 @@ MULTI_KERNELS = ("mc_chemistry_plain", "mc_chemistry_surrogate", "mc_chemistry_f
                  "mc_tms_wall", "mc_moments")
 
 
-def timed(lat, steps, warmup, names=SINGLE_KERNELS):
+def timed(lat, steps, warmup, names=SINGLE_KERNELS, run=None):
     """Warm-up, then exactly `steps` steps between two device synchronisations;
-    kernel launches are bracketed by HIP events on the library's stream."""
+    kernel launches are bracketed by HIP events on the library's stream.
+    `run(steps)` replaces lat.step for the timed steps (the surrogate's
+    maintained stepping)."""
     lat.step(warmup)
     lat.sync()
     lat.profile(True)
     t0 = time.perf_counter()
-    lat.step(steps)
+    (run or lat.step)(steps)
     lat.sync()
     t1 = time.perf_counter()
     kernels = {}
@@ -226,9 +232,25 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
                                  lat.surrogate_train(train_iters // 4, batch=batch, lr=3e-4, seed=2)])
         train_s = time.perf_counter() - t0
         lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
+        # maintained surrogate: the states the network cannot take (and the
+        # audited ones) join the training set, and a window of REFRESH_WINDOW
+        # steps with more than REFRESH_RATE of the updates falling back or
+        # failing their audit triggers a refresh (envelope widened to every
+        # captured state, REFRESH_ITERS more training iterations), inside the
+        # timed steps
+        lat.surrogate_capture(cap_steps * cells // cap_stride + 2 * cells, 1 << 30)
+        maint = {"step_s": 0.0, "refresh_s": 0.0, "refreshes": 0}
+
+        def run(k):
+            st, rs, nr = lat.step_with_refresh(k, window=REFRESH_WINDOW, max_rate=REFRESH_RATE, iters=REFRESH_ITERS,
+                                               batch=batch)
+            maint["step_s"] += st
+            maint["refresh_s"] += rs
+            maint["refreshes"] += nr
     else:
         lat.step(cap_steps)
-    sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS)
+        run = None
+    sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS, run=run if surrogate else None)
     T32, Y32 = fields(lat)
     sstats = lat.surrogate_stats() if surrogate else None
     diag = lat.diagnostics()
@@ -243,6 +265,10 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
                               "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-50:].mean()),
                               "captured_states": None, **sstats}
         extra["surrogate"]["captured_states"] = extra["surrogate"].pop("captured")
+        extra["surrogate"]["maintenance"] = {"window": REFRESH_WINDOW, "max_rate": REFRESH_RATE,
+                                             "iters": REFRESH_ITERS, "refreshes": maint["refreshes"],
+                                             "refresh_s": round(maint["refresh_s"], 3),
+                                             "step_s": round(maint["step_s"], 3)}
         extra["fallback_rate"] = round((sstats["fallback"] + sstats["audit_failures"]) /
                                        max(sstats["predicted"] + sstats["fallback"], 1), 6)
 
@@ -430,53 +456,64 @@ def mixing_c5_bench(args):
 def turbulence_report(args):
     """The 256^3 decaying-turbulence precision variant: FP32 shifted
     populations and FP64 plain, both fused, stepped side by side from the same
-    set-up for --steps steps (the configuration's 5000).  Every
-    --report-every steps: kinetic energy and enstrophy of each run (the
-    reference's output.cpp:212-276 sums) and their relative difference, and
-    the FP32 temperature field's error against FP64 (max and RMS relative over
-    the interior).  Once a run holds a non-finite cell it stops, and the step
-    is reported for each precision (the reference's model turns unstable here:
-    Init zeroes the velocity of its equilibria and fd_central treats periodic
-    ghosts as walls).  Progress lines go to stderr; one JSON line to stdout."""
+    set-up.  The configured state (Ma_rms 0.3, nu 0.005) for up to --steps
+    steps: every step for the first 20, then every --report-every steps, the
+    kinetic energy and enstrophy of each run (output.cpp:212-276 sums), their
+    relative difference, and the FP32 temperature field's error against FP64
+    (max and RMS relative over the interior).  Once a run holds a non-finite
+    cell it stops, and the step is reported for each precision.  The
+    reference's model does not survive this set-up (Init gives the mass
+    populations zero momentum while the energy populations carry the flow's
+    heat flux, lbm-init.cpp:22-60; fd_central treats periodic ghosts as
+    walls), so the same record is taken, every 50 steps, for lower Mach
+    numbers and a larger viscosity, to show where each turns non-finite.
+    Progress lines go to stderr; one JSON line to stdout."""
     from lbm_amd import cases
-    N = 256
-    fl, pr = cases.decaying_turbulence(N, nu=0.005)
     from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED
-    f32 = make_lattice(fl, pr, LBM_FP32_SHIFTED, LBM_MODE_FUSED)
-    f64 = make_lattice(fl, pr, LBM_FP64, LBM_MODE_FUSED)
-    rows, done = [], 0
-    diverged = {"fp32": None, "fp64": None}
-    t0 = time.perf_counter()
-    while done < args.steps and not (diverged["fp32"] and diverged["fp64"]):
-        k = min(args.report_every, args.steps - done)
-        for name, lat in (("fp32", f32), ("fp64", f64)):
-            if not diverged[name]:
-                lat.step(k)
-        done += k
-        a, b = f32.diagnostics(), f64.diagnostics()
-        for name, d in (("fp32", a), ("fp64", b)):
-            if d["nan_count"] and not diverged[name]:
-                diverged[name] = done
-        row = {"step": done, "ke_fp32": a["kinetic"], "ke_fp64": b["kinetic"], "ens_fp32": a["enstrophy"],
-               "ens_fp64": b["enstrophy"],
-               "ke_rel": abs(a["kinetic"] - b["kinetic"]) / abs(b["kinetic"]),
-               "ens_rel": abs(a["enstrophy"] - b["enstrophy"]) / abs(b["enstrophy"]),
-               "nan_fp32": a["nan_count"], "nan_fp64": b["nan_count"], "max_mach_fp64": b["max_mach"]}
-        if not (diverged["fp32"] or diverged["fp64"]):
-            T32 = f32.get_field("T")[1:-1, 1:-1, 1:-1]
-            T64 = f64.get_field("T")[1:-1, 1:-1, 1:-1]
-            rel = np.abs(T32 - T64) / np.abs(T64)
-            row["T_max_rel"] = float(rel.max())
-            row["T_rms_rel"] = float(np.sqrt(np.mean(rel * rel)))
-        rows.append(row)
-        print(f"turb256 report: step {done}: " + ", ".join(f"{key} {val:.4g}" for key, val in row.items()
-                                                          if key != "step"), file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    f32.close()
-    f64.close()
-    return {"workload": "decaying_turbulence_256^3_fp32_vs_fp64", "grid": [N, N, N], "params": pr,
-            "steps_requested": args.steps, "steps_run": done, "nonfinite_first_seen_at": diverged,
-            "sample_every": args.report_every, "wall_s": round(wall, 1), "rows": rows}
+    N = 256
+
+    def run(ma, nu, steps, every_of):
+        fl, pr = cases.decaying_turbulence(N, nu=nu, ma_rms=ma)
+        f32 = make_lattice(fl, pr, LBM_FP32_SHIFTED, LBM_MODE_FUSED)
+        f64 = make_lattice(fl, pr, LBM_FP64, LBM_MODE_FUSED)
+        rows, done = [], 0
+        diverged = {"fp32": None, "fp64": None}
+        t0 = time.perf_counter()
+        while done < steps and not (diverged["fp32"] and diverged["fp64"]):
+            k = min(every_of(done), steps - done)
+            for name, lat in (("fp32", f32), ("fp64", f64)):
+                if not diverged[name]:
+                    lat.step(k)
+            done += k
+            a, b = f32.diagnostics(), f64.diagnostics()
+            for name, d in (("fp32", a), ("fp64", b)):
+                if not diverged[name] and (d["nan_count"] or not np.isfinite(d["kinetic"])):
+                    diverged[name] = done
+            row = {"step": done, "ke_fp32": a["kinetic"], "ke_fp64": b["kinetic"], "ens_fp32": a["enstrophy"],
+                   "ens_fp64": b["enstrophy"], "max_mach_fp64": b["max_mach"]}
+            if not (diverged["fp32"] or diverged["fp64"]):
+                row["ke_rel"] = abs(a["kinetic"] - b["kinetic"]) / abs(b["kinetic"])
+                row["ens_rel"] = abs(a["enstrophy"] - b["enstrophy"]) / abs(b["enstrophy"])
+                T32 = f32.get_field("T")[1:-1, 1:-1, 1:-1]
+                T64 = f64.get_field("T")[1:-1, 1:-1, 1:-1]
+                rel = np.abs(T32 - T64) / np.abs(T64)
+                row["T_max_rel"] = float(rel.max())
+                row["T_rms_rel"] = float(np.sqrt(np.mean(rel * rel)))
+            rows.append(row)
+            print(f"turb256 report ma {ma} nu {nu}: step {done}: " +
+                  ", ".join(f"{key} {val:.4g}" for key, val in row.items() if key != "step"), file=sys.stderr,
+                  flush=True)
+        wall = time.perf_counter() - t0
+        f32.close()
+        f64.close()
+        return {"ma_rms": ma, "nu": nu, "params": pr, "steps_requested": steps, "steps_run": done,
+                "nonfinite_first_seen_at": diverged, "wall_s": round(wall, 1), "rows": rows}
+
+    stated = run(0.3, 0.005, args.steps, lambda done: 1 if done < 20 else args.report_every)
+    variants = [run(ma, nu, args.steps, lambda done: 50)
+                for ma, nu in ((0.1, 0.005), (0.1, 0.02), (0.05, 0.005), (0.05, 0.02), (0.02, 0.005), (0.02, 0.02))]
+    return {"workload": "decaying_turbulence_256^3_fp32_vs_fp64", "grid": [N, N, N], "configured": stated,
+            "variants": variants}
 
 
 def main():

@@ -424,8 +424,8 @@ __host__ __device__ inline R mix_viscosity_r(R lnT, const double X[NS])
     return tot;
 }
 
-template <typename R>
-__host__ __device__ inline R mix_conductivity_r(R lnT, const double X[NS])
+template <typename R, typename XT = double>
+__host__ __device__ inline R mix_conductivity_r(R lnT, const XT X[NS])
 {
     R s1 = 0, s2 = 0;
 #pragma unroll

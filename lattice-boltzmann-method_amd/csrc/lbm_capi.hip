@@ -126,6 +126,9 @@ struct lbm_ctx {
     void* smooth_tmp = nullptr;
     void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
+    double* omix = nullptr; // [2][n] the mixture collide's omega and omega_1 from the transport pass
+    bool omix_fresh = false; // omix holds the rates of the current fields: set by the transport pass
+                             // of lbm_collide_species, cleared by every call that changes the fields
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
     double mfref[mech::NS * Q] = {}, mgref[Q] = {};
     double* mref_dev = nullptr;  // [NS*27 + 27]
@@ -253,6 +256,8 @@ MultiView<P> make_mview(lbm_ctx* c, int src, int dst)
     v.tau_count = c->tau_count;
     v.src = static_cast<P*>(c->src);
     v.tsp = static_cast<P*>(c->tsp);
+    v.omix = c->omix;
+    v.use_omix = c->omix_fresh ? 1 : 0;
     v.tchem = c->tchem;
     v.step = c->step;
     if (c->capturing) {
@@ -449,7 +454,7 @@ void release(lbm_ctx* c)
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
-                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->twall_cell, c->cjc_cell,
+                    c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->omix,       c->twall_cell, c->cjc_cell,
                     c->side_slot, c->side_cell, c->fside, c->aa_pending, c->low_cell, c->low_sp,
                     c->low_keep,  c->low_count, c->smooth_tmp};
     for (void* p : ptrs)
@@ -602,6 +607,11 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         if ((e = hipMalloc(&c->tsp, tspb)) != hipSuccess) return bail("hipMalloc species transport", e);
         c->bytes += tspb;
         if (fp32(c)) {
+            if ((e = hipMalloc((void**)&c->omix, sizeof(double) * 2 * (size_t)n)) != hipSuccess)
+                return bail("hipMalloc mixture relaxation rates", e);
+            c->bytes += sizeof(double) * 2 * (size_t)n;
+        }
+        if (fp32(c)) {
             if ((e = hipMalloc((void**)&c->mref_dev, sizeof(double) * (mech::NS + 1) * Q)) != hipSuccess)
                 return bail("hipMalloc", e);
             (void)hipMemsetAsync(c->mref_dev, 0, sizeof(double) * (mech::NS + 1) * Q, c->stream);
@@ -693,6 +703,7 @@ int push_plan(lbm_ctx* c, const uint8_t* types, const std::vector<int>& tcell, c
 int lbm_set_types(lbm_ctx* c, const uint8_t* types)
 {
     if (!c || !types) return fail(c, LBM_E_ARG, "null argument");
+    c->omix_fresh = false;
     const Dims& d = c->d;
     for (int64_t i = 0; i < d.n; ++i)
         if (types[i] > TYPE_Q) return fail(c, LBM_E_ARG, "cell type out of range 0..9");
@@ -889,6 +900,7 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
 int lbm_set_field(lbm_ctx* c, int field, const double* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    c->omix_fresh = false;
     if (int rc = check_field(c, field)) return rc;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (int rc = materialize(c)) return rc;
@@ -1121,6 +1133,7 @@ int aa_upload(lbm_ctx* c, bool post, const double* host)
 int lbm_set_populations(lbm_ctx* c, int which, const double* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    c->omix_fresh = false;
     if (which < 0 || which > 3) return fail(c, LBM_E_ARG, "population set out of range");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (multi(c)) {
@@ -1262,6 +1275,7 @@ int species_field_ptr(lbm_ctx* c, int field, void** dev, size_t* count)
 int lbm_set_species_field(lbm_ctx* c, int field, const double* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    c->omix_fresh = false;
     void* dev;
     size_t cnt;
     if (int rc = species_field_ptr(c, field, &dev, &cnt)) return rc;
@@ -1299,6 +1313,7 @@ int lbm_get_species_field(lbm_ctx* c, int field, double* host)
 int lbm_init(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
+    c->omix_fresh = false;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede lbm_init");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (multi(c)) {
@@ -1430,6 +1445,7 @@ int lbm_collide_species(lbm_ctx* c)
         if (rc) return rc;
     }
     int rc = run_mc(c, McKernel::SpeciesTransport, "mc_species_transport", 0, 1);
+    c->omix_fresh = !rc && fp32(c);  // the FP32 pass also wrote the mixture collide's rates
     if (inplace(c)) {
         if (!rc) rc = aa_collide_species(c);
         ++c->step;
@@ -1497,6 +1513,7 @@ int lbm_stream(lbm_ctx* c)
 int lbm_tms_bc(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
+    c->omix_fresh = false;
     if (multi(c)) {
         if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
         if (!c->tms_unsupported.empty()) return fail(c, LBM_E_UNSUPPORTED, c->tms_unsupported);
@@ -1564,6 +1581,7 @@ int lbm_set_conjugate(lbm_ctx* c, int on)
 int lbm_moments(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
+    c->omix_fresh = false;
     if (!c->types_set) return fail(c, LBM_E_STATE, "lbm_set_types must precede stepping");
     if (multi(c)) {
         if (int rc = run_mc(c, McKernel::Moments, "mc_moments", 0, 0)) return rc;
@@ -2379,6 +2397,7 @@ int lbm_write_restart(lbm_ctx* c, const char* path, int step)
 int lbm_read_restart(lbm_ctx* c, const char* path, int* step_out)
 {
     if (!c || !path) return fail(c, LBM_E_ARG, "null argument");
+    c->omix_fresh = false;
     const bool mc = multi(c);
     File fl;
     fl.fp = fopen(path, "rb");

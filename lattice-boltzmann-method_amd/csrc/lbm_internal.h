@@ -29,6 +29,22 @@ inline unsigned xcd_grid(int64_t n, int block)
     const int64_t nb = (n + block - 1) / block;
     return (unsigned)((nb + N_XCD - 1) / N_XCD * N_XCD);
 }
+// Per-(cell, species) kernels: one workgroup per (cell block, species) pair,
+// the species fastest in xcd_block() order, so that the `ns` workgroups of a
+// cell block run on one XCD back to back and read the block's per-cell planes
+// (temperature, velocity, types, transport terms) from that XCD's L2 after
+// the first of them.  Workgroups past the last cell block get cb >= nb.
+inline unsigned xcd_species_grid(int64_t n, int block, int ns)
+{
+    const int64_t nb = (n + block - 1) / block;
+    return (unsigned)((nb * ns + N_XCD - 1) / N_XCD * N_XCD);
+}
+__device__ inline void xcd_species_block(int ns, int& cb, int& a)
+{
+    const unsigned L = xcd_block();
+    cb = (int)(L / (unsigned)ns);
+    a = (int)(L % (unsigned)ns);
+}
 
 // nbflag bits, computed once per lbm_set_types.
 enum : uint8_t {

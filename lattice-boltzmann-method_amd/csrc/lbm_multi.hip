@@ -549,6 +549,22 @@ __global__ __launch_bounds__(BLOCK) void k_mc_chem_list(MultiView<P> v, const un
 // omega_a and the impulse per species in v.tsp.  The collide pass then runs
 // one thread per (cell, species) (:452-520) with a small register footprint;
 // its per-species inputs are the species' own planes.
+// The mixture collide's relaxation rates (lbm-collision.cpp:241-252) from
+// the mixture viscosity mu_si and conductivity kap_si [SI] of the cell's
+// state, into v.omix: omega = 2 p / (p + 2 mu), omega_1 = 2 p c_p / (p c_p +
+// 2 kappa) in lattice units.
+template <typename P>
+__device__ __forceinline__ void mixture_rates(const MultiView<P>& v, int c, double Tsi, const double Y0[NS],
+                                              double mu_si, double kap_si)
+{
+    const double p = v.fld[LBM_FIELD_P][c];
+    const double cp = v.u.cp(cp_mass(Tsi, Y0));
+    const double mu = v.u.mu(mu_si);
+    const double kap = v.u.thermal_conductivity(kap_si);
+    v.omix[c] = 2 * p * 1 / (p * 1 + 2 * mu);
+    v.omix[v.d.n + c] = 2 * p * cp * 1 / (p * cp * 1 + 2 * kap);
+}
+
 // The transport pass of the FP32 path, all in binary32: species
 // viscosities and Wilke weights with sqrt(mu_a / mu_b) = sqrt(mu_a) / sqrt(mu_b)
 // (9 square roots and reciprocals in place of 45 divisions and square roots),
@@ -593,11 +609,14 @@ __device__ __forceinline__ void species_transport_fp32(const MultiView<P>& v, in
             den[a] += Xa[b] * pab;
             if (b != a) den[b] += Xa[a] * (mu[b] * rcp_r(mu[a]) * (float)(W[a] / W[b]) * pab);
         }
+    float mu_mix = 0.0f;
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
         const float visc = Xa[a] * (mu[a] * kmu) * rcp_r(den[a]);
         v.tsp[a * n + c] = P(2.0f * Xa[a] * p * rcp_r(Xa[a] * p + 2.0f * visc));
+        mu_mix += Xa[a] * mu[a] * rcp_r(den[a]);
     }
+    mixture_rates(v, c, Tsi, Y0, (double)mu_mix, (double)mix_conductivity_r<float>(lnT, Xa));
     const float rp_si = rcp_r((float)pressure<true>(Tsi, rho_si, Y0));
     float ua[3][NS], acc[NS][3];
 #pragma unroll
@@ -675,6 +694,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_species_transport(MultiView<P> v)
         const double visc = Xa[a] * (double)v.u.mu(mu[a]) / (double)den[a];
         v.tsp[a * n + c] = P(2 * Xa[a] * p / (Xa[a] * p + 2 * visc));
     }
+
     const double p_si = pressure(Tsi, rho_si, Y0);
     const R p_sir = (R)p_si;
     // Maxwell-Stefan friction with the binary diffusivities of the state.  The
@@ -768,14 +788,17 @@ __device__ __forceinline__ void species_collision(const MultiView<P>& v, int c, 
     }
 }
 
-// grid: (xcd_grid(n), NS); species a = blockIdx.y.  PUSH: the fused mode's
-// form, each post-collision population written to its streamed slot.
+// grid: xcd_species_grid(n, BLOCK, NS).  PUSH: the fused mode's form, each
+// post-collision population written to its streamed slot.
 template <typename P, bool PUSH>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
 {
-    const int c = xcd_block() * BLOCK + threadIdx.x;
-    const int a = blockIdx.y;
-    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    int cb, a;
+    xcd_species_block(NS, cb, a);
+    const int64_t c64 = (int64_t)cb * BLOCK + threadIdx.x;
+    if (c64 >= v.d.n) return;
+    const int c = (int)c64;
+    if (v.type[c] != TYPE_F) return;
     const int64_t n = v.d.n;
     const Ijk x = ijk(v.d, c);
     P fl[Q];
@@ -874,13 +897,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc_aa_save(MultiView<P> v, int first)
     }
 }
 
-// grid: (xcd_grid(n), NS).  Plain cells (not listed) and side cells.
+// grid: xcd_species_grid(n, BLOCK, NS).  Plain cells (not listed) and side cells.
 template <typename P, int SW>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide_species_aa(MultiView<P> v)
 {
-    const int c = xcd_block() * BLOCK + threadIdx.x;
-    const int a = blockIdx.y;
-    if (c >= v.d.n || v.type[c] != TYPE_F) return;
+    int cb, a;
+    xcd_species_block(NS, cb, a);
+    const int64_t c64 = (int64_t)cb * BLOCK + threadIdx.x;
+    if (c64 >= v.d.n) return;
+    const int c = (int)c64;
+    if (v.type[c] != TYPE_F) return;
     const int64_t n = v.d.n;
     const bool plain = v.nbflag[c] & NB_PUSH;
     if (plain && v.rhoa[a * n + c] <= SPECIES_MIN) return;  // listed: k_mc_collide_species_low
@@ -969,7 +995,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_side_push_aa(MultiView<P> v)
 }
 
 // --------------------------------------------------------------- Collide
-template <typename P>
+// OMIX: the relaxation rates come from v.omix (written by this step's
+// transport pass) instead of the cell's own transport evaluation.
+template <typename P, bool OMIX>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
@@ -990,19 +1018,25 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
             Y0[a] = FAST ? ra[a] * irho : ra[a] / rho;
         }
         normalize(Y0);
-        X_from_Y(Y0, X0);
         const double Tsi = v.u.si_temp(T);
-        const double lnT = log(Tsi);
-        const double cp = v.u.cp(cp_mass(Tsi, Y0));
         const double Wm = mean_W(Y0);
         const double Rmix = v.u.cp(R_UNIV / Wm);
-        // transport arithmetic R: binary64 on the plain path, binary32 on the FP32 path
-        using R = typename std::conditional<sizeof(P) == 4, float, double>::type;
-        const double mu = v.u.mu(mix_viscosity_r<R>((R)lnT, X0));
-        const double kap = v.u.thermal_conductivity(mix_conductivity_r<R>((R)lnT, X0));
         const double theta = v.u.energy_mass(R_UNIV * Tsi / Wm);
-        const double om1 = 2 * p * cp * 1 / (p * cp * 1 + 2 * kap);
-        const double om = 2 * p * 1 / (p * 1 + 2 * mu);
+        double om, om1;
+        if constexpr (OMIX) {
+            om = v.omix[c];
+            om1 = v.omix[n + c];
+        } else {
+            X_from_Y(Y0, X0);
+            const double lnT = log(Tsi);
+            const double cp = v.u.cp(cp_mass(Tsi, Y0));
+            // transport arithmetic R: binary64 on the plain path, binary32 on the FP32 path
+            using R = typename std::conditional<sizeof(P) == 4, float, double>::type;
+            const double mu = v.u.mu(mix_viscosity_r<R>((R)lnT, X0));
+            const double kap = v.u.thermal_conductivity(mix_conductivity_r<R>((R)lnT, X0));
+            om1 = 2 * p * cp * 1 / (p * cp * 1 + 2 * kap);
+            om = 2 * p * 1 / (p * 1 + 2 * mu);
+        }
         warn = (1 / om <= 0.5) || (1 / om1 <= 0.5);
         double Uint = Fd[LBM_FIELD_RHOE][c] / rho;
         for (int a = 0; a < NS; ++a) {
@@ -2391,15 +2425,20 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
         hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
         break;
     case McKernel::CollideSpecies:
-        hipLaunchKernelGGL((k_mc_collide_species<P, false>), dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_mc_collide_species<P, false>), dim3(xcd_species_grid(v.d.n, BLOCK, NS)), dim3(BLOCK), 0,
+                           s, v);
         break;
     case McKernel::CollideSpeciesPush:
-        hipLaunchKernelGGL((k_mc_collide_species<P, true>), dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+        hipLaunchKernelGGL((k_mc_collide_species<P, true>), dim3(xcd_species_grid(v.d.n, BLOCK, NS)), dim3(BLOCK), 0,
+                           s, v);
         if (v.nside > 0)
             hipLaunchKernelGGL(k_mc_side_push<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
                                s, v);
         break;
-    case McKernel::Collide: hipLaunchKernelGGL(k_mc_collide<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+    case McKernel::Collide:
+        if (v.use_omix) hipLaunchKernelGGL((k_mc_collide<P, true>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        else hipLaunchKernelGGL((k_mc_collide<P, false>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        break;
     case McKernel::StreamEnergy:
         hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), 1), dim3(BLOCK), 0, s, v, NS);
         break;
@@ -2475,7 +2514,7 @@ void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStre
 template <typename P>
 void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s)
 {
-    const dim3 g(xcd_grid(v.d.n, BLOCK), NS), gl(4 * cu_count()), gs((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS);
+    const dim3 g(xcd_species_grid(v.d.n, BLOCK, NS)), gl(4 * cu_count()), gs((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS);
     if (v.swapped) {
         hipLaunchKernelGGL((k_mc_collide_species_aa<P, 1>), g, dim3(BLOCK), 0, s, v);
         hipLaunchKernelGGL((k_mc_collide_species_low<P, 1>), gl, dim3(BLOCK), 0, s, v);

@@ -172,15 +172,22 @@ __global__ void k_to_bf(const float* in, uint16_t* out, int64_t n)
     if (i < n) out[i] = f2bf(in[i]);
 }
 
-// W[L] (FP32, [N][K]) into the fragment-ordered BF16 block of layer L.
+// W[L] (FP32, [N][K]) into the fragment-ordered BF16 block of layer L: the
+// 16-deep K-steps of layer 0, the 32-deep ones (k32_neuron order) of the others.
 __global__ void k_to_frag(const float* W, int N, int K, uint16_t* Wf)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N * K) return;
-    const int KS = K / 16;
-    const int q = i & 3, lane = (i >> 2) & 63, ts = i >> 8;
+    if (K == 16) {
+        const int q = i & 3, lane = (i >> 2) & 63, t = i >> 8;
+        const int n = 16 * t + (lane & 15), k = 4 * (lane >> 4) + q;
+        Wf[i] = f2bf(W[n * K + k]);
+        return;
+    }
+    const int KS = K / 32;
+    const int e = i & 7, lane = (i >> 3) & 63, ts = i >> 9;
     const int t = ts / KS, s = ts % KS;
-    const int n = 16 * t + (lane & 15), k = 16 * s + 4 * (lane >> 4) + q;
+    const int n = 16 * t + (lane & 15), k = k32_neuron(s, lane >> 4, e);
     Wf[i] = f2bf(W[n * K + k]);
 }
 

@@ -7,12 +7,18 @@
 //   A[i][k]: i = lane % 16, k = 4 (lane / 16) + 0..3
 //   B[k][j]: j = lane % 16, k = 4 (lane / 16) + 0..3
 //   D[i][j]: j = lane % 16, i = 4 (lane / 16) + 0..3
+// v_mfma_f32_16x16x32_bf16 (gfx950, twice the K per instruction at the same
+// issue cost): 8 BF16 per operand and lane, k = 8 (lane / 16) + 0..7, the same
+// D layout.
 //
 // In-step inference computes every layer transposed, Z^T = W H^T: the weights
 // are the A operand (i = neuron) and the activations the B operand (j = batch
 // row).  The D fragment of output tile t then holds neurons 16 t + 4 (lane / 16)
-// + 0..3 of row lane % 16, which is exactly the B fragment of K-step t of the
-// next layer, so activations stay in registers from layer to layer.
+// + 0..3 of row lane % 16.  Layer 0 (K = 16) runs 16x16x16; the 128-wide
+// layers run 16x16x32 with K-step s made of the D fragments of tiles 2s and
+// 2s + 1 side by side: its element e of lane group g is neuron k32_neuron(s, g,
+// e).  The weight fragments are stored in that K order, so activations stay
+// in registers from layer to layer and no lane exchanges data.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -25,6 +31,13 @@ namespace sur {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// Neuron held by element e (0..7) of lane group g in 32-deep K-step s.
+__host__ __device__ constexpr int k32_neuron(int s, int g, int e)
+{
+    return 32 * s + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+}
 
 // FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ uint16_t f2bf(float x)
@@ -56,10 +69,11 @@ __device__ __forceinline__ float gelu_grad(float x)
     return s + x * s * (1.0f - s) * (GELU_C * (1.0f + 3.0f * GELU_A * x * x));
 }
 
-// Fragment-ordered BF16 weights: for layer L with NT = N/16 output tiles and
-// KS = K/16 K-steps, element ((t KS + s) 64 + lane) 4 + q holds
-// W[16 t + lane % 16][16 s + 4 (lane / 16) + q], so a wavefront reads one
-// A fragment as 64 consecutive 8-byte words.
+// Fragment-ordered BF16 weights.  Layer 0 (K = 16): element (t 64 + lane) 4 + q
+// holds W[16 t + lane % 16][4 (lane / 16) + q].  Layers with K = 32 KS:
+// element ((t KS + s) 64 + lane) 8 + e holds W[16 t + lane % 16][k32_neuron(s,
+// lane / 16, e)].  A wavefront reads one A fragment as 64 consecutive 8- or
+// 16-byte words.
 constexpr int frag_offset(int L)
 {
     int off = 0;
@@ -73,6 +87,22 @@ constexpr int bias_offset(int L)
     int off = 0;
     for (int i = 0; i < L; ++i) off += DIMS[i + 1];
     return off;
+}
+
+__device__ __forceinline__ v8bf load_frag8(const uint16_t* p)
+{
+    const uint4 w = *reinterpret_cast<const uint4*>(p);
+    v8bf r;
+    __builtin_memcpy(&r, &w, 16);
+    return r;
+}
+// The B operand of a 32-deep K-step from the activations of two output tiles.
+__device__ __forceinline__ v8bf pair_frag(const v4s lo, const v4s hi)
+{
+    short t[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    v8bf r;
+    __builtin_memcpy(&r, t, 16);
+    return r;
 }
 
 __device__ __forceinline__ v4s load_frag(const uint16_t* p)
@@ -135,7 +165,8 @@ __device__ __forceinline__ void mlp_rows32(const uint16_t* Wf, const float* bias
                 act[bt][t] = act_frag(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, x[bt], b, 0, 0, 0));
         }
     }
-    // layers 1 and 2: 128 -> 128
+    // layers 1 and 2: 128 -> 128, 32-deep K-steps
+    constexpr int KS = 128 / 32;
 #pragma unroll
     for (int L = 1; L <= 2; ++L) {
         const uint16_t* WL = Wf + frag_offset(L);
@@ -144,10 +175,12 @@ __device__ __forceinline__ void mlp_rows32(const uint16_t* Wf, const float* bias
         for (int t = 0; t < H; ++t) {
             v4f acc0 = bias_frag(bL, 16 * t, lane), acc1 = acc0;
 #pragma unroll
-            for (int s = 0; s < H; ++s) {
-                const v4s a = load_frag(WL + ((t * H + s) * 64 + lane) * 4);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[0][s], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[1][s], acc1, 0, 0, 0);
+            for (int s = 0; s < KS; ++s) {
+                const v8bf a = load_frag8(WL + ((t * KS + s) * 64 + lane) * 8);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, pair_frag(act[0][2 * s], act[0][2 * s + 1]), acc0,
+                                                                0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, pair_frag(act[1][2 * s], act[1][2 * s + 1]), acc1,
+                                                                0, 0, 0);
             }
             nxt[0][t] = act_frag(acc0);
             nxt[1][t] = act_frag(acc1);
@@ -163,10 +196,10 @@ __device__ __forceinline__ void mlp_rows32(const uint16_t* Wf, const float* bias
     const float* b3 = bias + bias_offset(3);
     v4f acc0 = bias_frag(b3, 0, lane), acc1 = acc0;
 #pragma unroll
-    for (int s = 0; s < H; ++s) {
-        const v4s a = load_frag(W3 + (s * 64 + lane) * 4);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[0][s], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[1][s], acc1, 0, 0, 0);
+    for (int s = 0; s < KS; ++s) {
+        const v8bf a = load_frag8(W3 + (s * 64 + lane) * 8);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, pair_frag(act[0][2 * s], act[0][2 * s + 1]), acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, pair_frag(act[1][2 * s], act[1][2 * s + 1]), acc1, 0, 0, 0);
     }
     *reinterpret_cast<v4f*>(yout + r * 16 + kq) = acc0;
     *reinterpret_cast<v4f*>(yout + (16 + r) * 16 + kq) = acc1;

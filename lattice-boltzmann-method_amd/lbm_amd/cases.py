@@ -420,9 +420,9 @@ def transport_column(N, X, T_K=300.0, p_si=ONE_ATM, units=UNITS_TRANSPORT, minim
     return fl, np.arange(N, dtype=np.float64)
 
 
-def wave_column(kind, N, amp, X=None, T_K=300.0, units=UNITS_TRANSPORT, species=None):
-    """Single-mode waves on transport_column, phase 2 pi j / N over the
-    interior cells j = 0..N-1 (ghost-inclusive index j + 1):
+def wave_column(kind, N, amp, X=None, T_K=300.0, units=UNITS_TRANSPORT, species=None, phase=0.0):
+    """Single-mode waves on transport_column, phase 2 pi (j + phase) / N over
+    the interior cells j = 0..N-1 (ghost-inclusive index j + 1):
       'shear'      u = amp sin           (VISCOSITY_TEST_MULTICOMP, setup.cpp:1672)
       'thermal'    T = T0 (1 + amp sin), p uniform   (CONDUCTIVITY_TEST, setup.cpp:396)
       'diffusion'  X_a = X0_a + amp sin for the first named species and
@@ -432,7 +432,7 @@ def wave_column(kind, N, amp, X=None, T_K=300.0, units=UNITS_TRANSPORT, species=
                    (SOUNDSPEED_TEST, setup.cpp:463)"""
     X = X or X_AIR
     fl, j = transport_column(N, X, T_K, units=units)
-    ph = 2.0 * np.pi * j / N
+    ph = 2.0 * np.pi * (j + phase) / N
     sl = (1, slice(1, -1), 1)
     if kind == "shear":
         fl["u"][sl] = amp * np.sin(ph)

@@ -1,5 +1,6 @@
-"""Shared helpers of the GPU tests that compare a multicomponent mode (fused,
-in-place) with the strict calls bitwise, through the C ABI."""
+"""Shared helpers of the multicomponent GPU tests: a mode (fused, in-place)
+against the strict calls bitwise, and seeding the device context with the
+oracle's state, through the C ABI."""
 import numpy as np
 
 from lbm_amd import cases
@@ -55,3 +56,22 @@ def compare(fl, steps, mode, precision=LBM_FP64, reaction=True, tag=""):
     finally:
         strict.close()
         other.close()
+
+
+def seed_gpu_from_oracle(lat, s):
+    """Copies the oracle's full state into the device context."""
+    shape = lat.shape
+    for k in ("rho", "u", "v", "w", "p", "T", "rhoe", "HRR"):
+        lat.set_field(k, getattr(s, k).reshape(shape))
+    for (a, b), name in zip(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)), ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")):
+        lat.set_field(name, s.ptens[a, b].reshape(shape))
+    for d, name in enumerate(("qx", "qy", "qz")):
+        lat.set_field(name, s.eflux[d].reshape(shape))
+    lat.set_species_field("X", s.X.reshape((9,) + shape))
+    lat.set_species_field("rho", s.rho_a.reshape((9,) + shape))
+    for d, name in enumerate("uvw"):
+        lat.set_species_field(name, s.u_a[d].reshape((9,) + shape))
+    lat.set_populations(POP_F, s.f.reshape((9, 27) + shape))
+    lat.set_populations(POP_FPC, s.fpc.reshape((9, 27) + shape))
+    lat.set_populations(POP_G, s.g.reshape((27,) + shape))
+    lat.set_populations(POP_GPC, s.gpc.reshape((27,) + shape))

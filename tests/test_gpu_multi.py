@@ -13,6 +13,7 @@ import pytest
 
 from lbm_amd import cases
 from lbm_amd.capi import LBM_MODE_STRICT, POP_F, POP_FPC, POP_G, POP_GPC, MixtureLattice
+from mode_compare import seed_gpu_from_oracle
 from oracle.multicomp import MultiStateBC, Units
 
 pytestmark = pytest.mark.gpu
@@ -134,25 +135,6 @@ def test_mc_deterministic():
         lat.step(4)
         out.append((lat.get_populations(POP_F), lat.get_field("T")))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
-
-
-def seed_gpu_from_oracle(lat, s):
-    """Copies the oracle's full state into the device context."""
-    shape = lat.shape
-    for k in ("rho", "u", "v", "w", "p", "T", "rhoe", "HRR"):
-        lat.set_field(k, getattr(s, k).reshape(shape))
-    for (a, b), name in zip(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)), ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")):
-        lat.set_field(name, s.ptens[a, b].reshape(shape))
-    for d, name in enumerate(("qx", "qy", "qz")):
-        lat.set_field(name, s.eflux[d].reshape(shape))
-    lat.set_species_field("X", s.X.reshape((9,) + shape))
-    lat.set_species_field("rho", s.rho_a.reshape((9,) + shape))
-    for d, name in enumerate("uvw"):
-        lat.set_species_field(name, s.u_a[d].reshape((9,) + shape))
-    lat.set_populations(POP_F, s.f.reshape((9, 27) + shape))
-    lat.set_populations(POP_FPC, s.fpc.reshape((9, 27) + shape))
-    lat.set_populations(POP_G, s.g.reshape((27,) + shape))
-    lat.set_populations(POP_GPC, s.gpc.reshape((27,) + shape))
 
 
 @pytest.mark.parametrize("reaction", [False, True])

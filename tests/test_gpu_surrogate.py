@@ -273,6 +273,42 @@ def test_failed_audits_restore_plain_bitwise():
     assert np.array_equal(lat.get_field("HRR"), hrr0)
 
 
+def test_refresh_captures_fallbacks_and_widens_the_envelope():
+    """lbm_surrogate_refresh.  While capturing, each cell the network cannot
+    take joins the training set with the plain integrator's result: after a
+    step whose temperature envelope is cut at the middle of the box's range,
+    the new rows are exactly the plain step's rows of the cells above the cut.
+    The refresh rewrites the envelope over every captured row (the oracle's
+    norm_record of them), keeps the normalisation the network was trained
+    with, and the same states then stop falling back."""
+    fl = box()
+    n = int((fl["type"] == 0).sum())
+    lat = _capture_step(fl, stride=1)
+    plain = lat.surrogate_captured()
+    assert len(plain) == n
+    lat.surrogate_train(0)
+    st = lat.surrogate_norm()
+    cut = np.sort(plain[:, 0])[n // 2]  # a captured binary32 temperature
+    st[S.SS_HI] = cut
+    lat.set_surrogate_norm(st)
+    lat.surrogate_enable(True, audit_stride=0)
+    lat.Collide_Species()
+    k = lat.surrogate_stats()["fallback"]
+    hot = plain[plain[:, 0] > cut]
+    assert k == len(hot) and 0 < k < n
+    rows = lat.surrogate_captured()
+    assert len(rows) == n + k
+    order = lambda r: r[np.lexsort(r[:, ::-1].T)]  # noqa: E731 -- capture order is not defined
+    np.testing.assert_array_equal(order(rows[n:]), order(hot))
+    lat.surrogate_refresh(0)
+    st2 = lat.surrogate_norm()
+    np.testing.assert_array_equal(st2[:S.SS_LO], st[:S.SS_LO])
+    np.testing.assert_allclose(st2[S.SS_LO:], S.norm_record(rows)[S.SS_LO:], rtol=2e-6, atol=1e-30)
+    assert st2[S.SS_HI] > plain[:, 0].max()
+    lat.Collide_Species()
+    assert lat.surrogate_stats()["fallback"] == k
+
+
 def test_surrogate_requires_training_and_reaction():
     from lbm_amd.capi import LbmError
     lat = lattice(box())
