@@ -126,7 +126,7 @@ struct lbm_ctx {
     void* smooth_tmp = nullptr;
     void* src = nullptr;    // [NS][n] chemistry mass sources, element type of the populations
     void* tsp = nullptr;    // [4][NS][n] Collide_Species transport (omega_a, friction impulse)
-    double* omix = nullptr; // [2][n] the mixture collide's omega and omega_1 from the transport pass
+    double* omix = nullptr; // FP32: [7][n] the mixture collide's rates (transport pass) and cell terms
     bool omix_fresh = false; // omix holds the rates of the current fields: set by the transport pass
                              // of lbm_collide_species, cleared by every call that changes the fields
     // FP32 multicomponent storage: rest-state shifts (host copies and device tables)
@@ -607,9 +607,9 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         if ((e = hipMalloc(&c->tsp, tspb)) != hipSuccess) return bail("hipMalloc species transport", e);
         c->bytes += tspb;
         if (fp32(c)) {
-            if ((e = hipMalloc((void**)&c->omix, sizeof(double) * 2 * (size_t)n)) != hipSuccess)
-                return bail("hipMalloc mixture relaxation rates", e);
-            c->bytes += sizeof(double) * 2 * (size_t)n;
+            if ((e = hipMalloc((void**)&c->omix, sizeof(double) * 7 * (size_t)n)) != hipSuccess)
+                return bail("hipMalloc mixture collide terms", e);
+            c->bytes += sizeof(double) * 7 * (size_t)n;
         }
         if (fp32(c)) {
             if ((e = hipMalloc((void**)&c->mref_dev, sizeof(double) * (mech::NS + 1) * Q)) != hipSuccess)

@@ -30,9 +30,11 @@ struct MultiView {
     P* src;         // [NS][n] species mass added by the chemistry over the step (lattice units)
     P* tsp;         // [4][NS][n] Collide_Species transport of the step: omega_a, then the
                     // Maxwell-Stefan friction impulse per axis
-    double* omix;   // FP32 storage: [2][n] omega and omega_1 of the mixture collide,
-                    // written by the transport pass; the collide reads them when
-                    // use_omix is set (else it evaluates the mixture transport itself)
+    double* omix;   // FP32 storage: [7][n] omega and omega_1 of the mixture collide
+                    // (written by the transport pass), then theta, U - 3/2 theta and
+                    // the energy-flux correction dq[3] (written by the collide's
+                    // first kernel); used when use_omix is set, else the collide
+                    // evaluates the mixture transport itself in one kernel
     int use_omix;
     double* tchem;  // [n] temperature after the chemistry [K] (surrogate audit only)
     // chemistry surrogate (all null when unused)

@@ -995,13 +995,40 @@ __global__ __launch_bounds__(BLOCK) void k_mc_side_push_aa(MultiView<P> v)
 }
 
 // --------------------------------------------------------------- Collide
-// OMIX: the relaxation rates come from v.omix (written by this step's
-// transport pass) instead of the cell's own transport evaluation.
-template <typename P, bool OMIX>
+// PART 0: the whole collide, the relaxation rates from the cell's own
+// transport evaluation.  With the FP32 transport pass's rates in v.omix the
+// collide runs in two kernels: PART 1 evaluates the cell terms (the rates'
+// tau check, theta, U - 3/2 theta and the energy-flux correction dq, which
+// read the neighbours and every species) into v.omix; PART 2 relaxes the 27
+// energy populations from them, a streaming pass of few registers.  Both
+// forms do the same arithmetic in the same order.
+enum : int { OMIX_OM = 0, OMIX_OM1 = 1, OMIX_THETA = 2, OMIX_USHIFT = 3, OMIX_DQ = 4, OMIX_PLANES = 7 };
+
+template <typename P, int PART>
 __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
 {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     bool warn = false;
+    if constexpr (PART == 2) {
+        if (c >= v.d.n || v.type[c] != TYPE_F) return;
+        const int64_t n = v.d.n;
+        double* const* Fd = v.fld;
+        const double rho = Fd[LBM_FIELD_RHO][c];
+        const double vel[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
+        const double om = v.omix[OMIX_OM * n + c], om1 = v.omix[OMIX_OM1 * n + c];
+        const double theta = v.omix[OMIX_THETA * n + c], ushift = v.omix[OMIX_USHIFT * n + c];
+        const double dq[3] = {v.omix[OMIX_DQ * n + c], v.omix[(OMIX_DQ + 1) * n + c], v.omix[(OMIX_DQ + 2) * n + c]};
+        const GeqAxes<double> G = geq_axes<double>(theta, vel);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            const double ge = geq_dir<double>(G, l, rho, ushift);
+            const double cdq = 0.5 * (cx(l) * dq[0] + cy(l) * dq[1] + cz(l) * dq[2]);
+            const double gs = (c2(l) == 1) ? ge + cdq : ge;
+            const double g = ldg(v, v.g, l, c);
+            stg(v, v.go, l, c, g + om1 * (ge - g) + (om - om1) * (ge - gs));
+        }
+        return;
+    }
     if (c < v.d.n && v.type[c] == TYPE_F) {
         const int64_t n = v.d.n;
         const Ijk x = ijk(v.d, c);
@@ -1023,9 +1050,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
         const double Rmix = v.u.cp(R_UNIV / Wm);
         const double theta = v.u.energy_mass(R_UNIV * Tsi / Wm);
         double om, om1;
-        if constexpr (OMIX) {
-            om = v.omix[c];
-            om1 = v.omix[n + c];
+        if constexpr (PART == 1) {
+            om = v.omix[OMIX_OM * n + c];
+            om1 = v.omix[OMIX_OM1 * n + c];
         } else {
             X_from_Y(Y0, X0);
             const double lnT = log(Tsi);
@@ -1104,6 +1131,12 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
             }
             dq[b] = acc + qdiff[b] + qcorr[b] + 0.5 * 1 * vel[b] * dQ[b];
         }
+        if constexpr (PART == 1) {
+            v.omix[OMIX_THETA * n + c] = theta;
+            v.omix[OMIX_USHIFT * n + c] = Uint - 1.5 * theta;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) v.omix[(OMIX_DQ + b) * n + c] = dq[b];
+        } else {
         const GeqAxes<double> G = geq_axes<double>(theta, vel);
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
@@ -1112,6 +1145,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
             const double gs = (c2(l) == 1) ? ge + cdq : ge;
             const double g = ldg(v, v.g, l, c);
             stg(v, v.go, l, c, g + om1 * (ge - g) + (om - om1) * (ge - gs));
+        }
         }
     }
     const unsigned long long m = __ballot(warn);
@@ -2436,8 +2470,12 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
                                s, v);
         break;
     case McKernel::Collide:
-        if (v.use_omix) hipLaunchKernelGGL((k_mc_collide<P, true>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
-        else hipLaunchKernelGGL((k_mc_collide<P, false>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        if (v.use_omix) {
+            hipLaunchKernelGGL((k_mc_collide<P, 1>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+            hipLaunchKernelGGL((k_mc_collide<P, 2>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        } else {
+            hipLaunchKernelGGL((k_mc_collide<P, 0>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+        }
         break;
     case McKernel::StreamEnergy:
         hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), 1), dim3(BLOCK), 0, s, v, NS);
