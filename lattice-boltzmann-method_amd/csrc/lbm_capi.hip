@@ -160,6 +160,8 @@ struct lbm_ctx {
     uint16_t* bx = nullptr;  // training batch, BF16 features
     float* bt = nullptr;     // training batch, targets
     float* bloss = nullptr;  // per-iteration losses
+    float* adam_c12 = nullptr; // per-iteration Adam bias corrections [2][lcap] (device)
+    int* train_it = nullptr;   // the training loop's iteration counter (device)
     int bcap = 0, lcap = 0;
 
     bool prof = false;
@@ -453,7 +455,7 @@ void release(lbm_ctx* c)
                     c->fld_block, c->qd_block,  c->tau_count,  c->diag_part, c->diag_out,  c->sf[0],
                     c->sf[1],     c->sp_block,  c->tms_cell,   c->tms_dir,   c->src,       c->sur_stats,
                     c->stats_part, c->sur_count, c->chem_list, c->chem_count, c->tchem,    c->cap_buf,
-                    c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part,
+                    c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part, c->adam_c12, c->train_it,
                     c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->omix,       c->twall_cell, c->cjc_cell,
                     c->side_slot, c->side_cell, c->fside, c->aa_pending, c->low_cell, c->low_sp,
                     c->low_keep,  c->low_count, c->smooth_tmp};
@@ -1864,17 +1866,53 @@ int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t see
     }
     if (int rc = sur_work(c, batch)) return rc;
     if (int rc = sur_batch(c, batch)) return rc;
-    if (iters > c->lcap) {
+    if (iters > c->lcap || !c->adam_c12) {
         if (c->bloss) (void)hipFree(c->bloss);
+        if (c->adam_c12) (void)hipFree(c->adam_c12);
         c->bloss = nullptr;
-        HIP_TRY(c, hipMalloc((void**)&c->bloss, sizeof(float) * iters));
-        c->lcap = iters;
+        c->adam_c12 = nullptr;
+        HIP_TRY(c, hipMalloc((void**)&c->bloss, sizeof(float) * std::max(iters, 1)));
+        HIP_TRY(c, hipMalloc((void**)&c->adam_c12, sizeof(float) * 2 * std::max(iters, 1)));
+        c->lcap = std::max(iters, 1);
     }
-    for (int it = 0; it < iters; ++it) {
-        sur::gather_batch(c->cap_buf, rows, c->sur_stats, seed, it, batch, c->bx, c->bt, c->stream);
-        sur::mlp_grad(c->net, c->bx, c->bt, batch, c->work, c->stream);
-        sur::loss_sum(c->work.loss_rows, batch, 1.f / ((float)batch * sur::NOUT), c->bloss + it, c->stream);
-        sur::mlp_adam(c->net, c->work, lr, ++c->adam_step, c->stream);
+    if (!c->train_it) HIP_TRY(c, hipMalloc((void**)&c->train_it, sizeof(int)));
+    if (iters > 0) {
+        // one training iteration (batch draw, forward and backward passes,
+        // loss, Adam: 45 kernels) is captured once as a HIP graph and replayed
+        // `iters` times; the iteration index and the step's Adam corrections
+        // are read on the device
+        std::vector<float> c12(2 * (size_t)iters);
+        sur::adam_corrections(c->adam_step + 1, iters, c12.data());
+        const int zero = 0;
+        HIP_TRY(c, hipMemcpyAsync(c->adam_c12, c12.data(), sizeof(float) * c12.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->train_it, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        auto iteration = [&] {
+            sur::gather_batch(c->cap_buf, rows, c->sur_stats, seed, c->train_it, batch, c->bx, c->bt, c->stream);
+            sur::mlp_grad(c->net, c->bx, c->bt, batch, c->work, c->stream);
+            sur::loss_sum(c->work.loss_rows, batch, 1.f / ((float)batch * sur::NOUT), c->bloss, c->stream,
+                          c->train_it);
+            sur::mlp_adam(c->net, c->work, lr, c->adam_c12, c->train_it, c->stream);
+            sur::next_iteration(c->train_it, c->stream);
+        };
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool graphed = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+        if (graphed) {
+            iteration();
+            graphed = hipStreamEndCapture(c->stream, &graph) == hipSuccess && graph &&
+                      hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        }
+        if (graphed) {
+            for (int it = 0; it < iters; ++it) HIP_TRY(c, hipGraphLaunch(exec, c->stream));
+        } else {
+            (void)hipGetLastError();
+            for (int it = 0; it < iters; ++it) iteration();
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        c->adam_step += iters;
     }
     HIP_TRY(c, hipGetLastError());
     if (losses && iters > 0)

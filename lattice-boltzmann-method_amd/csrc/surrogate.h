@@ -52,8 +52,14 @@ void mlp_forward(const Mlp& net, const uint16_t* X, int M, Work& w, hipStream_t 
 // Gradients of the mean-squared loss over the NOUT outputs for one batch (M
 // multiple of 16) into w.gW / w.gb; per-row losses land in w.loss_rows.
 void mlp_grad(const Mlp& net, const uint16_t* X, const float* T, int M, Work& w, hipStream_t s);
-// One Adam step (step >= 1) with the gradients in w.
-void mlp_adam(Mlp& net, Work& w, float lr, int step, hipStream_t s);
+// One Adam step with the gradients in w; the bias corrections of the step are
+// c12[2 *it], c12[2 *it + 1] (device), filled by adam_corrections (host) for
+// steps first_step .. first_step + iters - 1 (first_step >= 1).
+constexpr float ADAM_B1 = 0.9f, ADAM_B2 = 0.999f, ADAM_EPS = 1e-8f;
+void adam_corrections(int first_step, int iters, float* c12);
+void mlp_adam(Mlp& net, Work& w, float lr, const float* c12, const int* it, hipStream_t s);
+// *it += 1 on the stream (the training loop's iteration counter, device).
+void next_iteration(int* it, hipStream_t s);
 // In-step style inference: Y [M][16] FP32 from normalised X [M][16] FP32 (any
 // M), activations in registers, weights staged in LDS.
 void mlp_infer(const Mlp& net, const float* X, int M, float* Y, hipStream_t s);
@@ -66,12 +72,13 @@ void mlp_infer_bf16(const Mlp& net, const uint16_t* X, int64_t rows, float* Y, h
 constexpr int STATS_BLOCKS = 256;
 void capture_stats(const float* cap, int64_t rows, double* part, float* stats, hipStream_t s,
                    bool envelope_only = false);
-// Training batch of B rows drawn (hash of seed, iteration, row) from the
+// Training batch of B rows drawn (hash of seed, iteration *it, row) from the
 // capture: BF16 normalised features [B][16] and FP32 normalised targets [B][16].
-void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, int it, int B, uint16_t* X,
-                  float* T, hipStream_t s);
-// out[0] = sum(loss_rows[0..B)) * scale, summed in a fixed order.
-void loss_sum(const float* loss_rows, int B, float scale, float* out, hipStream_t s);
+void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, const int* it, int B,
+                  uint16_t* X, float* T, hipStream_t s);
+// out[*idx] (out[0] when idx is null) = sum(loss_rows[0..B)) * scale, summed in
+// a fixed order.
+void loss_sum(const float* loss_rows, int B, float scale, float* out, hipStream_t s, const int* idx = nullptr);
 constexpr int NPARAM = 16 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 16 + 16;
 
 }  // namespace sur

@@ -93,12 +93,23 @@ __global__ __launch_bounds__(64) void k_layer_bwd_x(const uint16_t* dZ, int N, c
 }
 
 // Transposes a BF16 [M][N] matrix into [N][M] (for the weight gradient the
-// batch is the reduction index).
-__global__ void k_transpose_bf(const uint16_t* in, int M, int N, uint16_t* out)
+// batch is the reduction index): 64 x 64 tiles through LDS, so that the
+// reads run along rows of `in` and the writes along rows of `out`.
+// grid (ceil(M / 64), ceil(N / 64)), 256 threads.
+__global__ __launch_bounds__(256) void k_transpose_bf(const uint16_t* in, int M, int N, uint16_t* out)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
-    for (int n = 0; n < N; ++n) out[(size_t)n * M + m] = in[(size_t)m * N + n];
+    __shared__ uint16_t tile[64][65];
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int m = m0 + r, n = n0 + tx;
+        tile[r][tx] = (m < M && n < N) ? in[(size_t)m * N + n] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int n = n0 + r, m = m0 + tx;
+        if (n < N && m < M) out[(size_t)n * M + m] = tile[tx][r];
+    }
 }
 
 // Partial weight gradient over one chunk of rows: dW[n][k] = sum_m dZ[m][n] X[m][k]
@@ -146,12 +157,14 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* dZT, int M, f
     if (threadIdx.x == 0) db[nrow] = red[0];
 }
 
-// Adam on FP32 master weights; refreshes the BF16 copies W and W^T.
+// Adam on FP32 master weights; refreshes the BF16 copies W and W^T.  The
+// bias corrections of iteration *it are c12[2 it] and c12[2 it + 1].
 __global__ void k_adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
-                       float c1, float c2, uint16_t* wb, uint16_t* wbT, int rows, int cols)
+                       const float* c12, const int* it, uint16_t* wb, uint16_t* wbT, int rows, int cols)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const float c1 = c12[2 * *it], c2 = c12[2 * *it + 1];
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -366,11 +379,12 @@ __device__ inline uint64_t splitmix(uint64_t z)
     return z ^ (z >> 31);
 }
 
-__global__ void k_gather(const float* cap, int64_t rows, const float* st, uint64_t seed, int it, int B, uint16_t* X,
-                         float* T)
+__global__ void k_gather(const float* cap, int64_t rows, const float* st, uint64_t seed, const int* itp, int B,
+                         uint16_t* X, float* T)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
+    const int it = *itp;
     const uint64_t h = splitmix(seed ^ splitmix(((uint64_t)it << 32) | (uint64_t)b));
     const int64_t r = (int64_t)(h % (uint64_t)rows);
     const float* row = cap + r * CAP_STRIDE;
@@ -382,13 +396,18 @@ __global__ void k_gather(const float* cap, int64_t rows, const float* st, uint64
     }
 }
 
-void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, int it, int B, uint16_t* X,
-                  float* T, hipStream_t s)
+void gather_batch(const float* cap, int64_t rows, const float* stats, uint64_t seed, const int* it, int B,
+                  uint16_t* X, float* T, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gather, dim3(cdiv(B, 256)), dim3(256), 0, s, cap, rows, stats, seed, it, B, X, T);
 }
 
-__global__ __launch_bounds__(256) void k_loss_sum(const float* lr, int B, float scale, float* out)
+__global__ void k_next_iteration(int* it) { *it += 1; }
+
+void next_iteration(int* it, hipStream_t s) { hipLaunchKernelGGL(k_next_iteration, dim3(1), dim3(1), 0, s, it); }
+
+// out[*idx] (out[0] without idx) = sum(lr[0..B)) * scale
+__global__ __launch_bounds__(256) void k_loss_sum(const float* lr, int B, float scale, float* out, const int* idx)
 {
     __shared__ float red[256];
     float s = 0.f;
@@ -399,12 +418,12 @@ __global__ __launch_bounds__(256) void k_loss_sum(const float* lr, int B, float 
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0] * scale;
+    if (threadIdx.x == 0) out[idx ? *idx : 0] = red[0] * scale;
 }
 
-void loss_sum(const float* loss_rows, int B, float scale, float* out, hipStream_t s)
+void loss_sum(const float* loss_rows, int B, float scale, float* out, hipStream_t s, const int* idx)
 {
-    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, loss_rows, B, scale, out);
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, s, loss_rows, B, scale, out, idx);
 }
 
 void mlp_forward(const Mlp& net, const uint16_t* X, int M, Work& w, hipStream_t s)
@@ -428,8 +447,8 @@ void mlp_grad(const Mlp& net, const uint16_t* X, const float* T, int M, Work& w,
     for (int L = NL - 1; L >= 0; --L) {
         const int K = DIMS[L], N = DIMS[L + 1];
         const uint16_t* Xin = L == 0 ? X : w.H[L - 1];
-        hipLaunchKernelGGL(k_transpose_bf, dim3(cdiv(M, 256)), dim3(256), 0, s, w.dZ[L], M, N, w.dZT);
-        hipLaunchKernelGGL(k_transpose_bf, dim3(cdiv(M, 256)), dim3(256), 0, s, Xin, M, K, w.XT);
+        hipLaunchKernelGGL(k_transpose_bf, dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, s, w.dZ[L], M, N, w.dZT);
+        hipLaunchKernelGGL(k_transpose_bf, dim3(cdiv(M, 64), cdiv(K, 64)), dim3(256), 0, s, Xin, M, K, w.XT);
         const int nchunk = cdiv(M, w.chunk);
         hipLaunchKernelGGL(k_wgrad, dim3(N / 16, K / 16, nchunk), dim3(64), 0, s, w.dZT, w.XT, M, N, K, w.chunk,
                            w.part);
@@ -441,16 +460,24 @@ void mlp_grad(const Mlp& net, const uint16_t* X, const float* T, int M, Work& w,
     }
 }
 
-void mlp_adam(Mlp& net, Work& w, float lr, int step, hipStream_t s)
+void adam_corrections(int first_step, int iters, float* c12)
 {
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    for (int k = 0; k < iters; ++k) {
+        const float step = (float)(first_step + k);
+        c12[2 * k] = 1.f - powf(ADAM_B1, step);
+        c12[2 * k + 1] = 1.f - powf(ADAM_B2, step);
+    }
+}
+
+void mlp_adam(Mlp& net, Work& w, float lr, const float* c12, const int* it, hipStream_t s)
+{
+    const float b1 = ADAM_B1, b2 = ADAM_B2, eps = ADAM_EPS;
     for (int L = 0; L < NL; ++L) {
         const int n = DIMS[L] * DIMS[L + 1];
         hipLaunchKernelGGL(k_adam, dim3(cdiv(n, 256)), dim3(256), 0, s, net.W[L], w.gW[L], net.mW[L], net.vW[L], n, lr,
-                           b1, b2, eps, c1, c2, net.Wb[L], net.WbT[L], DIMS[L + 1], DIMS[L]);
+                           b1, b2, eps, c12, it, net.Wb[L], net.WbT[L], DIMS[L + 1], DIMS[L]);
         hipLaunchKernelGGL(k_adam, dim3(cdiv(DIMS[L + 1], 256)), dim3(256), 0, s, net.b[L], w.gb[L], net.mb[L],
-                           net.vb[L], DIMS[L + 1], lr, b1, b2, eps, c1, c2, (uint16_t*)nullptr, (uint16_t*)nullptr, 0,
+                           net.vb[L], DIMS[L + 1], lr, b1, b2, eps, c12, it, (uint16_t*)nullptr, (uint16_t*)nullptr, 0,
                            1);
         hipLaunchKernelGGL(k_to_frag, dim3(cdiv(n, 256)), dim3(256), 0, s, net.W[L], DIMS[L + 1], DIMS[L],
                            net.Wf + frag_offset(L));

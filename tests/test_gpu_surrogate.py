@@ -342,8 +342,9 @@ def _restore(lat, snap):
             lat.set_populations(k[1], v)
 
 
-def _mass_fractions(lat):
-    rho_a = lat.get_species_field("rho").reshape(9, -1)
+def _mass_fractions(lat, F):
+    """Y_a of the fluid cells F (the ghost cells hold no species)."""
+    rho_a = lat.get_species_field("rho").reshape(9, -1)[:, F]
     return rho_a / rho_a.sum(axis=0, keepdims=True)
 
 
@@ -372,7 +373,7 @@ def test_trained_surrogate_gain_over_plain_integrator():
     lat.step(9)
     t_plain, n_plain = lat.profile_read("mc_chemistry_plain")
     T_plain = lat.get_field("T").ravel()[F]
-    Y_plain = _mass_fractions(lat)[:, F]
+    Y_plain = _mass_fractions(lat, F)
 
     _restore(lat, snap)
     lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
@@ -382,7 +383,7 @@ def test_trained_surrogate_gain_over_plain_integrator():
     t_sur, n_sur = lat.profile_read("mc_chemistry_surrogate")
     t_fb, _ = lat.profile_read("mc_chemistry_fallback")
     T_sur = lat.get_field("T").ravel()[F]
-    Y_sur = _mass_fractions(lat)[:, F]
+    Y_sur = _mass_fractions(lat, F)
     stats = lat.surrogate_stats()
 
     untrained = lattice(fl)
