@@ -175,28 +175,48 @@ MC_POP_TRAFFIC = {"mc_collide_species": 2 * 9 * 27, "mc_collide_species_push": 2
                   "mc_moments": 10 * 27}
 
 
-def cpu_baseline_mixing():
-    """The oracle's numpy restatement of the multicomponent step (the
-    reference itself needs Cantera, which is absent) on a bounded sample of
-    the same workload: a 32x64x8 mixing layer, 2 steps, one process."""
+def cpu_info():
+    """CPU model and the logical CPUs this process may run on."""
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, allowed
+
+
+def cpu_baseline_mixing(threads):
+    """The compiled C++/OpenMP restatement of the multicomponent step
+    (oracle/mc_cpu.cpp: FP64, plain chemistry, the same thermo, kinetics and
+    transport tables; pinned against the numpy oracle) on a bounded sample of
+    the same workload: 64x128x128 cells (a quarter of the 256x128x128 layer)
+    of its periodic twin cases.double_mixing_layer (the same streams and
+    profiles, the fuel stream between two layers, so that the box is
+    periodic: the restatement streams periodic boxes only; the GPU's NSCBC
+    outflow pass is 3.6 % of its step), 2 steps.  The reference itself needs
+    Cantera, which is absent."""
+    import tempfile
     from lbm_amd import cases
-    from oracle.multicomp import MultiStateBC, Units
-    nx, ny, nz, steps = 32, 64, 8, 2
-    fl = cases.mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
-    s = MultiStateBC(nx + 2, ny + 2, nz + 2, Units(*cases.UNITS_FLAME))
-    s.type[:] = fl["type"].ravel()
-    for k in ("rho", "u", "v", "w", "p", "T"):
-        getattr(s, k)[:] = fl[k].ravel()
-    s.X[:] = fl["X"].reshape(9, -1)
-    s.init()
-    s.moments()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        s.step(True)
-    sec = time.perf_counter() - t0
-    return {"value": nx * ny * nz * steps / sec / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"numpy restatement of the multicomponent step (oracle/multicomp.py), {nx}x{ny}x{nz} "
-                      f"reacting mixing layer, {steps} steps, {sec:.1f} s, one process"}
+    from oracle import mc_cpu
+    nx, ny, nz, steps = 64, 128, 128, 2
+    model, allowed = cpu_info()
+    threads = max(1, min(threads, allowed))
+    fl = cases.double_mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "mixing_slab.bin")
+        mc_cpu.write_case(path, fl, *cases.UNITS_FLAME, True)
+        r = mc_cpu.run(path, steps, threads, timeout=600)
+    return {"value": round(r["MLUPS"], 6), "unit": "MLUPS", "cores": r["threads"], "kind": "port",
+            "sample": f"compiled C++/OpenMP multicomponent step (oracle/mc_cpu.cpp, FP64, plain chemistry), "
+                      f"{nx}x{ny}x{nz} periodic twin of the reacting mixing layer, {steps} steps, "
+                      f"{r['seconds']:.1f} s, {r['threads']} OpenMP threads of {allowed} allowed, {model}"}
 
 
 def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_256x128x128"):
@@ -554,7 +574,7 @@ def main():
         rec = mixing_layer_bench(args)
         if not args.no_cpu_baseline:
             try:
-                rec["cpu_baseline"] = cpu_baseline_mixing()
+                rec["cpu_baseline"] = cpu_baseline_mixing(args.cpu_threads)
             except Exception as e:  # the baseline is reported, never required
                 rec["cpu_baseline"] = {"error": str(e)[:200]}
         print(json.dumps(rec))

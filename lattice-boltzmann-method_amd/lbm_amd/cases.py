@@ -331,6 +331,36 @@ def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, no
     return fl
 
 
+def double_mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, noise=0.01, seed=20261206,
+                        X_fuel=None, X_ox=None, minim=1e-6):
+    """Periodic twin of mixing_layer: the same streams and profiles, with the
+    fuel stream in the middle half of y between two tanh layers at ny/4 and
+    3 ny/4, so that the box is periodic in every direction (TYPE_P ghosts)
+    and smooth across its faces; u = U_s (2 s - 1) as in mixing_layer."""
+    rng = np.random.default_rng(seed)
+    fl = default_fields(nx, ny, nz)
+    periodic_ghosts(fl)
+    t = fl["type"]
+    shape = t.shape
+    delta = delta or max(ny / 32.0, 1.0)
+    j = np.arange(shape[1], dtype=np.float64) - 0.5  # interior cell j sits at j - 1/2 of [0, ny)
+    y1, y2 = 0.25 * ny, 0.75 * ny
+    s = 1.0 - 0.5 * (np.tanh((j - y1) / delta) - np.tanh((j - y2) / delta))  # 1 oxidiser .. 0 fuel
+    s3 = np.broadcast_to(s[None, :, None], shape)
+    Xf = mole_vector(_all_species(X_fuel or X_FUEL_ML, minim))
+    Xo = mole_vector(_all_species(X_ox or X_AIR, minim))
+    fl["X"] = (1.0 - s3)[None] * Xf[:, None, None, None] + s3[None] * Xo[:, None, None, None]
+    fl["T"][...] = (T_fuel + (T_ox - T_fuel) * s3) / LATTICE_K
+    fl["p"][...] = lattice_p(ONE_ATM)
+    fl["u"][...] = U_s * (2.0 * s3 - 1.0)
+    fl["v"][...] = 0.0
+    fl["w"][...] = 0.0
+    m = t == TYPE_F
+    for k in ("u", "v", "w"):
+        fl[k][m] += noise * U_s * rng.uniform(-1.0, 1.0, int(m.sum()))
+    return fl
+
+
 # phi = 1 burnt state (setup.cpp:2106-2116)
 T_BURNT_PHI1 = 2387.637
 X_BURNT_PHI1 = {"H2": 0.014565, "H": 0.00181, "O": 0.00060738, "O2": 0.0056094, "OH": 0.0072848, "H2O": 0.32437,
