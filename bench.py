@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-train-graph", action="store_true",
+                    help="launch the surrogate's training iterations one by one instead of replaying a HIP "
+                         "graph (for rocprofv3 kernel tracing, which faults inside the graph calls)")
     ap.add_argument("--mode", default="fused", choices=["fused", "strict", "inplace"],
                     help="multicomponent workloads: the fused mode (Collide_Species writes the streamed slots; "
                          "bitwise equal to strict), one kernel per reference call, or the in-place mode (one "
@@ -538,6 +541,8 @@ def turbulence_report(args):
 
 def main():
     args = parse()
+    if args.no_train_graph:
+        os.environ["LBM_AMD_TRAIN_GRAPH"] = "0"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}); the multi-GPU x-slab "

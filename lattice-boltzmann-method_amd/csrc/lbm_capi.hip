@@ -1897,7 +1897,12 @@ int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t see
         };
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        bool graphed = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+        // LBM_AMD_TRAIN_GRAPH=0 launches the iterations one by one (the same
+        // kernels and arguments): rocprofv3's kernel tracing of this ROCm
+        // faults on the host inside the graph calls
+        const char* env = getenv("LBM_AMD_TRAIN_GRAPH");
+        bool graphed = !(env && env[0] == '0') &&
+                       hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
         if (graphed) {
             iteration();
             graphed = hipStreamEndCapture(c->stream, &graph) == hipSuccess && graph &&
