@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # surrogate maintenance in the multicomponent runs: refresh after a window of
 # steps in which more than REFRESH_RATE of the chemistry updates went to the
 # plain integrator
-REFRESH_WINDOW, REFRESH_RATE, REFRESH_ITERS = 25, 0.005, 200
+REFRESH_WINDOW, REFRESH_RATE, REFRESH_ITERS = 10, 0.003, 100
 
 
 def parse():
@@ -61,6 +61,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--refresh-window", type=int, default=REFRESH_WINDOW,
+                    help="surrogate maintenance: steps per fallback-rate window")
+    ap.add_argument("--refresh-rate", type=float, default=REFRESH_RATE,
+                    help="surrogate maintenance: fallback + audit-failure share of a window that triggers a refresh")
+    ap.add_argument("--refresh-iters", type=int, default=REFRESH_ITERS,
+                    help="surrogate maintenance: training iterations per refresh")
     ap.add_argument("--no-train-graph", action="store_true",
                     help="launch the surrogate's training iterations one by one instead of replaying a HIP "
                          "graph (for rocprofv3 kernel tracing, which faults inside the graph calls)")
@@ -259,14 +265,14 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
         # audited ones) join the training set, and a window of REFRESH_WINDOW
         # steps with more than REFRESH_RATE of the updates falling back or
         # failing their audit triggers a refresh (envelope widened to every
-        # captured state, REFRESH_ITERS more training iterations), inside the
-        # timed steps
+        # captured state, REFRESH_ITERS more training iterations) on a second
+        # stream beside the steps, inside the timed region
         lat.surrogate_capture(cap_steps * cells // cap_stride + 2 * cells, 1 << 30)
         maint = {"step_s": 0.0, "refresh_s": 0.0, "refreshes": 0}
 
         def run(k):
-            st, rs, nr = lat.step_with_refresh(k, window=REFRESH_WINDOW, max_rate=REFRESH_RATE, iters=REFRESH_ITERS,
-                                               batch=batch)
+            st, rs, nr = lat.step_with_refresh(k, window=args.refresh_window, max_rate=args.refresh_rate,
+                                               iters=args.refresh_iters, batch=batch, overlap=True)
             maint["step_s"] += st
             maint["refresh_s"] += rs
             maint["refreshes"] += nr
@@ -288,8 +294,9 @@ def mixing_layer_bench(args, nx=256, ny=128, nz=128, tag="reacting_mixing_layer_
                               "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-50:].mean()),
                               "captured_states": None, **sstats}
         extra["surrogate"]["captured_states"] = extra["surrogate"].pop("captured")
-        extra["surrogate"]["maintenance"] = {"window": REFRESH_WINDOW, "max_rate": REFRESH_RATE,
-                                             "iters": REFRESH_ITERS, "refreshes": maint["refreshes"],
+        extra["surrogate"]["maintenance"] = {"window": args.refresh_window, "max_rate": args.refresh_rate,
+                                             "iters": args.refresh_iters, "overlapped": True,
+                                             "refreshes": maint["refreshes"],
                                              "refresh_s": round(maint["refresh_s"], 3),
                                              "step_s": round(maint["step_s"], 3)}
         extra["fallback_rate"] = round((sstats["fallback"] + sstats["audit_failures"]) /

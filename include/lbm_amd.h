@@ -287,6 +287,15 @@ int lbm_surrogate_enable(lbm_ctx* ctx, int on, int audit_stride, double audit_to
  * (envelope fallbacks and audits) are captured, so a refresh teaches the
  * network the states it fell back on. */
 int lbm_surrogate_refresh(lbm_ctx* ctx, int iters, int batch, float lr, uint64_t seed, float* losses);
+/* The same refresh without stopping the steps: a copy of the network and
+ * envelope is trained on a second stream from the rows captured so far, while
+ * the steps keep using the serving network; lbm_surrogate_refresh_poll applies
+ * the result (network and envelope together, ordered before the next step) once
+ * the training has finished, *applied = 1 (with wait != 0 it waits for it).
+ * One refresh at a time (LBM_E_STATE otherwise); calls that change the network,
+ * the envelope or the capture buffer wait for and apply an unfinished one. */
+int lbm_surrogate_refresh_async(lbm_ctx* ctx, int iters, int batch, float lr, uint64_t seed);
+int lbm_surrogate_refresh_poll(lbm_ctx* ctx, int wait, int* applied);
 int lbm_surrogate_read_stats(lbm_ctx* ctx, lbm_surrogate_stats* out);
 int lbm_surrogate_get_params(lbm_ctx* ctx, float* host);
 int lbm_surrogate_set_params(lbm_ctx* ctx, const float* host);

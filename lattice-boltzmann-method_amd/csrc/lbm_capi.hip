@@ -162,6 +162,16 @@ struct lbm_ctx {
     float* bloss = nullptr;  // per-iteration losses
     float* adam_c12 = nullptr; // per-iteration Adam bias corrections [2][lcap] (device)
     int* train_it = nullptr;   // the training loop's iteration counter (device)
+    std::vector<float> c12_host;  // host copy of adam_c12 while its upload is in flight
+    hipGraph_t tgraph = nullptr;  // the captured training iteration of the last call
+    hipGraphExec_t texec = nullptr;
+    // asynchronous refresh: a training copy of the network on its own stream
+    sur::Mlp tnet;
+    float* stats_next = nullptr;  // [SS_SIZE] the envelope the copy is trained with
+    hipStream_t tstream = nullptr;
+    hipEvent_t ev_tstart = nullptr, ev_tdone = nullptr;
+    bool refreshing = false;
+    int refresh_iters = 0;
     int bcap = 0, lcap = 0;
 
     bool prof = false;
@@ -444,6 +454,8 @@ int set_multi_reference_state(lbm_ctx* c)
     return LBM_OK;
 }
 
+int refresh_finish(lbm_ctx* c, bool wait, int* applied);
+
 void release(lbm_ctx* c)
 {
     for (auto& r : c->pending) {
@@ -463,6 +475,16 @@ void release(lbm_ctx* c)
         if (p) (void)hipFree(p);
     if (c->low_count_host) (void)hipHostFree(c->low_count_host);
     if (c->ev_save) (void)hipEventDestroy(c->ev_save);
+    if (c->tstream) {  // an asynchronous refresh may still read the capture rows
+        (void)hipStreamSynchronize(c->tstream);
+        (void)hipStreamDestroy(c->tstream);
+    }
+    if (c->texec) (void)hipGraphExecDestroy(c->texec);
+    if (c->tgraph) (void)hipGraphDestroy(c->tgraph);
+    if (c->ev_tstart) (void)hipEventDestroy(c->ev_tstart);
+    if (c->ev_tdone) (void)hipEventDestroy(c->ev_tdone);
+    if (c->tnet.Wf) sur::mlp_free(c->tnet);
+    if (c->stats_next) (void)hipFree(c->stats_next);
     if (c->net_ready) sur::mlp_free(c->net);
     if (c->work.cap) sur::work_free(c->work);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1780,6 +1802,7 @@ int64_t lbm_surrogate_param_count(void) { return sur::NPARAM; }
 int lbm_surrogate_init(lbm_ctx* c, uint64_t seed)
 {
     if (!c) return LBM_E_ARG;
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;  // no refresh in flight
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->net_ready) sur::mlp_free(c->net);
@@ -1795,6 +1818,7 @@ int lbm_surrogate_init(lbm_ctx* c, uint64_t seed)
 int lbm_surrogate_capture(lbm_ctx* c, int64_t max_rows, int stride)
 {
     if (!c) return LBM_E_ARG;
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;  // no refresh in flight
     if (int rc = need_reacting(c)) return rc;
     if (max_rows < 0 || max_rows > ((int64_t)1 << 31) || stride < 0)
         return fail(c, LBM_E_ARG, "capture needs 0 <= max_rows <= 2^31 and stride >= 0");
@@ -1849,21 +1873,11 @@ int lbm_surrogate_get_capture(lbm_ctx* c, float* host, int64_t max_rows, int64_t
     return LBM_OK;
 }
 
-int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t seed, float* losses)
+namespace {
+
+// Buffers of a training call of `iters` iterations on `batch` rows.
+int train_buffers(lbm_ctx* c, int iters, int batch)
 {
-    if (!c) return LBM_E_ARG;
-    if (iters < 0 || batch < 16 || batch % 16 || batch > (1 << 20) || !(lr > 0.f))
-        return fail(c, LBM_E_ARG, "train needs iters >= 0, batch a multiple of 16 in [16, 2^20], lr > 0");
-    if (int rc = sur_common(c)) return rc;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    unsigned long long have = 0;
-    if (c->cap_count) HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
-    const int64_t rows = std::min<int64_t>((int64_t)have, c->cap_max);
-    if (rows < 1) return fail(c, LBM_E_STATE, "no captured rows to train on (lbm_surrogate_capture, then step)");
-    if (!c->norm_ready) {
-        sur::capture_stats(c->cap_buf, rows, c->stats_part, c->sur_stats, c->stream);
-        c->norm_ready = true;
-    }
     if (int rc = sur_work(c, batch)) return rc;
     if (int rc = sur_batch(c, batch)) return rc;
     if (iters > c->lcap || !c->adam_c12) {
@@ -1876,47 +1890,116 @@ int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t see
         c->lcap = std::max(iters, 1);
     }
     if (!c->train_it) HIP_TRY(c, hipMalloc((void**)&c->train_it, sizeof(int)));
+    return LBM_OK;
+}
+
+void train_release(lbm_ctx* c)
+{
+    if (c->texec) (void)hipGraphExecDestroy(c->texec);
+    if (c->tgraph) (void)hipGraphDestroy(c->tgraph);
+    c->texec = nullptr;
+    c->tgraph = nullptr;
+}
+
+// Enqueues `iters` training iterations of `net` on stream st, from the first
+// `rows` capture rows normalised by `stats`, with Adam steps adam_step + 1 ...
+// One iteration (batch draw, forward and backward passes, loss, Adam: 45
+// kernels) is captured once as a HIP graph and replayed `iters` times; the
+// iteration index and the step's Adam corrections are read on the device.
+// The graph is kept in c->tgraph / c->texec until train_release, after st
+// has finished.
+int train_enqueue(lbm_ctx* c, sur::Mlp& net, const float* stats, int64_t rows, int iters, int batch, float lr,
+                  uint64_t seed, hipStream_t st)
+{
+    train_release(c);
+    c->c12_host.assign(2 * (size_t)iters, 0.f);
+    sur::adam_corrections(c->adam_step + 1, iters, c->c12_host.data());
+    static const int zero = 0;
+    HIP_TRY(c, hipMemcpyAsync(c->adam_c12, c->c12_host.data(), sizeof(float) * c->c12_host.size(),
+                              hipMemcpyHostToDevice, st));
+    HIP_TRY(c, hipMemcpyAsync(c->train_it, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    auto iteration = [&] {
+        sur::gather_batch(c->cap_buf, rows, stats, seed, c->train_it, batch, c->bx, c->bt, st);
+        sur::mlp_grad(net, c->bx, c->bt, batch, c->work, st);
+        sur::loss_sum(c->work.loss_rows, batch, 1.f / ((float)batch * sur::NOUT), c->bloss, st, c->train_it);
+        sur::mlp_adam(net, c->work, lr, c->adam_c12, c->train_it, st);
+        sur::next_iteration(c->train_it, st);
+    };
+    // LBM_AMD_TRAIN_GRAPH=0 launches the iterations one by one (the same
+    // kernels and arguments): rocprofv3's kernel tracing of this ROCm faults
+    // on the host inside the graph calls
+    const char* env = getenv("LBM_AMD_TRAIN_GRAPH");
+    bool graphed = !(env && env[0] == '0') && hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
+    if (graphed) {
+        iteration();
+        graphed = hipStreamEndCapture(st, &c->tgraph) == hipSuccess && c->tgraph &&
+                  hipGraphInstantiate(&c->texec, c->tgraph, nullptr, nullptr, 0) == hipSuccess;
+    }
+    if (graphed) {
+        for (int it = 0; it < iters; ++it) HIP_TRY(c, hipGraphLaunch(c->texec, st));
+    } else {
+        (void)hipGetLastError();
+        train_release(c);
+        for (int it = 0; it < iters; ++it) iteration();
+    }
+    HIP_TRY(c, hipGetLastError());
+    return LBM_OK;
+}
+
+int64_t captured_rows(lbm_ctx* c)
+{
+    unsigned long long have = 0;
+    if (c->cap_count && hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return std::min<int64_t>((int64_t)have, c->cap_max);
+}
+
+// Applies a finished asynchronous refresh (the network and envelope it
+// trained replace the serving ones, on the step stream); with wait, waits
+// for it first.  *applied = 1 when this call applied one.
+int refresh_finish(lbm_ctx* c, bool wait, int* applied)
+{
+    if (applied) *applied = 0;
+    if (!c->refreshing) return LBM_OK;
+    if (wait) {
+        HIP_TRY(c, hipEventSynchronize(c->ev_tdone));
+    } else {
+        const hipError_t q = hipEventQuery(c->ev_tdone);
+        if (q == hipErrorNotReady) return LBM_OK;
+        if (q != hipSuccess) return fail(c, LBM_E_HIP, std::string("surrogate refresh: ") + hipGetErrorString(q));
+    }
+    train_release(c);
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_tdone, 0));
+    HIP_TRY(c, sur::mlp_copy(c->net, c->tnet, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->sur_stats, c->stats_next, sizeof(float) * sur::SS_SIZE, hipMemcpyDeviceToDevice,
+                              c->stream));
+    c->adam_step += c->refresh_iters;
+    c->refreshing = false;
+    if (applied) *applied = 1;
+    return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_surrogate_train(lbm_ctx* c, int iters, int batch, float lr, uint64_t seed, float* losses)
+{
+    if (!c) return LBM_E_ARG;
+    if (iters < 0 || batch < 16 || batch % 16 || batch > (1 << 20) || !(lr > 0.f))
+        return fail(c, LBM_E_ARG, "train needs iters >= 0, batch a multiple of 16 in [16, 2^20], lr > 0");
+    if (int rc = sur_common(c)) return rc;
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int64_t rows = captured_rows(c);
+    if (rows < 0) return fail(c, LBM_E_HIP, "reading the capture count");
+    if (rows < 1) return fail(c, LBM_E_STATE, "no captured rows to train on (lbm_surrogate_capture, then step)");
+    if (!c->norm_ready) {
+        sur::capture_stats(c->cap_buf, rows, c->stats_part, c->sur_stats, c->stream);
+        c->norm_ready = true;
+    }
+    if (int rc = train_buffers(c, iters, batch)) return rc;
     if (iters > 0) {
-        // one training iteration (batch draw, forward and backward passes,
-        // loss, Adam: 45 kernels) is captured once as a HIP graph and replayed
-        // `iters` times; the iteration index and the step's Adam corrections
-        // are read on the device
-        std::vector<float> c12(2 * (size_t)iters);
-        sur::adam_corrections(c->adam_step + 1, iters, c12.data());
-        const int zero = 0;
-        HIP_TRY(c, hipMemcpyAsync(c->adam_c12, c12.data(), sizeof(float) * c12.size(), hipMemcpyHostToDevice,
-                                  c->stream));
-        HIP_TRY(c, hipMemcpyAsync(c->train_it, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-        auto iteration = [&] {
-            sur::gather_batch(c->cap_buf, rows, c->sur_stats, seed, c->train_it, batch, c->bx, c->bt, c->stream);
-            sur::mlp_grad(c->net, c->bx, c->bt, batch, c->work, c->stream);
-            sur::loss_sum(c->work.loss_rows, batch, 1.f / ((float)batch * sur::NOUT), c->bloss, c->stream,
-                          c->train_it);
-            sur::mlp_adam(c->net, c->work, lr, c->adam_c12, c->train_it, c->stream);
-            sur::next_iteration(c->train_it, c->stream);
-        };
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        // LBM_AMD_TRAIN_GRAPH=0 launches the iterations one by one (the same
-        // kernels and arguments): rocprofv3's kernel tracing of this ROCm
-        // faults on the host inside the graph calls
-        const char* env = getenv("LBM_AMD_TRAIN_GRAPH");
-        bool graphed = !(env && env[0] == '0') &&
-                       hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
-        if (graphed) {
-            iteration();
-            graphed = hipStreamEndCapture(c->stream, &graph) == hipSuccess && graph &&
-                      hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-        }
-        if (graphed) {
-            for (int it = 0; it < iters; ++it) HIP_TRY(c, hipGraphLaunch(exec, c->stream));
-        } else {
-            (void)hipGetLastError();
-            for (int it = 0; it < iters; ++it) iteration();
-        }
+        if (int rc = train_enqueue(c, c->net, c->sur_stats, rows, iters, batch, lr, seed, c->stream)) return rc;
         HIP_TRY(c, hipStreamSynchronize(c->stream));
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        train_release(c);
         c->adam_step += iters;
     }
     HIP_TRY(c, hipGetLastError());
@@ -1931,13 +2014,53 @@ int lbm_surrogate_refresh(lbm_ctx* c, int iters, int batch, float lr, uint64_t s
     if (!c) return LBM_E_ARG;
     if (!c->norm_ready) return fail(c, LBM_E_STATE, "no trained network to refresh (lbm_surrogate_train first)");
     HIP_TRY(c, hipSetDevice(c->cfg.device));
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    unsigned long long have = 0;
-    if (c->cap_count) HIP_TRY(c, hipMemcpy(&have, c->cap_count, sizeof(have), hipMemcpyDeviceToHost));
-    const int64_t rows = std::min<int64_t>((int64_t)have, c->cap_max);
+    const int64_t rows = captured_rows(c);
     if (rows < 1) return fail(c, LBM_E_STATE, "no captured rows");
     sur::capture_stats(c->cap_buf, rows, c->stats_part, c->sur_stats, c->stream, true);
     return lbm_surrogate_train(c, iters, batch, lr, seed, losses);
+}
+
+int lbm_surrogate_refresh_async(lbm_ctx* c, int iters, int batch, float lr, uint64_t seed)
+{
+    if (!c) return LBM_E_ARG;
+    if (iters < 1 || batch < 16 || batch % 16 || batch > (1 << 20) || !(lr > 0.f))
+        return fail(c, LBM_E_ARG, "refresh needs iters >= 1, batch a multiple of 16 in [16, 2^20], lr > 0");
+    if (!c->norm_ready) return fail(c, LBM_E_STATE, "no trained network to refresh (lbm_surrogate_train first)");
+    if (c->refreshing) return fail(c, LBM_E_STATE, "a refresh is in flight (lbm_surrogate_refresh_poll)");
+    if (int rc = sur_common(c)) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int64_t rows = captured_rows(c);
+    if (rows < 1) return fail(c, LBM_E_STATE, "no captured rows");
+    if (int rc = train_buffers(c, iters, batch)) return rc;
+    if (!c->tnet.Wf) {
+        HIP_TRY(c, sur::mlp_alloc_empty(c->tnet));
+        HIP_TRY(c, hipMalloc((void**)&c->stats_next, sizeof(float) * sur::SS_SIZE));
+        HIP_TRY(c, hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking));
+        HIP_TRY(c, hipEventCreateWithFlags(&c->ev_tstart, hipEventDisableTiming));
+        HIP_TRY(c, hipEventCreateWithFlags(&c->ev_tdone, hipEventDisableTiming));
+    }
+    // the training copy starts from the serving network and envelope; the
+    // rows [0, rows) were written before this call, later steps append past them
+    HIP_TRY(c, sur::mlp_copy(c->tnet, c->net, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->stats_next, c->sur_stats, sizeof(float) * sur::SS_SIZE, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_tstart, c->stream));
+    HIP_TRY(c, hipStreamWaitEvent(c->tstream, c->ev_tstart, 0));
+    sur::capture_stats(c->cap_buf, rows, c->stats_part, c->stats_next, c->tstream, true);
+    if (int rc = train_enqueue(c, c->tnet, c->stats_next, rows, iters, batch, lr, seed, c->tstream)) return rc;
+    HIP_TRY(c, hipEventRecord(c->ev_tdone, c->tstream));
+    c->refresh_iters = iters;
+    c->refreshing = true;
+    return LBM_OK;
+}
+
+int lbm_surrogate_refresh_poll(lbm_ctx* c, int wait, int* applied)
+{
+    if (!c || !applied) return fail(c, LBM_E_ARG, "null argument");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    return refresh_finish(c, wait != 0, applied);
 }
 
 int lbm_surrogate_enable(lbm_ctx* c, int on, int audit_stride, double audit_tol)
@@ -2004,6 +2127,7 @@ int lbm_surrogate_get_params(lbm_ctx* c, float* host)
 int lbm_surrogate_set_params(lbm_ctx* c, const float* host)
 {
     if (!c || !host) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;  // no refresh in flight
     if (int rc = sur_common(c)) return rc;
     if (int rc = copy_params(c, const_cast<float*>(host), false)) return rc;
     for (int L = 0; L < sur::NL; ++L) {
@@ -2032,6 +2156,7 @@ int lbm_surrogate_get_norm(lbm_ctx* c, float* host96)
 int lbm_surrogate_set_norm(lbm_ctx* c, const float* host96)
 {
     if (!c || !host96) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;  // no refresh in flight
     if (int rc = sur_common(c)) return rc;
     for (int q = 0; q < 16; ++q)
         if (!(host96[sur::SS_FSTD + q] > 0.f) || !(host96[sur::SS_TSTD + q] >= 0.f))
@@ -2123,6 +2248,7 @@ int lbm_surrogate_forward_noalloc(lbm_ctx* c, const float* X, int64_t m, float* 
 int lbm_surrogate_gradient(lbm_ctx* c, const float* X, const float* T, int64_t m, float* grad, float* loss)
 {
     if (!c || !X || !T || !grad || !loss) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = refresh_finish(c, true, nullptr)) return rc;  // shares the training buffers
     if (m < 16 || m % 16 || m > (1 << 20)) return fail(c, LBM_E_ARG, "m must be a multiple of 16 in [16, 2^20]");
     if (int rc = sur_common(c)) return rc;
     if (int rc = sur_work(c, (int)m)) return rc;

@@ -42,6 +42,10 @@ constexpr int CAP_STRIDE = 32;
 enum { SS_FMEAN = 0, SS_FSTD = 16, SS_TMEAN = 32, SS_TSTD = 48, SS_LO = 64, SS_HI = 80, SS_SIZE = 96 };
 
 hipError_t mlp_alloc(Mlp& net, uint64_t seed, hipStream_t s);
+// The arrays of a network, contents undefined; mlp_copy copies every array
+// (weights, biases, Adam moments, BF16 copies, fragments) on stream s.
+hipError_t mlp_alloc_empty(Mlp& net);
+hipError_t mlp_copy(Mlp& dst, const Mlp& src, hipStream_t s);
 void mlp_free(Mlp& net);
 hipError_t work_alloc(Work& w, int cap);
 void work_free(Work& w);

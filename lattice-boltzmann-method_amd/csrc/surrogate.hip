@@ -12,6 +12,8 @@
 #include <stdint.h>
 
 #include "surrogate.h"
+
+#include <utility>
 #include "surrogate_device.h"
 
 namespace lbm {
@@ -549,6 +551,42 @@ hipError_t mlp_alloc(Mlp& net, uint64_t seed, hipStream_t s)
     }
     refresh_bf16(net, s);
     return hipStreamSynchronize(s);
+}
+
+hipError_t mlp_alloc_empty(Mlp& net)
+{
+    hipError_t e;
+    if ((e = hipMalloc((void**)&net.Wf, 2 * FRAG_TOTAL)) != hipSuccess) return e;
+    for (int L = 0; L < NL; ++L) {
+        const int rows = DIMS[L + 1], n = rows * DIMS[L];
+        float** f32s[] = {&net.W[L], &net.mW[L], &net.vW[L]};
+        for (float** p : f32s)
+            if ((e = hipMalloc((void**)p, sizeof(float) * n)) != hipSuccess) return e;
+        float** bs[] = {&net.b[L], &net.mb[L], &net.vb[L]};
+        for (float** p : bs)
+            if ((e = hipMalloc((void**)p, sizeof(float) * rows)) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&net.Wb[L], 2 * n)) != hipSuccess) return e;
+        if ((e = hipMalloc((void**)&net.WbT[L], 2 * n)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t mlp_copy(Mlp& dst, const Mlp& src, hipStream_t s)
+{
+    hipError_t e = hipMemcpyAsync(dst.Wf, src.Wf, 2 * FRAG_TOTAL, hipMemcpyDeviceToDevice, s);
+    for (int L = 0; L < NL && e == hipSuccess; ++L) {
+        const size_t rows = DIMS[L + 1], n = rows * DIMS[L];
+        const std::pair<void*, const void*> f[] = {{dst.W[L], src.W[L]}, {dst.mW[L], src.mW[L]}, {dst.vW[L], src.vW[L]}};
+        for (const auto& pr : f)
+            if (e == hipSuccess) e = hipMemcpyAsync(pr.first, pr.second, sizeof(float) * n, hipMemcpyDeviceToDevice, s);
+        const std::pair<void*, const void*> b[] = {{dst.b[L], src.b[L]}, {dst.mb[L], src.mb[L]}, {dst.vb[L], src.vb[L]}};
+        for (const auto& pr : b)
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(pr.first, pr.second, sizeof(float) * rows, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dst.Wb[L], src.Wb[L], 2 * n, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dst.WbT[L], src.WbT[L], 2 * n, hipMemcpyDeviceToDevice, s);
+    }
+    return e;
 }
 
 void mlp_free(Mlp& net)

@@ -39,7 +39,7 @@ EXPORTS = [
     "lbm_surrogate_get_capture", "lbm_surrogate_train", "lbm_surrogate_enable", "lbm_surrogate_read_stats",
     "lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm", "lbm_surrogate_set_norm",
     "lbm_surrogate_forward", "lbm_surrogate_gradient", "lbm_write_vtk", "lbm_write_restart", "lbm_read_restart",
-    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate", "lbm_set_smoothing", "lbm_surrogate_refresh",
+    "lbm_mechanism", "lbm_species_name", "lbm_set_conjugate", "lbm_set_smoothing", "lbm_surrogate_refresh", "lbm_surrogate_refresh_async", "lbm_surrogate_refresh_poll",
 ]
 
 LBM_VTK_SI, LBM_VTK_MASS_FRACTIONS = 1, 2
@@ -144,6 +144,8 @@ def load_library(path=None, mechanism="h2air"):
     L.lbm_surrogate_get_capture.argtypes = [vp, fp, i64, ctypes.POINTER(i64)]
     L.lbm_surrogate_train.argtypes = [vp, i32, i32, ctypes.c_float, u64, fp]
     L.lbm_surrogate_refresh.argtypes = [vp, i32, i32, ctypes.c_float, u64, fp]
+    L.lbm_surrogate_refresh_async.argtypes = [vp, i32, i32, ctypes.c_float, u64]
+    L.lbm_surrogate_refresh_poll.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int)]
     L.lbm_surrogate_enable.argtypes = [vp, i32, i32, ctypes.c_double]
     L.lbm_surrogate_read_stats.argtypes = [vp, ctypes.POINTER(SurrogateStats)]
     for fn in ("lbm_surrogate_get_params", "lbm_surrogate_set_params", "lbm_surrogate_get_norm",
@@ -489,17 +491,35 @@ class MixtureLattice(Lattice):
                                                   _fp(losses)), "surrogate_refresh")
         return losses[:iters]
 
-    def step_with_refresh(self, nsteps, window=25, max_rate=0.005, iters=300, batch=4096, lr=3e-4):
+    def surrogate_refresh_async(self, iters, batch=4096, lr=3e-4, seed=3):
+        """Starts a refresh on a second stream; the steps continue with the
+        serving network until surrogate_refresh_poll applies the result."""
+        self._check(self._L.lbm_surrogate_refresh_async(self._h, int(iters), int(batch), float(lr), int(seed)),
+                    "surrogate_refresh_async")
+
+    def surrogate_refresh_poll(self, wait=False):
+        """Applies a finished refresh; True when this call applied one."""
+        applied = ctypes.c_int(0)
+        self._check(self._L.lbm_surrogate_refresh_poll(self._h, 1 if wait else 0, ctypes.byref(applied)),
+                    "surrogate_refresh_poll")
+        return bool(applied.value)
+
+    def step_with_refresh(self, nsteps, window=25, max_rate=0.005, iters=300, batch=4096, lr=3e-4, overlap=False):
         """`nsteps` steps in windows of `window`; after a window in which more
         than `max_rate` of the chemistry updates fell back to (or were
         overruled by) the plain integrator, the network is refreshed on the
-        captured rows (lbm_surrogate_refresh; capture must be on).  Returns
-        (step seconds, refresh seconds, refresh count), each timed between
-        device synchronisations."""
+        captured rows (capture must be on): with overlap=False by
+        lbm_surrogate_refresh between two windows, with overlap=True by
+        lbm_surrogate_refresh_async on a second stream while the next windows
+        step with the serving network, the result applied at the first window
+        end after it finished (and at the end).  Returns (step seconds,
+        refresh seconds, refresh count), each timed between device
+        synchronisations; with overlap the training time is inside the steps'."""
         import time
         step_s = refresh_s = 0.0
         refreshes = 0
         done = 0
+        in_flight = False
         prev = self.surrogate_stats()
         while done < nsteps:
             k = min(window, nsteps - done)
@@ -513,11 +533,24 @@ class MixtureLattice(Lattice):
             bad = (st["fallback"] - prev["fallback"]) + (st["audit_failures"] - prev["audit_failures"])
             upd = (st["predicted"] - prev["predicted"]) + (st["fallback"] - prev["fallback"])
             prev = st
-            if done < nsteps and upd > 0 and bad > max_rate * upd:
-                t0 = time.perf_counter()
-                self.surrogate_refresh(iters, batch=batch, lr=lr, seed=1000 + refreshes)
-                refresh_s += time.perf_counter() - t0
-                refreshes += 1
+            t0 = time.perf_counter()
+            if in_flight:
+                if self.surrogate_refresh_poll(wait=done >= nsteps):
+                    in_flight = False
+                    refreshes += 1
+            elif done < nsteps and upd > 0 and bad > max_rate * upd:
+                if overlap:
+                    self.surrogate_refresh_async(iters, batch=batch, lr=lr, seed=1000 + refreshes)
+                    in_flight = True
+                else:
+                    self.surrogate_refresh(iters, batch=batch, lr=lr, seed=1000 + refreshes)
+                    refreshes += 1
+            refresh_s += time.perf_counter() - t0
+        if in_flight:
+            t0 = time.perf_counter()
+            self.surrogate_refresh_poll(wait=True)
+            refresh_s += time.perf_counter() - t0
+            refreshes += 1
         return step_s, refresh_s, refreshes
 
     def surrogate_enable(self, on=True, audit_stride=64, audit_tol=0.5):

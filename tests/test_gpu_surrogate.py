@@ -309,6 +309,36 @@ def test_refresh_captures_fallbacks_and_widens_the_envelope():
     assert lat.surrogate_stats()["fallback"] == k
 
 
+def test_async_refresh_matches_the_synchronous_one():
+    """lbm_surrogate_refresh_async trains a copy of the network on a second
+    stream while the steps go on with the serving network (unchanged until
+    the poll); applied by lbm_surrogate_refresh_poll, the network and the
+    envelope are bitwise those of lbm_surrogate_refresh with the same
+    iterations, batch, rate and seed from the same state: the same kernels on
+    the same rows.  (One lattice, restored between the two: rows appended by
+    two lattices land in orders that differ, and the batch draws by index.)"""
+    fl = box()
+    lat = _capture_step(fl, stride=1)
+    lat.surrogate_train(20, batch=256, lr=1e-3, seed=5)
+    lat.surrogate_enable(True, audit_stride=4)
+    lat.Collide_Species()  # audited states join the capture
+    params, norm = lat.surrogate_params().copy(), lat.surrogate_norm().copy()
+    lat.set_surrogate_params(params)  # Adam moments and step count reset
+    lat.surrogate_refresh(30, batch=256, lr=3e-4, seed=9)
+    sync_params, sync_norm = lat.surrogate_params().copy(), lat.surrogate_norm().copy()
+    assert not np.array_equal(sync_params, params) and not np.array_equal(sync_norm, norm)
+    lat.set_surrogate_params(params)
+    lat.set_surrogate_norm(norm)
+    lat.surrogate_refresh_async(30, batch=256, lr=3e-4, seed=9)
+    lat.Collide_Species()  # steps while the refresh trains (appending rows past those it reads)
+    assert np.array_equal(lat.surrogate_params(), params)
+    assert np.array_equal(lat.surrogate_norm(), norm)
+    assert lat.surrogate_refresh_poll(wait=True)
+    assert not lat.surrogate_refresh_poll(wait=True)
+    np.testing.assert_array_equal(lat.surrogate_params(), sync_params)
+    np.testing.assert_array_equal(lat.surrogate_norm(), sync_norm)
+
+
 def test_surrogate_requires_training_and_reaction():
     from lbm_amd.capi import LbmError
     lat = lattice(box())
