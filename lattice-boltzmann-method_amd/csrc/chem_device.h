@@ -1,10 +1,12 @@
 // Thermo, kinetics, transport and unit conversions of the multicomponent
-// path, as __host__ __device__ functions over the generated H2-air tables.
+// path, as __host__ __device__ functions over the generated mechanism tables
+// (mech.h).
 //
 // These replace the Cantera calls the reference makes per cell (ideal-gas
 // NASA7 thermo, setState_UV / TD / TP, getNetProductionRates with
-// Arrhenius, third-body and Troe falloff reactions and reverse rates from
-// equilibrium constants, mixture-averaged transport) and the binary32
+// Arrhenius, third-body, Troe and Lindemann falloff and pressure-dependent
+// Arrhenius (PLOG) reactions and reverse rates from equilibrium constants,
+// mixture-averaged transport) and the binary32
 // Units_Conv crossings between lattice and SI units (units.hpp:75-131).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -272,6 +274,31 @@ __host__ __device__ inline double conc_power(double c, double o)
     return c > 0.0 ? pow(c, o) : 0.0;
 }
 
+// Pressure-dependent Arrhenius (PLOG) rate of reaction r at ln P [Pa]:
+// ln k interpolated linearly in ln P between the two bracketing levels, the
+// first (last) level's rate below (above) the table (Cantera's PlogRate).
+__host__ __device__ inline double plog_rate(int r, double lnT, double invT, double lnP)
+{
+    const int n = PLOG_N[r];
+    auto lnk = [&](int i) { return PLOG_LNA[r][i] + PLOG_B[r][i] * lnT - PLOG_EA_R[r][i] * invT; };
+    if (lnP <= PLOG_LNP[r][0]) return exp(lnk(0));
+    for (int i = 0; i + 1 < n; ++i) {
+        if (lnP < PLOG_LNP[r][i + 1]) {
+            const double w = (lnP - PLOG_LNP[r][i]) * (1.0 / (PLOG_LNP[r][i + 1] - PLOG_LNP[r][i]));
+            const double l1 = lnk(i);
+            return exp(l1 + (lnk(i + 1) - l1) * w);
+        }
+    }
+    return exp(lnk(n - 1));
+}
+
+constexpr bool any_plog()
+{
+    for (int r = 0; r < NR; ++r)
+        if (PLOG_N[r] > 0) return true;
+    return false;
+}
+
 // Net molar production rates [kmol/m^3/s] at (T, rho, Y), SI units.
 __host__ __device__ inline void net_production(double T, double rho, const double Y[NS], double wdot[NS])
 {
@@ -284,10 +311,18 @@ __host__ __device__ inline void net_production(double T, double rho, const doubl
         wdot[k] = 0.0;
     }
     const double ln_pref_RT = log(P_REF / (R_UNIV * T));
+    // ln p of the PLOG rates, p = R T sum_k C_k
+    double lnP = 0.0;
+    if constexpr (any_plog()) {
+        double Ct = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) Ct += C[k];
+        lnP = log(R_UNIV * T * Ct);
+    }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        double kf = ARR_A[r] * exp(ARR_B[r] * lnT - ARR_EA_R[r] * invT);
-        if (KIND[r] != 0) {
+        double kf = KIND[r] == 3 ? plog_rate(r, lnT, invT, lnP) : ARR_A[r] * exp(ARR_B[r] * lnT - ARR_EA_R[r] * invT);
+        if (KIND[r] == 1 || KIND[r] == 2) {
             double M = 0;
 #pragma unroll
             for (int k = 0; k < NS; ++k) M += EFF[r][k] * C[k];
@@ -296,14 +331,18 @@ __host__ __device__ inline void net_production(double T, double rho, const doubl
             } else {
                 const double k0 = LOW_A[r] * exp(LOW_B[r] * lnT - LOW_EA_R[r] * invT);
                 const double Pr = k0 * M / kf;
-                double Fc = (1.0 - TROE[r][0]) * exp(-T / TROE[r][1]) + TROE[r][0] * exp(-T / TROE[r][2]);
-                if (TROE_HAS_T2[r]) Fc += exp(-TROE[r][3] * invT);
-                const double lFc = log10(Fc);
-                const double lPr = log10(Pr > 1e-300 ? Pr : 1e-300);
-                const double cc = -0.4 - 0.67 * lFc;
-                const double nn = 0.75 - 1.27 * lFc;
-                const double f1 = (lPr + cc) / (nn - 0.14 * (lPr + cc));
-                kf = kf * Pr / (1.0 + Pr) * exp10(lFc / (1.0 + f1 * f1));
+                if (TROE_ON[r]) {
+                    double Fc = (1.0 - TROE[r][0]) * exp(-T / TROE[r][1]) + TROE[r][0] * exp(-T / TROE[r][2]);
+                    if (TROE_HAS_T2[r]) Fc += exp(-TROE[r][3] * invT);
+                    const double lFc = log10(Fc);
+                    const double lPr = log10(Pr > 1e-300 ? Pr : 1e-300);
+                    const double cc = -0.4 - 0.67 * lFc;
+                    const double nn = 0.75 - 1.27 * lFc;
+                    const double f1 = (lPr + cc) / (nn - 0.14 * (lPr + cc));
+                    kf = kf * Pr / (1.0 + Pr) * exp10(lFc / (1.0 + f1 * f1));
+                } else {  // Lindemann: broadening factor 1
+                    kf = kf * Pr / (1.0 + Pr);
+                }
             }
         }
         double dG = 0, dn = 0, cf = 1, cr = 1;

@@ -538,6 +538,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         g_create_error = "unknown precision";
         return LBM_E_ARG;
     }
+    // the FP32 moment pass stages its per-species sums in LDS, 16 species at most
+    if (cfg->n_species > 16 && cfg->precision == LBM_FP32_SHIFTED) {
+        g_create_error = std::string("LBM_FP32_SHIFTED holds at most 16 species; mechanism ") + mech::MECH_NAME +
+                         " has " + std::to_string(mech::NS) + " (use LBM_FP64)";
+        return LBM_E_UNSUPPORTED;
+    }
     if (cfg->mode != LBM_MODE_FUSED && cfg->mode != LBM_MODE_STRICT && cfg->mode != LBM_MODE_INPLACE) {
         g_create_error = "unknown mode";
         return LBM_E_ARG;
@@ -1728,8 +1734,20 @@ namespace {
 
 constexpr uint64_t SUR_DEFAULT_SEED = 20261202ull;
 
+// The network's 16-wide input and output layers hold the NS + 3 features and
+// NS + 1 targets of mechanisms of up to 11 species; a build for a larger
+// mechanism refuses the surrogate's entry points.
+int sur_supported(lbm_ctx* c)
+{
+    if (sur::ENABLED) return LBM_OK;
+    return fail(c, LBM_E_UNSUPPORTED,
+                std::string("the chemistry surrogate's 16-wide layers hold at most 11 species; mechanism ") +
+                    mech::MECH_NAME + " has " + std::to_string(mech::NS));
+}
+
 int sur_common(lbm_ctx* c)
 {
+    if (int rc = sur_supported(c)) return rc;
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     if (!c->sur_stats) {
         HIP_TRY(c, hipMalloc((void**)&c->sur_stats, sizeof(float) * sur::SS_SIZE));
@@ -1802,6 +1820,7 @@ int64_t lbm_surrogate_param_count(void) { return sur::NPARAM; }
 int lbm_surrogate_init(lbm_ctx* c, uint64_t seed)
 {
     if (!c) return LBM_E_ARG;
+    if (int rc = sur_supported(c)) return rc;
     if (int rc = refresh_finish(c, true, nullptr)) return rc;  // no refresh in flight
     HIP_TRY(c, hipSetDevice(c->cfg.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1818,6 +1837,7 @@ int lbm_surrogate_init(lbm_ctx* c, uint64_t seed)
 int lbm_surrogate_capture(lbm_ctx* c, int64_t max_rows, int stride)
 {
     if (!c) return LBM_E_ARG;
+    if (int rc = sur_supported(c)) return rc;
     if (int rc = refresh_finish(c, true, nullptr)) return rc;  // no refresh in flight
     if (int rc = need_reacting(c)) return rc;
     if (max_rows < 0 || max_rows > ((int64_t)1 << 31) || stride < 0)
@@ -2226,6 +2246,7 @@ extern "C" {
 int lbm_surrogate_forward_noalloc(lbm_ctx* c, const float* X, int64_t m, float* Y)
 {
     if (!c || !X || !Y) return fail(c, LBM_E_ARG, "null argument");
+    if (int rc = sur_supported(c)) return rc;
     if (m < 1 || m > SUR_NOALLOC_ROWS)
         return fail(c, LBM_E_ARG, "m out of range [1, " + std::to_string(SUR_NOALLOC_ROWS) +
                                       "] (the allocation-free pass stages its rows in a fixed device array)");

@@ -443,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_sur_decode(MultiView<P> v, const f
         chem_state(v, c, false, s);
         const float4* pr4 = reinterpret_cast<const float4*>(Y + (int64_t)c * 16);
         const float4 y0 = pr4[0], y1 = pr4[1], y2 = pr4[2];
-        static_assert(NOUT <= 12, "three float4 loads cover the real outputs");
+        static_assert(!sur::ENABLED || NOUT <= 12, "three float4 loads cover the real outputs");
         const float pr[12] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w, y2.x, y2.y, y2.z, y2.w};
         // the imbalance of the predicted sources is removed in proportion to each
         // species' target scale (an inert species, scale 0, keeps its mean)
@@ -1454,7 +1454,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(P)
     // species moments.  FP32: the per-species results go through LDS, one
     // column per thread (the rolled loop would put register arrays it indexes
     // by species into scratch memory)
-    constexpr bool LDS_STAGE = sizeof(P) == 4;
+    constexpr bool LDS_STAGE = sizeof(P) == 4 && NS <= 16;
     __shared__ double s_rr[LDS_STAGE ? NS : 1][BLOCK];
     __shared__ float s_B[LDS_STAGE ? 3 * NS : 1][BLOCK];
     double Pt[6] = {0, 0, 0, 0, 0, 0};
@@ -2449,71 +2449,97 @@ template void launch_mc_state_sums<double>(const uint8_t*, const double*, const 
 // ------------------------------------------------------------- launchers
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
+// FP32 contexts hold at most 16 species (the moment pass stages per-species
+// sums in LDS) and lbm_create refuses more: a build for a larger mechanism
+// compiles the FP64 kernels only, and the surrogate's kernels only for
+// mechanisms its 16-wide layers hold.
+template <typename P>
+constexpr bool kernels_built()
+{
+    return sizeof(P) == 8 || NS <= 16;
+}
+
 template <typename P>
 void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
 {
-    switch (k) {
-    case McKernel::Init: hipLaunchKernelGGL(k_mc_init<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
-    case McKernel::Chemistry: hipLaunchKernelGGL(k_mc_chemistry<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
-    case McKernel::SpeciesTransport:
-        hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
-        break;
-    case McKernel::CollideSpecies:
-        hipLaunchKernelGGL((k_mc_collide_species<P, false>), dim3(xcd_species_grid(v.d.n, BLOCK, NS)), dim3(BLOCK), 0,
-                           s, v);
-        break;
-    case McKernel::CollideSpeciesPush:
-        hipLaunchKernelGGL((k_mc_collide_species<P, true>), dim3(xcd_species_grid(v.d.n, BLOCK, NS)), dim3(BLOCK), 0,
-                           s, v);
-        if (v.nside > 0)
-            hipLaunchKernelGGL(k_mc_side_push<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        switch (k) {
+        case McKernel::Init: hipLaunchKernelGGL(k_mc_init<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+        case McKernel::Chemistry: hipLaunchKernelGGL(k_mc_chemistry<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v); break;
+        case McKernel::SpeciesTransport:
+            hipLaunchKernelGGL(k_mc_species_transport<P>, dim3(xcd_grid(v.d.n, BLOCK)), dim3(BLOCK), 0, s, v);
+            break;
+        case McKernel::CollideSpecies:
+            hipLaunchKernelGGL((k_mc_collide_species<P, false>), dim3(xcd_species_grid(v.d.n, BLOCK, NS)), dim3(BLOCK), 0,
                                s, v);
-        break;
-    case McKernel::Collide:
-        if (v.use_omix) {
-            hipLaunchKernelGGL((k_mc_collide<P, 1>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
-            hipLaunchKernelGGL((k_mc_collide<P, 2>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
-        } else {
-            hipLaunchKernelGGL((k_mc_collide<P, 0>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+            break;
+        case McKernel::CollideSpeciesPush:
+            hipLaunchKernelGGL((k_mc_collide_species<P, true>), dim3(xcd_species_grid(v.d.n, BLOCK, NS)), dim3(BLOCK), 0,
+                               s, v);
+            if (v.nside > 0)
+                hipLaunchKernelGGL(k_mc_side_push<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
+                                   s, v);
+            break;
+        case McKernel::Collide:
+            if (v.use_omix) {
+                hipLaunchKernelGGL((k_mc_collide<P, 1>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+                hipLaunchKernelGGL((k_mc_collide<P, 2>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+            } else {
+                hipLaunchKernelGGL((k_mc_collide<P, 0>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+            }
+            break;
+        case McKernel::StreamEnergy:
+            hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), 1), dim3(BLOCK), 0, s, v, NS);
+            break;
+        case McKernel::SideFill:
+            if (v.nside > 0)
+                hipLaunchKernelGGL(k_mc_side_fill<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
+                                   s, v);
+            break;
+        case McKernel::Unstream:
+            hipLaunchKernelGGL(k_mc_unstream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
+            break;
+        case McKernel::Stream:
+            hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS + 1), dim3(BLOCK), 0, s, v, 0);
+            break;
+        case McKernel::Moments:
+            if (v.swapped) hipLaunchKernelGGL((k_mc_moments<P, 1>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+            else hipLaunchKernelGGL((k_mc_moments<P, 0>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
+            break;
         }
-        break;
-    case McKernel::StreamEnergy:
-        hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), 1), dim3(BLOCK), 0, s, v, NS);
-        break;
-    case McKernel::SideFill:
-        if (v.nside > 0)
-            hipLaunchKernelGGL(k_mc_side_fill<P>, dim3((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS), dim3(BLOCK), 0,
-                               s, v);
-        break;
-    case McKernel::Unstream:
-        hipLaunchKernelGGL(k_mc_unstream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS), dim3(BLOCK), 0, s, v);
-        break;
-    case McKernel::Stream:
-        hipLaunchKernelGGL(k_mc_stream<P>, dim3(xcd_grid(v.d.n, BLOCK), NS + 1), dim3(BLOCK), 0, s, v, 0);
-        break;
-    case McKernel::Moments:
-        if (v.swapped) hipLaunchKernelGGL((k_mc_moments<P, 1>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
-        else hipLaunchKernelGGL((k_mc_moments<P, 0>), grid1(v.d.n), dim3(BLOCK), 0, s, v);
-        break;
     }
 }
 
 template <typename P>
 void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s)
 {
-    if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+    }
 }
 
 template <typename P>
 void launch_mc_tms_in(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s)
 {
-    if (count > 0) hipLaunchKernelGGL(k_mc_tms_in<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        if (count > 0) hipLaunchKernelGGL(k_mc_tms_in<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+    }
 }
 
 template <typename P>
 void launch_mc_tms_wall(const MultiView<P>& v, const int* cells, int count, hipStream_t s)
 {
-    if (count > 0) hipLaunchKernelGGL(k_mc_tms_wall<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, count);
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        if (count > 0) hipLaunchKernelGGL(k_mc_tms_wall<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, count);
+    }
 }
 
 static int cu_count()
@@ -2528,39 +2554,56 @@ __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 template <typename P>
 void launch_mc_aa_save(const MultiView<P>& v, bool first, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, v.low_count);
-    const dim3 g(xcd_grid(v.d.n, BLOCK), NS);
-    if (v.swapped) hipLaunchKernelGGL((k_mc_aa_save<P, 1>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
-    else hipLaunchKernelGGL((k_mc_aa_save<P, 0>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, v.low_count);
+        const dim3 g(xcd_grid(v.d.n, BLOCK), NS);
+        if (v.swapped) hipLaunchKernelGGL((k_mc_aa_save<P, 1>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
+        else hipLaunchKernelGGL((k_mc_aa_save<P, 0>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
+    }
 }
 
 template <typename P>
 void launch_mc_diag_mach(const MultiView<P>& v, double* partials, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_diag_mach<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, partials);
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        hipLaunchKernelGGL(k_mc_diag_mach<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, partials);
+    }
 }
 
 template <typename P>
 void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStream_t s)
 {
-    for (int a = 0; a < NS; ++a) {
-        hipLaunchKernelGGL(k_mc_smooth<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, a, static_cast<P*>(tmp), alpha);
-        hipLaunchKernelGGL(k_mc_smooth_store<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, a, static_cast<const P*>(tmp));
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        for (int a = 0; a < NS; ++a) {
+            hipLaunchKernelGGL(k_mc_smooth<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, a, static_cast<P*>(tmp), alpha);
+            hipLaunchKernelGGL(k_mc_smooth_store<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, a, static_cast<const P*>(tmp));
+        }
     }
 }
 
 template <typename P>
 void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s)
 {
-    const dim3 g(xcd_species_grid(v.d.n, BLOCK, NS)), gl(4 * cu_count()), gs((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS);
-    if (v.swapped) {
-        hipLaunchKernelGGL((k_mc_collide_species_aa<P, 1>), g, dim3(BLOCK), 0, s, v);
-        hipLaunchKernelGGL((k_mc_collide_species_low<P, 1>), gl, dim3(BLOCK), 0, s, v);
-        if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 1>), gs, dim3(BLOCK), 0, s, v);
+    if constexpr (!kernels_built<P>()) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
     } else {
-        hipLaunchKernelGGL((k_mc_collide_species_aa<P, 0>), g, dim3(BLOCK), 0, s, v);
-        hipLaunchKernelGGL((k_mc_collide_species_low<P, 0>), gl, dim3(BLOCK), 0, s, v);
-        if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 0>), gs, dim3(BLOCK), 0, s, v);
+        const dim3 g(xcd_species_grid(v.d.n, BLOCK, NS)), gl(4 * cu_count()),
+            gs((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS);
+        if (v.swapped) {
+            hipLaunchKernelGGL((k_mc_collide_species_aa<P, 1>), g, dim3(BLOCK), 0, s, v);
+            hipLaunchKernelGGL((k_mc_collide_species_low<P, 1>), gl, dim3(BLOCK), 0, s, v);
+            if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 1>), gs, dim3(BLOCK), 0, s, v);
+        } else {
+            hipLaunchKernelGGL((k_mc_collide_species_aa<P, 0>), g, dim3(BLOCK), 0, s, v);
+            hipLaunchKernelGGL((k_mc_collide_species_low<P, 0>), gl, dim3(BLOCK), 0, s, v);
+            if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 0>), gs, dim3(BLOCK), 0, s, v);
+        }
     }
 }
 
@@ -2569,16 +2612,24 @@ template <typename P>
 void launch_mc_sur_chemistry(const MultiView<P>& v, const sur::Mlp& net, uint16_t* X, float* Y, uint8_t* flag,
                              int64_t rows, unsigned* list, unsigned* count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, count);
-    hipLaunchKernelGGL(k_mc_sur_features<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, X, flag);
-    sur::mlp_infer_bf16(net, X, rows, Y, s);
-    hipLaunchKernelGGL(k_mc_sur_decode<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, Y, flag, list, count);
+    if constexpr (!kernels_built<P>() || !sur::ENABLED) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, count);
+        hipLaunchKernelGGL(k_mc_sur_features<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, X, flag);
+        sur::mlp_infer_bf16(net, X, rows, Y, s);
+        hipLaunchKernelGGL(k_mc_sur_decode<P>, grid1(v.d.n), dim3(BLOCK), 0, s, v, Y, flag, list, count);
+    }
 }
 
 template <typename P>
 void launch_mc_chem_list(const MultiView<P>& v, const unsigned* list, const unsigned* count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_chem_list<P>, dim3(4 * cu_count()), dim3(BLOCK), 0, s, v, list, count);
+    if constexpr (!kernels_built<P>() || !sur::ENABLED) {
+        return;  // refused before launch (lbm_create, surrogate entry points)
+    } else {
+        hipLaunchKernelGGL(k_mc_chem_list<P>, dim3(4 * cu_count()), dim3(BLOCK), 0, s, v, list, count);
+    }
 }
 
 template void launch_mc<double>(McKernel, const MultiView<double>&, hipStream_t);

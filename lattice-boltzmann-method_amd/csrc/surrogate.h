@@ -12,7 +12,10 @@ constexpr int NL = 4;                              // weight layers
 constexpr int DIMS[NL + 1] = {16, 128, 128, 128, 16};
 constexpr int NIN = mech::NS + 3;                  // T, p, Y_k, dt
 constexpr int NOUT = mech::NS + 1;                 // dT, dY_k
-static_assert(NIN <= 16 && NOUT <= 12, "features and targets must fit the 16-wide layers");
+// The features and targets fit the 16-wide layers (mechanisms of up to 11
+// species); builds for larger mechanisms compile the surrogate but refuse its
+// entry points.
+constexpr bool ENABLED = NIN <= 16 && NOUT <= 12;
 
 // FP32 master weights W[L] [DIMS[L+1]][DIMS[L]], biases, Adam moments and the
 // BF16 copies (row-major and transposed) the MFMA kernels read.

@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_NAMES = {"h2air": "liblbm_amd.so", "ch4_2s": "liblbm_amd_ch4_2s.so", "c3h8": "liblbm_amd_c3h8.so"}
+LIB_NAMES = {"h2air": "liblbm_amd.so", "ch4_2s": "liblbm_amd_ch4_2s.so", "c3h8": "liblbm_amd_c3h8.so",
+             "nh3": "liblbm_amd_nh3.so"}
 MECHANISMS = tuple(LIB_NAMES)
 LIB_PATH = os.path.join(_PKG_DIR, LIB_NAMES["h2air"])
 
@@ -447,7 +448,7 @@ class Lattice:
 class MixtureLattice(Lattice):
     """Multicomponent lattice, mirroring LBM(Nx, Ny, Nz, species) of
     lbm-main.cpp:38-118 with units.set_m_kg_s(dx, dt), on the library of
-    `mechanism` ("h2air": 9 species, "ch4_2s" and "c3h8": 6).  `SPECIES` lists
+    `mechanism` ("h2air": 9 species, "ch4_2s" and "c3h8": 6, "nh3": 31).  `SPECIES` lists
     the species in the order of every per-species array."""
 
     def __init__(self, nx, ny, nz, dx_si, dt_si, reaction=True, mode=LBM_MODE_STRICT, precision=LBM_FP64, device=0,

@@ -170,7 +170,7 @@ def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.
     with np.errstate(divide="ignore", invalid="ignore"):
         dot = (KX * comps[0] + KY * comps[1] + KZ * comps[2]) / np.where(K2 > 0, K2, 1.0)
     comps = [comps[0] - KX * dot, comps[1] - KY * dot, comps[2] - KZ * dot]
-    vel = [np.fft.irfftn(c, s=(N, N, N)) for c in comps]
+    vel = [np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2)) for c in comps]
     c0 = math.sqrt(gamma * R * T0)
     urms = math.sqrt(float(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)))
     scale = ma_rms * c0 / urms
@@ -192,13 +192,18 @@ def decaying_turbulence(N, seed=20261205, k0=4.0, ma_rms=0.3, t_fluct=0.05, R=1.
 H2AIR_SPECIES = ["H2", "H", "O", "O2", "OH", "H2O", "HO2", "H2O2", "N2"]
 # species order of the two-step mechanisms (the libraries' lbm_species_name)
 MECH_SPECIES = {"h2air": H2AIR_SPECIES, "ch4_2s": ["O2", "H2O", "CH4", "CO", "CO2", "N2"],
-                "c3h8": ["CO", "C3H8", "CO2", "H2O", "N2", "O2"]}
+                "c3h8": ["CO", "C3H8", "CO2", "H2O", "N2", "O2"],
+                "nh3": ["AR", "N2", "HE", "H2", "H", "O2", "O", "H2O", "OH", "H2O2", "HO2", "NO", "N2O", "NO2", "HNO",
+                        "HNO2", "HONO", "HONO2", "N2H2", "H2NN", "NH2OH", "HNOH", "NH3", "N2H4", "N", "NO3", "NH",
+                        "NNH", "NH2", "H2NO", "N2H3"]}
 # phi = 1 H2-air, unburnt mole fractions (setup.cpp:2108-2116)
 X_UNBURNT_PHI1 = {"H2": 0.29586, "O2": 0.14793, "N2": 0.55621}
 X_AIR = {"O2": 0.21, "N2": 0.79}
 # stoichiometric fuel-air mixtures of the two-step mechanisms (air = O2 + 3.76 N2)
 X_CH4_AIR_PHI1 = {"CH4": 1.0 / 10.52, "O2": 2.0 / 10.52, "N2": 7.52 / 10.52}
 X_C3H8_AIR_PHI1 = {"C3H8": 1.0 / 24.8, "O2": 5.0 / 24.8, "N2": 18.8 / 24.8}
+# phi = 1 ammonia/hydrogen (60/40 by mole) in air: 0.6 NH3 + 0.4 H2 + 0.65 (O2 + 3.76 N2)
+X_NH3_H2_AIR_PHI1 = {"NH3": 0.6 / 4.094, "H2": 0.4 / 4.094, "O2": 0.65 / 4.094, "N2": 2.444 / 4.094}
 UNITS_FLAME = (5e-6, 1e-9)  # units.set_m_kg_s(dx, dt) of the flame cases (setup.cpp:2019)
 LATTICE_K = 12000.0          # temperature unit of set_m_kg_s(dx, dt) (units.hpp:36)
 ONE_ATM = 101325.0

@@ -10,14 +10,16 @@ call sites in lbm-collision.cpp:201-247,342-445, lbm-moment.cpp:201-368,
 lbm-init.cpp:99-134).  Cantera itself (unpinned version >= 3.0, not vendored)
 is absent here, so this is a restatement of the published models:
   * NASA7 ideal-gas thermo, standard pressure 1 atm;
-  * mass-action kinetics with Arrhenius, third-body efficiencies, Troe
-    falloff, reverse rates from equilibrium constants (Gibbs energies) for
+  * mass-action kinetics with Arrhenius, third-body efficiencies, Troe and
+    Lindemann falloff, pressure-dependent Arrhenius (PLOG: ln k linear in
+    ln p between the tabulated levels, the end levels' rates outside them,
+    as Cantera's PlogRate), reverse rates from equilibrium constants (Gibbs energies) for
     reversible reactions, explicit fractional forward orders (a
     concentration factor C^o of 0 for C <= 0 when o is not an integer,
     Cantera's rule for such exponents);
   * mixture-averaged transport: Wilke viscosity, the combination-averaged
     conductivity 1/2 (sum X l + 1/sum(X/l)), binary diffusion from the
-    ln-polynomial fits of mech/<mechanism>.json (h2air, ch4_2s, c3h8).
+    ln-polynomial fits of mech/<mechanism>.json (h2air, ch4_2s, c3h8, nh3).
 Vectorised over cells: every state argument is an array (..., NS) or (...).
 Pinned by tests/test_oracle_chem.py (closed forms, conservation, the
 reference-held adiabatic flame state, equilibrium, brute force).
@@ -33,7 +35,7 @@ _MECH = os.path.join(_MECH_DIR, "h2air.json")
 
 
 def mechanism_path(name):
-    """JSON tables of a mechanism by name (h2air, ch4_2s, c3h8)."""
+    """JSON tables of a mechanism by name (h2air, ch4_2s, c3h8, nh3)."""
     return os.path.join(_MECH_DIR, name + ".json")
 
 
@@ -62,6 +64,7 @@ class Mechanism:
         self.arr = np.zeros((self.nr, 3))
         self.low = np.zeros((self.nr, 3))
         self.troe = []
+        self.plog = {}  # reaction -> (ln p [Pa], ln A, b, Ea/R) arrays of its levels
         self.eff = np.ones((self.nr, self.ns))
         for r, e in enumerate(rx):
             for k, v in e["reactants"].items():
@@ -77,6 +80,10 @@ class Mechanism:
             if "k0" in e:
                 self.low[r] = (e["k0"]["A"], e["k0"]["b"], e["k0"]["Ea_R"])
             self.troe.append(e.get("troe"))
+            if "plog" in e:
+                lv = e["plog"]
+                self.plog[r] = tuple(np.array([x[k] for x in lv], dtype=np.float64) for k in ("P", "A", "b", "Ea_R"))
+                self.plog[r] = (np.log(self.plog[r][0]), np.log(self.plog[r][1])) + self.plog[r][2:]
             for k2, v in e.get("efficiencies", {}).items():
                 self.eff[r, idx[k2]] = v
         tr = d["transport"]
@@ -201,13 +208,20 @@ class Mechanism:
         RT = self.R * T
         kf = self.arr[:, 0] * np.exp(self.arr[:, 1] * lnT[..., None] - self.arr[:, 2] / T[..., None])
         M = C @ self.eff.T  # (..., NR)
+        if self.plog:
+            lnP = np.log(RT * C.sum(axis=-1))
         for r, kind in enumerate(self.kind):
             if kind == "three-body":
                 kf[..., r] *= M[..., r]
+            elif kind == "pressure-dependent-Arrhenius":
+                kf[..., r] = self.plog_rate(r, lnT, T, lnP)
             elif kind == "falloff":
                 k0 = self.low[r, 0] * np.exp(self.low[r, 1] * lnT - self.low[r, 2] / T)
                 Pr = k0 * M[..., r] / kf[..., r]
                 t = self.troe[r]
+                if t is None:  # Lindemann
+                    kf[..., r] = kf[..., r] * Pr / (1.0 + Pr)
+                    continue
                 Fc = (1 - t["A"]) * np.exp(-T / t["T3"]) + t["A"] * np.exp(-T / t["T1"])
                 if t.get("T2") is not None:
                     Fc = Fc + np.exp(-t["T2"] / T)
@@ -226,6 +240,24 @@ class Mechanism:
         Cf = np.prod(_conc_power(C[..., None, :], self.ordf), axis=-1)
         Cr = np.prod(_conc_power(C[..., None, :], self.nur), axis=-1)
         return kf * Cf, kr * Cr
+
+
+    def plog_rate(self, r, lnT, T, lnP):
+        """PLOG rate of reaction r: ln k of each level is ln A + b ln T - Ea/(R T);
+        between levels i and i+1 ln k is linear in ln p, below the first level
+        (above the last) the first (last) level's rate applies."""
+        lp, lA, b, Ea = self.plog[r]
+        lnk = lA + b * np.asarray(lnT)[..., None] - Ea / np.asarray(T)[..., None]  # (..., levels)
+        lnP = np.asarray(lnP)
+        if len(lp) == 1:
+            return np.exp(lnk[..., 0]) + 0.0 * lnP
+        i = np.clip(np.searchsorted(lp, lnP, side="right") - 1, 0, len(lp) - 2)
+        w = (lnP - lp[i]) * (1.0 / (lp[i + 1] - lp[i]))
+        l1 = np.take_along_axis(lnk, i[..., None], -1)[..., 0]
+        l2 = np.take_along_axis(lnk, (i + 1)[..., None], -1)[..., 0]
+        out = np.exp(l1 + (l2 - l1) * w)
+        out = np.where(lnP <= lp[0], np.exp(lnk[..., 0]), out)
+        return np.where(lnP >= lp[-1], np.exp(lnk[..., -1]), out)
 
 
 def _conc_power(C, o):

@@ -55,6 +55,17 @@ def test_species_count_must_match_the_library():
     assert b"ch4_2s" in lib.lbm_create_error()
 
 
+def test_fp32_is_refused_for_the_31_species_mechanism():
+    """The FP32 moment pass stages per-species sums in LDS for at most 16
+    species: the ammonia build refuses LBM_FP32_SHIFTED before touching a
+    device."""
+    lib = capi.load_library(mechanism="nh3")
+    cfg = capi.Config(4, 4, 4, 31, capi.LBM_FP32_SHIFTED, 1, 0, 0.0, 0.0, 1.4, 0.0, 1.0, 1.0, 5e-6, 1e-9, 1)
+    h = ctypes.c_void_p()
+    assert lib.lbm_create(ctypes.byref(cfg), ctypes.byref(h)) == -4  # LBM_E_UNSUPPORTED
+    assert b"16 species" in lib.lbm_create_error()
+
+
 def test_config_struct_matches_header():
     txt = open(os.path.join(REPO, "include", "lbm_amd.h")).read()
     body = re.search(r"typedef struct \{(.*?)\} lbm_config;", txt, re.S).group(1)
