@@ -559,8 +559,10 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return LBM_E_ARG;
     }
     const int64_t n = (int64_t)(cfg->nx + 2) * (cfg->ny + 2) * (cfg->nz + 2);
-    if (n * Q >= ((int64_t)1 << 40) || n >= ((int64_t)1 << 31)) {
-        g_create_error = "grid too large";
+    // at most 2^29 cells: a plane of doubles stays within 32-bit byte offsets
+    // (kernels address planes as a uniform base plus a 32-bit offset)
+    if (n >= ((int64_t)1 << 29)) {
+        g_create_error = "grid too large (at most 2^29 cells with ghosts)";
         return LBM_E_ARG;
     }
     int ndev = 0;

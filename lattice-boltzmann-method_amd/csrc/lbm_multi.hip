@@ -38,6 +38,21 @@ __device__ inline Ijk ijk(const Dims& d, int c)
 }
 __device__ inline int cell(const Dims& d, int i, int j, int k) { return (i * d.ny + j) * d.nz + k; }
 
+// Element i of a plane as a wave-uniform base plus a 32-bit byte offset: the
+// access compiles to one global load or store with a scalar base and a
+// 32-bit VGPR offset, not per-lane 64-bit address arithmetic (grids hold
+// fewer than 2^29 cells, so a plane of doubles stays below 2^32 bytes).
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, int i)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& at32(const T* base, int i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+
 __device__ inline bool fluidlike(uint8_t t)
 {
     return t == TYPE_F || t == TYPE_I || t == TYPE_O || t == TYPE_I_C || t == TYPE_O_C;
@@ -198,10 +213,15 @@ template <typename P>
 __device__ __forceinline__ P species_post(const MultiView<P>& v, int a, int l, P fl, double om, double fe, double fdu,
                                           double fr)
 {
-    const double f = (double)fl + fshift(v, a, l);
-    double x = f + om * (fe - f) + (fdu - fe) + fr;
-    if constexpr (sizeof(P) == 4) x -= fshift(v, a, l);
-    return (P)x;
+    if constexpr (sizeof(P) == 4) {
+        // the same update on the stored deviation f - shift:
+        // (1 - om) (fl - fe) + fdu + fr - om shift, five operations for eight
+        const double x = (1.0 - om) * ((double)fl - fe) + (fdu + fr - om * fshift(v, a, l));
+        return (P)x;
+    } else {
+        const double f = (double)fl;
+        return f + om * (fe - f) + (fdu - fe) + fr;
+    }
 }
 
 }  // namespace
@@ -739,12 +759,14 @@ __device__ __forceinline__ void species_collision(const MultiView<P>& v, int c, 
 {
     const int64_t n = v.d.n;
     double* const* Fd = v.fld;
-    const double T = Fd[LBM_FIELD_T][c];
-    const double umix[3] = {Fd[LBM_FIELD_U][c], Fd[LBM_FIELD_V][c], Fd[LBM_FIELD_W][c]};
-    const double ra = v.rhoa[a * n + c];
-    const double src = v.reaction ? v.src[a * n + c] : 0.0;
-    const double om = v.tsp[a * n + c];
-    const double Fa[3] = {v.tsp[(NS + a) * n + c], v.tsp[(2 * NS + a) * n + c], v.tsp[(3 * NS + a) * n + c]};
+    const auto* rhoa = v.rhoa + a * n;
+    const double T = at32(Fd[LBM_FIELD_T], c);
+    const double umix[3] = {at32(Fd[LBM_FIELD_U], c), at32(Fd[LBM_FIELD_V], c), at32(Fd[LBM_FIELD_W], c)};
+    const double ra = at32(rhoa, c);
+    const double src = v.reaction ? (double)at32(v.src + a * n, c) : 0.0;
+    const double om = at32(v.tsp + a * n, c);
+    const double Fa[3] = {at32(v.tsp + (NS + a) * n, c), at32(v.tsp + (2 * NS + a) * n, c),
+                          at32(v.tsp + (3 * NS + a) * n, c)};
     const int nm[3] = {cell(v.d, x.i - 1, x.j, x.k), cell(v.d, x.i, x.j - 1, x.k), c - 1};
     const int np[3] = {cell(v.d, x.i + 1, x.j, x.k), cell(v.d, x.i, x.j + 1, x.k), c + 1};
     const double* Tf = Fd[LBM_FIELD_T];
@@ -756,33 +778,42 @@ __device__ __forceinline__ void species_collision(const MultiView<P>& v, int c, 
         if (cy(l)) mom[1] += cy(l) * f;
         if (cz(l)) mom[2] += cz(l) * f;
     }
-    const double uraw[3] = {mom[0] / ra, mom[1] / ra, mom[2] / ra};
+    // FP32 path: one reciprocal for the three divisions
+    constexpr bool FAST = sizeof(P) == 4;
+    const double ira = FAST ? 1.0 / ra : 0.0;
+    const double uraw[3] = {FAST ? mom[0] * ira : mom[0] / ra, FAST ? mom[1] * ira : mom[1] / ra,
+                            FAST ? mom[2] * ira : mom[2] / ra};
     const double udu[3] = {uraw[0] + Fa[0], uraw[1] + Fa[1], uraw[2] + Fa[2]};
     const double Ra = v.u.cp(R_UNIV / W[a]);
     const double theta = Ra * T;
     double dQ[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+        const auto* uad = v.ua + (d * NS + a) * n;
         auto Qf = [&](int m) {
-            const double r = v.rhoa[a * n + m], ud = v.ua[(d * NS + a) * n + m];
-            return r * ud * (1.0 - 3.0 * Ra * Tf[m]) - r * (ud * ud * ud);
+            const double r = at32(rhoa, m), ud = at32(uad, m);
+            return r * ud * (1.0 - 3.0 * Ra * at32(Tf, m)) - r * (ud * ud * ud);
         };
-        dQ[d] = fd_central<double>(Qf(nm[d]), Qf(c), Qf(np[d]), v.type[nm[d]], v.type[np[d]]);
+        dQ[d] = fd_central<double>(Qf(nm[d]), Qf(c), Qf(np[d]), at32(v.type, nm[d]), at32(v.type, np[d]));
     }
     double corr[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) corr[d] = 1 * (2.0 - om) / (2.0 * ra * om) * dQ[d];
-    const double zero[3] = {0, 0, 0};
-    Axis3<double> Fe[3], Fu[3], Fr[3];
+    Axis3<double> Fe[3], Fu[3];
     feq_axes<double>(Fe, uraw, theta, corr);
     feq_axes<double>(Fu, udu, theta, corr);
-    feq_axes<double>(Fr, umix, theta, zero);
+    // the chemistry source's equilibrium: binary32 on the FP32 path (a
+    // correction far below the stored deviations' resolution), binary64 else
+    using RS = typename std::conditional<sizeof(P) == 4, float, double>::type;
+    const RS zero[3] = {0, 0, 0}, um[3] = {(RS)umix[0], (RS)umix[1], (RS)umix[2]};
+    Axis3<RS> Fr[3];
+    feq_axes<RS>(Fr, um, (RS)theta, zero);
     const bool low = ra <= SPECIES_MIN;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         const double fe = feq_dir<double>(Fe, l, ra);
         const double fdu = feq_dir<double>(Fu, l, ra);
-        const double fr = feq_dir<double>(Fr, l, src);
+        const double fr = (double)feq_dir<RS>(Fr, l, (RS)src);
         if (low && fr <= 0.0) keep(l);  // post-collision value left as it was
         else emit(l, species_post(v, a, l, fl[l], om, fe, fdu, fr));
     }
@@ -801,31 +832,32 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide_species(MultiView<P> v)
     if (v.type[c] != TYPE_F) return;
     const int64_t n = v.d.n;
     const Ijk x = ijk(v.d, c);
+    const P* A = v.f + (int64_t)a * Q * n;
+    P* B = v.fo + (int64_t)a * Q * n;
     P fl[Q];
 #pragma unroll
-    for (int l = 0; l < Q; ++l) fl[l] = spop(v.f, a, l, n, c);
+    for (int l = 0; l < Q; ++l) fl[l] = at32(A + l * n, c);
     auto none = [](int) {};
     if constexpr (!PUSH) {
-        species_collision(v, c, a, x, fl, [&](int l, P pv) { spop(v.fo, a, l, n, c) = pv; }, none);
+        species_collision(v, c, a, x, fl, [&](int l, P pv) { at32(B + l * n, c) = pv; }, none);
     } else {
         // A cell with NB_PUSH streams every population to a fluid cell that
         // TMS_BC leaves alone: the post-collision value a low species keeps
         // is the one streamed there last step.  The other cells collide into
         // the side store as the strict form does into fo; k_mc_side_push
         // then streams them.
-        if (!(v.nbflag[c] & NB_PUSH)) {
-            P* side = v.fside + (int64_t)a * Q * v.nside + v.side_slot[c];
-            species_collision(v, c, a, x, fl, [&](int l, P pv) { side[(int64_t)l * v.nside] = pv; }, none);
+        if (!(at32(v.nbflag, c) & NB_PUSH)) {
+            P* side = v.fside + (int64_t)a * Q * v.nside;
+            const int slot = at32(v.side_slot, c);
+            species_collision(v, c, a, x, fl, [&](int l, P pv) { at32(side + l * v.nside, slot) = pv; }, none);
             return;
         }
-        const P* A = v.f + (int64_t)a * Q * n;
-        P* B = v.fo + (int64_t)a * Q * n;
         const Wrap3 w(v.d, x);
         species_collision(
-            v, c, a, x, fl, [&](int l, P pv) { B[(int64_t)l * n + w.at(v.d, l, 1)] = pv; },
+            v, c, a, x, fl, [&](int l, P pv) { at32(B + l * n, w.at(v.d, l, 1)) = pv; },
             [&](int l) {
-                const int64_t dst = (int64_t)l * n + w.at(v.d, l, 1);
-                B[dst] = A[dst];
+                const int dst = w.at(v.d, l, 1);
+                at32(B + l * n, dst) = at32(A + l * n, dst);
             });
     }
 }
