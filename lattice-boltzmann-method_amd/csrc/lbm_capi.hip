@@ -72,6 +72,7 @@ struct lbm_ctx {
     int* tms_cell = nullptr;
     int* tms_dir = nullptr;
     int tms_count = 0;
+    double* tms_rec = nullptr;  // per-cell values between the outflow sweep's launches
     int* tin_cell = nullptr;  // TMS_BC inflow interface cells (TYPE_I, TYPE_I_C)
     int* tin_dir = nullptr;
     int tin_count = 0;
@@ -470,7 +471,7 @@ void release(lbm_ctx* c)
                     c->cap_count, c->bx,        c->bt,         c->bloss,     c->mref_dev,  c->sums_part, c->adam_c12, c->train_it,
                     c->sur_x,     c->sur_y,     c->sur_flag,  c->tin_cell,  c->tin_dir,   c->sbc_cell,  c->tsp,        c->omix,       c->twall_cell, c->cjc_cell,
                     c->side_slot, c->side_cell, c->fside, c->aa_pending, c->low_cell, c->low_sp,
-                    c->low_keep,  c->low_count, c->smooth_tmp};
+                    c->low_keep,  c->low_count, c->smooth_tmp, c->tms_rec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->low_count_host) (void)hipHostFree(c->low_count_host);
@@ -853,11 +854,14 @@ int lbm_set_types(lbm_ctx* c, const uint8_t* types)
     if (multi(c)) {
         if (c->tms_cell) (void)hipFree(c->tms_cell);
         if (c->tms_dir) (void)hipFree(c->tms_dir);
+        if (c->tms_rec) (void)hipFree(c->tms_rec);
         c->tms_cell = c->tms_dir = nullptr;
+        c->tms_rec = nullptr;
         c->tms_count = (int)tcell.size();
         if (c->tms_count) {
             HIP_TRY(c, hipMalloc((void**)&c->tms_cell, sizeof(int) * tcell.size()));
             HIP_TRY(c, hipMalloc((void**)&c->tms_dir, sizeof(int) * tdir.size()));
+            HIP_TRY(c, hipMalloc((void**)&c->tms_rec, sizeof(double) * mc_tms_record_fields() * tcell.size()));
             HIP_TRY(c, hipMemcpy(c->tms_cell, tcell.data(), sizeof(int) * tcell.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMemcpy(c->tms_dir, tdir.data(), sizeof(int) * tdir.size(), hipMemcpyHostToDevice));
         }
@@ -1551,7 +1555,8 @@ int lbm_tms_bc(lbm_ctx* c)
         if (!c->tms_unsupported.empty()) return fail(c, LBM_E_UNSUPPORTED, c->tms_unsupported);
         // outflow, inflow and isothermal-wall sweeps (lbm-bc.cpp:126, 463, 941)
         if (int rc = DISPATCH(c, launch(c, "mc_tms_outflow", [&] {
-                launch_mc_tms<P>(make_mview<P>(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->stream);
+                launch_mc_tms<P>(make_mview<P>(c, 0, 0), c->tms_cell, c->tms_dir, c->tms_count, c->tms_rec,
+                                 c->stream);
             })))
             return rc;
         if (int rc = DISPATCH(c, launch(c, "mc_tms_inflow", [&] {

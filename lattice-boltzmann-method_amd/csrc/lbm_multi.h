@@ -117,9 +117,11 @@ void launch_mc_diag_mach(const MultiView<P>& v, double* partials, hipStream_t s)
 constexpr double NSCBC_SIGMA_OUT = 0.25;
 
 // TMS_BC outflow reconstruction over `count` interface cells (cell index and
-// the axis-aligned direction whose upstream neighbour is the TYPE_O_C ghost).
+// the axis-aligned direction whose upstream neighbour is the TYPE_O_C ghost);
+// `rec` holds mc_tms_record_fields() x count doubles of per-cell values.
 template <typename P>
-void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s);
+void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, double* rec, hipStream_t s);
+int mc_tms_record_fields();
 
 // TMS_BC inflow reconstruction for TYPE_I_C ghosts (same list layout; runs
 // after the outflow sweep).

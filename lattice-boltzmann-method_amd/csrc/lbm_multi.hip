@@ -1051,13 +1051,17 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
         const double theta = v.omix[OMIX_THETA * n + c], ushift = v.omix[OMIX_USHIFT * n + c];
         const double dq[3] = {v.omix[OMIX_DQ * n + c], v.omix[(OMIX_DQ + 1) * n + c], v.omix[(OMIX_DQ + 2) * n + c]};
         const GeqAxes<double> G = geq_axes<double>(theta, vel);
+        // all loads before the first store (v.go might alias v.g for all the
+        // compiler knows, so interleaved each load would wait for the last store)
+        double g[Q];
+#pragma unroll
+        for (int l = 0; l < Q; ++l) g[l] = ldg(v, v.g, l, c);
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
             const double ge = geq_dir<double>(G, l, rho, ushift);
             const double cdq = 0.5 * (cx(l) * dq[0] + cy(l) * dq[1] + cz(l) * dq[2]);
             const double gs = (c2(l) == 1) ? ge + cdq : ge;
-            const double g = ldg(v, v.g, l, c);
-            stg(v, v.go, l, c, g + om1 * (ge - g) + (om - om1) * (ge - gs));
+            stg(v, v.go, l, c, g[l] + om1 * (ge - g[l]) + (om - om1) * (ge - gs));
         }
         return;
     }
@@ -1170,13 +1174,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc_collide(MultiView<P> v)
             for (int b = 0; b < 3; ++b) v.omix[(OMIX_DQ + b) * n + c] = dq[b];
         } else {
         const GeqAxes<double> G = geq_axes<double>(theta, vel);
+        double g[Q];
+#pragma unroll
+        for (int l = 0; l < Q; ++l) g[l] = ldg(v, v.g, l, c);
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
             const double ge = geq_dir<double>(G, l, rho, Uint - 1.5 * theta);
             const double cdq = 0.5 * (cx(l) * dq[0] + cy(l) * dq[1] + cz(l) * dq[2]);
             const double gs = (c2(l) == 1) ? ge + cdq : ge;
-            const double g = ldg(v, v.g, l, c);
-            stg(v, v.go, l, c, g + om1 * (ge - g) + (om - om1) * (ge - gs));
+            stg(v, v.go, l, c, g[l] + om1 * (ge - g[l]) + (om - om1) * (ge - gs));
         }
         }
     }
@@ -1206,9 +1212,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc_stream(MultiView<P> v, int first)
         const int si[3] = {x.i + 1 == nx - 1 ? 1 : x.i + 1, x.i, x.i - 1 == 0 ? nx - 2 : x.i - 1};
         const int sj[3] = {x.j + 1 == ny - 1 ? 1 : x.j + 1, x.j, x.j - 1 == 0 ? ny - 2 : x.j - 1};
         const int sk[3] = {x.k + 1 == nz - 1 ? 1 : x.k + 1, x.k, x.k - 1 == 0 ? nz - 2 : x.k - 1};
+        // all 27 pulls before the first store: the buffers are distinct, but
+        // interleaved each load would wait for the previous store
+        P val[Q];
 #pragma unroll
-        for (int l = 0; l < Q; ++l)
-            dst_base[(int64_t)l * n + c] = src_base[(int64_t)l * n + cell(v.d, si[cx(l) + 1], sj[cy(l) + 1], sk[cz(l) + 1])];
+        for (int l = 0; l < Q; ++l) val[l] = at32(src_base + l * n, cell(v.d, si[cx(l) + 1], sj[cy(l) + 1], sk[cz(l) + 1]));
+#pragma unroll
+        for (int l = 0; l < Q; ++l) at32(dst_base + l * n, c) = val[l];
         return;
     }
 #pragma unroll
@@ -1810,8 +1820,26 @@ __device__ inline void sound_DP(const Units& u, double rho, double p, double Y[N
 
 }  // namespace
 
+// The sweep runs in four launches over a record of per-entry values
+// (tms_rec, [TR_*][count] doubles), one thread per entry or per (entry,
+// population set), with the arithmetic of a single per-cell pass:
+//   k_mc_tms_target   the target state (TYPE_O: calculate_moment_point of the
+//                     downstream cell; TYPE_O_C: the NSCBC LODI step);
+//   k_mc_tms_local    per (entry, species): the local moments with the target
+//                     populations in the unknown directions;
+//   k_mc_tms_close    per entry: mixture sums, calculate_temp, local state;
+//   k_mc_tms_rebuild  per (entry, species or energy): the reconstruction.
+// Each entry is a distinct cell, and only the last launch writes populations,
+// each thread those of its own cell.
+enum {
+    TR_RHO_OUT = 0, TR_VOUT = 1, TR_TSO = 4, TR_U_OUT = 5, TR_TH_OUT = 6, TR_MASK = 7, TR_RA_OUT = 8,
+    TR_RA_LOC = TR_RA_OUT + NS, TR_MA = TR_RA_LOC + NS, TR_RHO_LOC = TR_MA + 3 * NS, TR_VLOC = TR_RHO_LOC + 1,
+    TR_TSL = TR_VLOC + 3, TR_U_LOC = TR_TSL + 1, TR_TH_LOC = TR_U_LOC + 1, TR_COUNT = TR_TH_LOC + 1
+};
+
 template <typename P>
-__global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells, const int* dirs, int count)
+__global__ __launch_bounds__(64) void k_mc_tms_target(MultiView<P> v, const int* cells, const int* dirs, int count,
+                                                      double* rec)
 {
     const int t = blockIdx.x * 64 + threadIdx.x;
     if (t >= count) return;
@@ -1914,48 +1942,83 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     }
     // ---- equilibria of the target state (calculate_feq_geq, lbm-edf.cpp:198-209)
     const double Tso = v.u.si_temp(T_out);
-    const double U_out = v.u.energy_mass(u_mass(Tso, Yo));
-    const double th_out = v.u.energy_mass(R_UNIV * Tso / mean_W(Yo));
     uint32_t unknown = 0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const uint8_t tq = v.type[cell(v.d, x.i - cx(q), x.j - cy(q), x.k - cz(q))];
         if (tq == TYPE_O || tq == TYPE_O_C) unknown |= 1u << q;
     }
-    const double zero[3] = {0, 0, 0};
-    // ---- local moments with the target populations in the unknown directions
-    double ra_loc[NS], ma[NS][3], re_loc = 0;
-    double rho_loc = 0, mloc[3] = {0, 0, 0};
-    for (int a = 0; a < NS; ++a) {
-        const double th_a = v.u.energy_mass(R_UNIV * Tso / W[a]);
-        Axis3<double> Fo[3];
-        feq_axes<double>(Fo, vout, th_a, zero);
-        double r = 0, m0 = 0, m1 = 0, m2 = 0;
+    auto R = [&](int f) -> double& { return rec[(int64_t)f * count + t]; };
+    R(TR_RHO_OUT) = rho_out;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out[a]) : ldf_in(v, a, q, c);
-            r += f;
-            if (cx(q)) m0 += cx(q) * f;
-            if (cy(q)) m1 += cy(q) * f;
-            if (cz(q)) m2 += cz(q) * f;
-        }
-        ra_loc[a] = r;
-        ma[a][0] = m0;
-        ma[a][1] = m1;
-        ma[a][2] = m2;
+    for (int d = 0; d < 3; ++d) R(TR_VOUT + d) = vout[d];
+    R(TR_TSO) = Tso;
+    R(TR_U_OUT) = v.u.energy_mass(u_mass(Tso, Yo));
+    R(TR_TH_OUT) = v.u.energy_mass(R_UNIV * Tso / mean_W(Yo));
+    R(TR_MASK) = (double)unknown;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) R(TR_RA_OUT + a) = ra_out[a];
+}
+
+// grid (count / 64, NS): local moments of species blockIdx.y
+template <typename P>
+__global__ __launch_bounds__(64) void k_mc_tms_local(MultiView<P> v, const int* cells, int count, double* rec)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const int a = blockIdx.y;
+    const int c = cells[t];
+    auto R = [&](int f) -> double& { return rec[(int64_t)f * count + t]; };
+    const uint32_t unknown = (uint32_t)R(TR_MASK);
+    const double vout[3] = {R(TR_VOUT), R(TR_VOUT + 1), R(TR_VOUT + 2)};
+    const double ra_out = R(TR_RA_OUT + a);
+    const double zero[3] = {0, 0, 0};
+    const double th_a = v.u.energy_mass(R_UNIV * R(TR_TSO) / W[a]);
+    Axis3<double> Fo[3];
+    feq_axes<double>(Fo, vout, th_a, zero);
+    double r = 0, m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double f = (unknown >> q & 1u) ? feq_dir<double>(Fo, q, ra_out) : ldf_in(v, a, q, c);
+        r += f;
+        if (cx(q)) m0 += cx(q) * f;
+        if (cy(q)) m1 += cy(q) * f;
+        if (cz(q)) m2 += cz(q) * f;
     }
+    R(TR_RA_LOC + a) = r;
+    R(TR_MA + 3 * a) = m0;
+    R(TR_MA + 3 * a + 1) = m1;
+    R(TR_MA + 3 * a + 2) = m2;
+}
+
+template <typename P>
+__global__ __launch_bounds__(64) void k_mc_tms_close(MultiView<P> v, const int* cells, int count, double* rec)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t];
+    double* const* Fd = v.fld;
+    auto R = [&](int f) -> double& { return rec[(int64_t)f * count + t]; };
+    const uint32_t unknown = (uint32_t)R(TR_MASK);
+    const double rho_out = R(TR_RHO_OUT), th_out = R(TR_TH_OUT), U_out = R(TR_U_OUT);
+    const double vout[3] = {R(TR_VOUT), R(TR_VOUT + 1), R(TR_VOUT + 2)};
+    double re_loc = 0, rho_loc = 0, mloc[3] = {0, 0, 0};
     const GeqAxes<double> Go = geq_axes<double>(th_out, vout);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         re_loc += (unknown >> q & 1u) ? geq_dir<double>(Go, q, rho_out, U_out - 1.5 * th_out)
                                       : ldg(v, v.g, q, c);
+    double ra_loc[NS], ma[NS][3];
     for (int a = 0; a < NS; ++a) {
+        ra_loc[a] = R(TR_RA_LOC + a);
         rho_loc += ra_loc[a];
-        mloc[0] += ma[a][0];
-        mloc[1] += ma[a][1];
-        mloc[2] += ma[a][2];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ma[a][d] = ra_loc[a] > 0.0 ? ma[a][d] / ra_loc[a] : 0.0;
+        for (int d = 0; d < 3; ++d) {
+            ma[a][d] = R(TR_MA + 3 * a + d);
+            mloc[d] += ma[a][d];
+            ma[a][d] = ra_loc[a] > 0.0 ? ma[a][d] / ra_loc[a] : 0.0;
+            R(TR_MA + 3 * a + d) = ma[a][d];
+        }
     }
     double vloc[3];
 #pragma unroll
@@ -1970,30 +2033,61 @@ __global__ __launch_bounds__(64) void k_mc_tms(MultiView<P> v, const int* cells,
     normalize(Yl);
     const double T_loc = v.u.temp(T_from_UV(v.u.si_energy_mass(Uloc), Yl, v.u.si_temp(Fd[LBM_FIELD_T][c])));
     const double Tsl = v.u.si_temp(T_loc);
-    const double U_loc = v.u.energy_mass(u_mass(Tsl, Yl));
-    const double th_loc = v.u.energy_mass(R_UNIV * Tsl / mean_W(Yl));
-    // ---- reconstruction: unknown 2 f_out - f_loc, known f_out + f - f_loc
-    for (int a = 0; a < NS; ++a) {
-        const double tho = v.u.energy_mass(R_UNIV * Tso / W[a]);
-        const double thl = v.u.energy_mass(R_UNIV * Tsl / W[a]);
+    R(TR_RHO_LOC) = rho_loc;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) R(TR_VLOC + d) = vloc[d];
+    R(TR_TSL) = Tsl;
+    R(TR_U_LOC) = v.u.energy_mass(u_mass(Tsl, Yl));
+    R(TR_TH_LOC) = v.u.energy_mass(R_UNIV * Tsl / mean_W(Yl));
+}
+
+// grid (count / 64, NS + 1): reconstruction of species blockIdx.y (NS: the
+// energy populations); unknown 2 f_out - f_loc, known f_out + f - f_loc
+template <typename P>
+__global__ __launch_bounds__(64) void k_mc_tms_rebuild(MultiView<P> v, const int* cells, int count, const double* rec)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const int a = blockIdx.y;
+    const int c = cells[t];
+    auto R = [&](int f) { return rec[(int64_t)f * count + t]; };
+    const uint32_t unknown = (uint32_t)R(TR_MASK);
+    const double vout[3] = {R(TR_VOUT), R(TR_VOUT + 1), R(TR_VOUT + 2)};
+    if (a < NS) {
+        const double zero[3] = {0, 0, 0};
+        const double ma[3] = {R(TR_MA + 3 * a), R(TR_MA + 3 * a + 1), R(TR_MA + 3 * a + 2)};
+        const double ra_out = R(TR_RA_OUT + a), ra_loc = R(TR_RA_LOC + a);
+        const double tho = v.u.energy_mass(R_UNIV * R(TR_TSO) / W[a]);
+        const double thl = v.u.energy_mass(R_UNIV * R(TR_TSL) / W[a]);
         Axis3<double> Fo[3], Fl[3];
         feq_axes<double>(Fo, vout, tho, zero);
-        feq_axes<double>(Fl, ma[a], thl, zero);
+        feq_axes<double>(Fl, ma, thl, zero);
+        // every load before the first store: the stores go to the same
+        // population set, so interleaved they would serialise the loads
+        double fin[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fin[q] = ldf_in(v, a, q, c);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const double fo = feq_dir<double>(Fo, q, ra_out[a]);
-            const double flc = feq_dir<double>(Fl, q, ra_loc[a]);
-            const double fin = ldf_in(v, a, q, c);
-            stf_in(v, a, q, c, (unknown >> q & 1u) ? 2 * fo - flc : fo + fin - flc);
+            const double fo = feq_dir<double>(Fo, q, ra_out);
+            const double flc = feq_dir<double>(Fl, q, ra_loc);
+            stf_in(v, a, q, c, (unknown >> q & 1u) ? 2 * fo - flc : fo + fin[q] - flc);
         }
+        return;
     }
+    const double rho_out = R(TR_RHO_OUT), th_out = R(TR_TH_OUT), U_out = R(TR_U_OUT);
+    const double rho_loc = R(TR_RHO_LOC), th_loc = R(TR_TH_LOC), U_loc = R(TR_U_LOC);
+    const double vloc[3] = {R(TR_VLOC), R(TR_VLOC + 1), R(TR_VLOC + 2)};
+    const GeqAxes<double> Go = geq_axes<double>(th_out, vout);
     const GeqAxes<double> Gl = geq_axes<double>(th_loc, vloc);
+    double gin[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) gin[q] = ldg(v, v.g, q, c);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const double go = geq_dir<double>(Go, q, rho_out, U_out - 1.5 * th_out);
         const double gl = geq_dir<double>(Gl, q, rho_loc, U_loc - 1.5 * th_loc);
-        const double gin = ldg(v, v.g, q, c);
-        stg(v, v.g, q, c, (unknown >> q & 1u) ? 2 * go - gl : go + gin - gl);
+        stg(v, v.g, q, c, (unknown >> q & 1u) ? 2 * go - gl : go + gin[q] - gl);
     }
 }
 
@@ -2545,14 +2639,22 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s)
 }
 
 template <typename P>
-void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s)
+void launch_mc_tms(const MultiView<P>& v, const int* cells, const int* dirs, int count, double* rec, hipStream_t s)
 {
     if constexpr (!kernels_built<P>()) {
         return;  // refused before launch (lbm_create, surrogate entry points)
     } else {
-        if (count > 0) hipLaunchKernelGGL(k_mc_tms<P>, dim3((count + 63) / 64), dim3(64), 0, s, v, cells, dirs, count);
+        if (count <= 0) return;
+        const unsigned nb = (unsigned)((count + 63) / 64);
+        hipLaunchKernelGGL(k_mc_tms_target<P>, dim3(nb), dim3(64), 0, s, v, cells, dirs, count, rec);
+        hipLaunchKernelGGL(k_mc_tms_local<P>, dim3(nb, NS), dim3(64), 0, s, v, cells, count, rec);
+        hipLaunchKernelGGL(k_mc_tms_close<P>, dim3(nb), dim3(64), 0, s, v, cells, count, rec);
+        hipLaunchKernelGGL(k_mc_tms_rebuild<P>, dim3(nb, NS + 1), dim3(64), 0, s, v, cells, count,
+                           static_cast<const double*>(rec));
     }
 }
+
+int mc_tms_record_fields() { return TR_COUNT; }
 
 template <typename P>
 void launch_mc_tms_in(const MultiView<P>& v, const int* cells, const int* dirs, int count, hipStream_t s)
@@ -2674,14 +2776,14 @@ template void launch_mc_aa_save<double>(const MultiView<double>&, bool, hipStrea
 template void launch_mc_aa_save<float>(const MultiView<float>&, bool, hipStream_t);
 template void launch_mc_aa_collide<double>(const MultiView<double>&, hipStream_t);
 template void launch_mc_aa_collide<float>(const MultiView<float>&, hipStream_t);
-template void launch_mc_tms<float>(const MultiView<float>&, const int*, const int*, int, hipStream_t);
+template void launch_mc_tms<float>(const MultiView<float>&, const int*, const int*, int, double*, hipStream_t);
 template void launch_mc_sur_chemistry<float>(const MultiView<float>&, const sur::Mlp&, uint16_t*, float*,
                                              uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);
 template void launch_mc_chem_list<float>(const MultiView<float>&, const unsigned*, const unsigned*, hipStream_t);
 template void launch_mc_sur_chemistry<double>(const MultiView<double>&, const sur::Mlp&, uint16_t*, float*,
                                               uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);
 template void launch_mc_chem_list<double>(const MultiView<double>&, const unsigned*, const unsigned*, hipStream_t);
-template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
+template void launch_mc_tms<double>(const MultiView<double>&, const int*, const int*, int, double*, hipStream_t);
 template void launch_mc_tms_in<double>(const MultiView<double>&, const int*, const int*, int, hipStream_t);
 template void launch_mc_tms_wall<double>(const MultiView<double>&, const int*, int, hipStream_t);
 template void launch_mc_tms_wall<float>(const MultiView<float>&, const int*, int, hipStream_t);
