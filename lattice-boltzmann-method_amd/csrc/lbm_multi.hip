@@ -1806,13 +1806,20 @@ __device__ inline double T_from_DP(const Units& u, double rho, double p, double 
     return u.temp(Tsi);
 }
 
-// frozen sound speed (lattice) and gamma at the DP state of a cell
-__device__ inline void sound_DP(const Units& u, double rho, double p, double Y[NS], double* a_lat, double* gam)
+// frozen sound speed (lattice) and gamma at the DP state of a cell.  The
+// state takes a normalised copy of Y (setMassFractions clips and scales its
+// own copy): the caller's mass fractions stay as they were, since the
+// boundary update builds the outflow densities from them
+// (impose_nscbc.cpp:479-545)
+__device__ inline void sound_DP(const Units& u, double rho, double p, const double Y[NS], double* a_lat, double* gam)
 {
+    double Yn[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) Yn[k] = Y[k];
     double Tsi;
-    T_from_DP(u, rho, p, Y, &Tsi);
-    const double cp = cp_mass(Tsi, Y);
-    const double Wm = mean_W(Y);
+    T_from_DP(u, rho, p, Yn, &Tsi);
+    const double cp = cp_mass(Tsi, Yn);
+    const double Wm = mean_W(Yn);
     const double cv = cp - R_UNIV / Wm;
     *gam = cp / cv;
     *a_lat = u.u(sqrt(*gam * R_UNIV * Tsi / Wm));
@@ -2147,7 +2154,8 @@ __global__ __launch_bounds__(64) void k_mc_tms_in(MultiView<P> v, const int* cel
     double Yn[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) Yn[a] = Yc[a];
-    sound_DP(v.u, rho, p, Yn, &a_s, &gam);  // normalises Yn
+    sound_DP(v.u, rho, p, Yn, &a_s, &gam);
+    normalize(Yn);  // the DP state's mass fractions (setMassFractions)
     // fd_uw (FD.cpp:64-74) with the interface direction as the velocity
     auto fd_uw = [&](double s0, double s1) { return isign < 0 ? (s0 - s1) / 1 : (-s0 + s1) / 1; };
     const double du = fd_uw(Fd[LBM_FIELD_U + idir][c], Fd[LBM_FIELD_U + idir][c1]);

@@ -248,8 +248,8 @@ def reacting_box(nx, ny, nz, T_K=1400.0, X=None, amp_T=0.0, amp_u=0.0, seed=2026
 X_FUEL_ML = {"H2": 0.3, "N2": 0.7}  # N2-diluted hydrogen stream of the mixing layer
 
 
-def _all_species(d, minim=1e-6):
-    out = {n: minim for n in H2AIR_SPECIES}
+def _all_species(d, minim=1e-6, species=None):
+    out = {n: minim for n in (species or H2AIR_SPECIES)}
     out.update(d)
     return out
 
@@ -301,13 +301,14 @@ def reacting_channel(nx, ny, nz, T_K=900.0, U=0.01, T_in_K=950.0, T_wall_K=800.0
 
 
 def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, noise=0.01, seed=20261206,
-                 X_fuel=None, X_ox=None, minim=1e-6):
+                 X_fuel=None, X_ox=None, minim=1e-6, species=None):
     """Temporal H2-air / hot-air mixing layer: periodic x and z, NSCBC outflow
     (TYPE_O_C) on both y faces (the y ghost layers are typed last, so edges and
     corners are TYPE_O_C).  Lower stream: N2-diluted H2 at T_fuel; upper
     stream: air at T_ox; u = U_s tanh((y - y0)/delta) with seeded broadband
     noise of `noise` U_s on u, v, w; p = 1 atm everywhere (also the outflow
-    target pressure held by the ghosts)."""
+    target pressure held by the ghosts).  `species`: the species order of a
+    mechanism other than H2-air (the others at `minim`)."""
     rng = np.random.default_rng(seed)
     fl = default_fields(nx, ny, nz)
     t = fl["type"]
@@ -322,8 +323,8 @@ def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, no
     j = np.arange(shape[1], dtype=np.float64)
     s = 0.5 * (1.0 + np.tanh((j - 0.5 * (shape[1] - 1)) / delta))  # 0 fuel side .. 1 oxidiser side
     s3 = np.broadcast_to(s[None, :, None], shape)
-    Xf = mole_vector(_all_species(X_fuel or X_FUEL_ML, minim))
-    Xo = mole_vector(_all_species(X_ox or X_AIR, minim))
+    Xf = mole_vector(_all_species(X_fuel or X_FUEL_ML, minim, species), species=species)
+    Xo = mole_vector(_all_species(X_ox or X_AIR, minim, species), species=species)
     fl["X"] = (1.0 - s3)[None] * Xf[:, None, None, None] + s3[None] * Xo[:, None, None, None]
     fl["T"][...] = (T_fuel + (T_ox - T_fuel) * s3) / LATTICE_K
     fl["p"][...] = lattice_p(ONE_ATM)
@@ -337,7 +338,7 @@ def mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, no
 
 
 def double_mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=None, noise=0.01, seed=20261206,
-                        X_fuel=None, X_ox=None, minim=1e-6):
+                        X_fuel=None, X_ox=None, minim=1e-6, species=None):
     """Periodic twin of mixing_layer: the same streams and profiles, with the
     fuel stream in the middle half of y between two tanh layers at ny/4 and
     3 ny/4, so that the box is periodic in every direction (TYPE_P ghosts)
@@ -352,8 +353,8 @@ def double_mixing_layer(nx, ny, nz, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=N
     y1, y2 = 0.25 * ny, 0.75 * ny
     s = 1.0 - 0.5 * (np.tanh((j - y1) / delta) - np.tanh((j - y2) / delta))  # 1 oxidiser .. 0 fuel
     s3 = np.broadcast_to(s[None, :, None], shape)
-    Xf = mole_vector(_all_species(X_fuel or X_FUEL_ML, minim))
-    Xo = mole_vector(_all_species(X_ox or X_AIR, minim))
+    Xf = mole_vector(_all_species(X_fuel or X_FUEL_ML, minim, species), species=species)
+    Xo = mole_vector(_all_species(X_ox or X_AIR, minim, species), species=species)
     fl["X"] = (1.0 - s3)[None] * Xf[:, None, None, None] + s3[None] * Xo[:, None, None, None]
     fl["T"][...] = (T_fuel + (T_ox - T_fuel) * s3) / LATTICE_K
     fl["p"][...] = lattice_p(ONE_ATM)

@@ -67,11 +67,12 @@ def seed_gpu_from_oracle(lat, s):
         lat.set_field(name, s.ptens[a, b].reshape(shape))
     for d, name in enumerate(("qx", "qy", "qz")):
         lat.set_field(name, s.eflux[d].reshape(shape))
-    lat.set_species_field("X", s.X.reshape((9,) + shape))
-    lat.set_species_field("rho", s.rho_a.reshape((9,) + shape))
+    ns = s.ns
+    lat.set_species_field("X", s.X.reshape((ns,) + shape))
+    lat.set_species_field("rho", s.rho_a.reshape((ns,) + shape))
     for d, name in enumerate("uvw"):
-        lat.set_species_field(name, s.u_a[d].reshape((9,) + shape))
-    lat.set_populations(POP_F, s.f.reshape((9, 27) + shape))
-    lat.set_populations(POP_FPC, s.fpc.reshape((9, 27) + shape))
+        lat.set_species_field(name, s.u_a[d].reshape((ns,) + shape))
+    lat.set_populations(POP_F, s.f.reshape((ns, 27) + shape))
+    lat.set_populations(POP_FPC, s.fpc.reshape((ns, 27) + shape))
     lat.set_populations(POP_G, s.g.reshape((27,) + shape))
     lat.set_populations(POP_GPC, s.gpc.reshape((27,) + shape))

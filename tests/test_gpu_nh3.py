@@ -21,6 +21,7 @@ from lbm_amd.capi import LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_IN
     POP_F, POP_G, POP_GPC, LbmError, MixtureLattice
 from oracle.chem import Mechanism, mechanism_path
 from oracle.multicomp import MultiStateBC, Units
+from mode_compare import seed_gpu_from_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -199,3 +200,47 @@ def test_fused_and_inplace_modes_are_bitwise_strict(mode):
     assert np.array_equal(out[mode]["f"].reshape(ns, 27, -1)[:, :, Ff],
                           out[LBM_MODE_STRICT]["f"].reshape(ns, 27, -1)[:, :, Ff])
     assert np.array_equal(out[mode]["g"].reshape(27, -1)[:, Ff], out[LBM_MODE_STRICT]["g"].reshape(27, -1)[:, Ff])
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_mixing_layer_outflow_calls_match_oracle(reaction):
+    """An NH3/H2 stream against hot air with NSCBC outflow on both y faces:
+    every call of three steps on the GPU and in the oracle from the same
+    state (the device re-seeded from the oracle before each call: fd_fuw's
+    upwind choice branches on ~0 diffusion velocities next to the outflow
+    faces), the outflow sweep's four launches included."""
+    fl = cases.mixing_layer(4, 8, 4, T_fuel=600.0, T_ox=1600.0, U_s=0.02, X_fuel={"NH3": 0.3, "H2": 0.1, "N2": 0.6},
+                            species=cases.MECH_SPECIES[MECH])
+    lat, s = pair(fl, reaction=reaction)
+    ns = s.ns
+    try:
+        lat.Init()
+        s.init()
+        F = s.type == 0
+        for step in range(1, 4):
+            seed_gpu_from_oracle(lat, s)
+            lat.Collide_Species()
+            s.collide_species(reaction)
+            e = relerr(lat.get_populations(POP_FPC).reshape(ns, 27, -1)[:, :, F], s.fpc[:, :, F])
+            assert e < TOL, f"step {step} species fpc {e:.3e}"
+            seed_gpu_from_oracle(lat, s)
+            lat.Collide()
+            s.collide()
+            e = relerr(lat.get_populations(POP_GPC).reshape(27, -1)[:, F], s.gpc[:, F])
+            assert e < TOL, f"step {step} gpc {e:.3e}"
+            seed_gpu_from_oracle(lat, s)
+            lat.Streaming()
+            s.stream()
+            seed_gpu_from_oracle(lat, s)
+            lat.TMS_BC()
+            s.tms_bc()
+            e = relerr(lat.get_populations(POP_F).reshape(ns, 27, -1)[:, :, F], s.f[:, :, F])
+            assert e < TOL, f"step {step} species f after TMS {e:.3e}"
+            e = relerr(lat.get_populations(POP_G).reshape(27, -1)[:, F], s.g[:, F])
+            assert e < TOL, f"step {step} g after TMS {e:.3e}"
+            seed_gpu_from_oracle(lat, s)
+            lat.calculate_moment()
+            s.moments()
+            check_state(lat, s, TOL, f"step {step}")
+    finally:
+        lat.close()
