@@ -255,7 +255,7 @@ def _all_species(d, minim=1e-6, species=None):
 
 
 def reacting_channel(nx, ny, nz, T_K=900.0, U=0.01, T_in_K=950.0, T_wall_K=800.0, X=None, X_in=None,
-                     p_out_factor=1.0):
+                     p_out_factor=1.0, species=None):
     """Multicomponent channel with the fixed-state boundary kinds of TMS_BC.
 
     Inflow ghost (TYPE_I: u = U, v = 0.05 U, T_in, composition X_in) at
@@ -264,7 +264,8 @@ def reacting_channel(nx, ny, nz, T_K=900.0, U=0.01, T_in_K=950.0, T_wall_K=800.0
     (moving with u = -0.3 U), periodic in z; the x faces are typed last.
     Interior: u = U with transverse waves, T_K, 1 atm, composition X (every
     species present, so that the SPECIES_MIN clamp stays out of the way).
-    Temperatures in K; 'X' holds mole fractions (NS, ...)."""
+    Temperatures in K; 'X' holds mole fractions (NS, ...); `species`: the
+    species order of a mechanism other than H2-air."""
     fl = default_fields(nx, ny, nz)
     t = fl["type"]
     t[:, :, 0] = TYPE_P
@@ -275,9 +276,9 @@ def reacting_channel(nx, ny, nz, T_K=900.0, U=0.01, T_in_K=950.0, T_wall_K=800.0
     t[-1, :, :] = TYPE_O
     shape = t.shape
     i, j, k = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
-    Xv = mole_vector(_all_species(X or X_UNBURNT_PHI1))
-    Xi = mole_vector(_all_species(X_in or X_AIR))
-    fl["X"] = np.zeros((len(H2AIR_SPECIES),) + shape)
+    Xv = mole_vector(_all_species(X or X_UNBURNT_PHI1, species=species), species=species)
+    Xi = mole_vector(_all_species(X_in or X_AIR, species=species), species=species)
+    fl["X"] = np.zeros((len(species or H2AIR_SPECIES),) + shape)
     fl["X"][:] = Xv[:, None, None, None]
     fl["T"][:] = T_K / LATTICE_K
     fl["p"][:] = lattice_p(ONE_ATM)

@@ -5,7 +5,10 @@ rates), through the C ABI of its own library (liblbm_amd_nh3.so).
 The library is compared call by call with the numpy oracle running the same
 mechanism (oracle/chem.py, pinned in test_oracle_nh3.py) on seeded boxes of
 an NH3/H2-air mixture hot enough (1800-2000 K) for the nitrogen chemistry to
-run within a few lattice steps.  Bar: max relative error 1e-10 in rho, u, T,
+run within a few lattice steps, and on the boundary kinds of TMS_BC: an
+NH3/H2 mixing layer with NSCBC outflow, a channel with fixed-state inflow,
+outflow and isothermal walls, and a column between a LODI inflow and an
+NSCBC outflow (per call, from the oracle's state).  Bar: max relative error 1e-10 in rho, u, T,
 p, rho_a and X after each step (the FP64 parity bar), the post-collision
 populations on the same scale.  The fused and in-place modes are bitwise
 equal to the strict calls.  FP32 populations and the chemistry surrogate
@@ -242,5 +245,102 @@ def test_mixing_layer_outflow_calls_match_oracle(reaction):
             lat.calculate_moment()
             s.moments()
             check_state(lat, s, TOL, f"step {step}")
+    finally:
+        lat.close()
+
+
+def per_call_steps(lat, s, reaction, steps=3, moments_first=True):
+    """Collide_Species, Collide, Streaming, TMS_BC and calculate_moment on
+    both sides from the oracle's state before every call; the populations
+    after TMS_BC and the fields after each call that writes them checked."""
+    ns = s.ns
+    F = s.type == 0
+    if moments_first:
+        lat.calculate_moment()
+        s.moments()
+    for step in range(1, steps + 1):
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide_Species()
+        s.collide_species(reaction)
+        e = relerr(lat.get_populations(POP_FPC).reshape(ns, 27, -1)[:, :, F], s.fpc[:, :, F])
+        assert e < TOL, f"step {step} species fpc {e:.3e}"
+        seed_gpu_from_oracle(lat, s)
+        lat.Collide()
+        s.collide()
+        seed_gpu_from_oracle(lat, s)
+        lat.Streaming()
+        s.stream()
+        seed_gpu_from_oracle(lat, s)
+        lat.TMS_BC()
+        s.tms_bc()
+        e = relerr(lat.get_populations(POP_F).reshape(ns, 27, -1)[:, :, F], s.f[:, :, F])
+        assert e < TOL, f"step {step} species f after TMS {e:.3e}"
+        e = relerr(lat.get_populations(POP_G).reshape(27, -1)[:, F], s.g[:, F])
+        assert e < TOL, f"step {step} g after TMS {e:.3e}"
+        check_state(lat, s, TOL, f"step {step} after TMS")
+        seed_gpu_from_oracle(lat, s)
+        lat.calculate_moment()
+        s.moments()
+        check_state(lat, s, TOL, f"step {step}")
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_channel_fixed_state_boundaries_match_oracle(reaction):
+    """TYPE_I inflow of air, TYPE_O outflow (calculate_moment_point of the
+    downstream cell), isothermal TYPE_S walls, one moving, around the NH3/H2
+    mixture at 1500 K: per call, from the oracle's state."""
+    fl = cases.reacting_channel(8, 5, 3, T_K=1500.0, T_in_K=1400.0, T_wall_K=1200.0, X=SEEDED,
+                                X_in=dict({n: 1e-6 for n in cases.MECH_SPECIES[MECH]}, O2=0.21, N2=0.79),
+                                p_out_factor=0.995, species=cases.MECH_SPECIES[MECH])
+    lat, s = pair(fl, reaction=reaction)
+    try:
+        lat.Init()
+        s.init()
+        per_call_steps(lat, s, reaction)
+    finally:
+        lat.close()
+
+
+def inflow_outflow_column(nx, T_K=1500.0, u_in=0.01, seed=5):
+    """Periodic y/z column between a TYPE_I_C inflow ghost (inline LODI
+    target towards its fresh NH3/H2-air state at 1300 K) and a TYPE_O_C
+    outflow ghost (NSCBC) at 1 atm, with a seeded +-2 % temperature and
+    velocity perturbation inside."""
+    rng = np.random.default_rng(seed)
+    fl = cases.default_fields(nx, 3, 3)
+    t = fl["type"]
+    for ax in (1, 2):
+        idx = [slice(None)] * 3
+        for end in (0, -1):
+            idx[ax] = end
+            t[tuple(idx)] = cases.TYPE_P
+    t[0] = cases.TYPE_I_C
+    t[-1] = cases.TYPE_O_C
+    ns = len(cases.MECH_SPECIES[MECH])
+    X = cases.mole_vector(SEEDED, species=cases.MECH_SPECIES[MECH])
+    fl["X"] = np.broadcast_to(X[:, None, None, None], (ns,) + t.shape).copy()
+    fl["T"][...] = T_K / cases.LATTICE_K
+    fl["T"][0] = 1300.0 / cases.LATTICE_K
+    fl["p"][...] = cases.lattice_p(cases.ONE_ATM)
+    fl["u"][...] = u_in
+    fl["v"][...] = 0.0
+    fl["w"][...] = 0.0
+    m = t == cases.TYPE_F
+    fl["T"][m] *= 1.0 + 0.02 * rng.uniform(-1.0, 1.0, int(m.sum()))
+    fl["u"][m] *= 1.0 + 0.02 * rng.uniform(-1.0, 1.0, int(m.sum()))
+    return fl
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_inflow_outflow_column_calls_match_oracle(reaction):
+    """TYPE_I_C inflow (inline LODI, setState_TP density) and TYPE_O_C
+    outflow (NSCBC) of the 31-species mixture: per call, from the oracle's
+    state."""
+    fl = inflow_outflow_column(10)
+    lat, s = pair(fl, reaction=reaction)
+    try:
+        lat.Init()
+        s.init()
+        per_call_steps(lat, s, reaction)
     finally:
         lat.close()
