@@ -126,3 +126,49 @@ def test_fused_reads_between_collide_species_and_streaming():
     finally:
         strict.close()
         fused.close()
+
+
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_overlapped_step_with_surrogate_matches_serial_calls(precision, monkeypatch):
+    """lbm_step in the fused mode runs the chemistry (network pass, fallback
+    list) on a second stream beside the transport pass and the mixture
+    collide.  With the same network, envelope and audits, four steps of it
+    are bitwise those of the five calls issued one by one (a context created
+    with LBM_AMD_OVERLAP=0 steps serially), surrogate counters included."""
+    fl = cases.mixing_layer(6, 12, 5, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=3.0)
+    F = fl["type"].ravel() == 0
+    lats = []
+    try:
+        for env in ("1", "0"):
+            monkeypatch.setenv("LBM_AMD_OVERLAP", env)
+            lat = make(fl, LBM_MODE_FUSED, precision)
+            lats.append(lat)
+        ov, ser = lats
+        print(f"overlapped vs serial surrogate steps precision {precision}", flush=True)
+        ov.surrogate_capture(1 << 16, 1)
+        ov.step(1)  # the strict first step, capturing the plain chemistry's states
+        ov.surrogate_capture_stop()
+        ov.surrogate_train(40, batch=256, lr=1e-3, seed=3)
+        params, norm = ov.surrogate_params().copy(), ov.surrogate_norm().copy()
+        ser.step(1)
+        ser.set_surrogate_params(params)
+        ser.set_surrogate_norm(norm)
+        ov.set_surrogate_params(params)
+        for lat in lats:
+            lat.surrogate_enable(True, audit_stride=3, audit_tol=0.05)
+        for step in range(1, 5):
+            ov.step(1)
+            ser.Collide_Species()
+            ser.Collide()
+            ser.Streaming()
+            ser.TMS_BC()
+            ser.calculate_moment()
+            assert_same(snapshot(ov, F), snapshot(ser, F), f"step {step}")
+            a, b = ov.surrogate_stats(), ser.surrogate_stats()
+            a.pop("captured"), b.pop("captured")  # only `ov` captured training rows
+            assert a == b
+        st = ov.surrogate_stats()
+        assert st["predicted"] > 0 and st["audited"] > 0
+    finally:
+        for lat in lats:
+            lat.close()
