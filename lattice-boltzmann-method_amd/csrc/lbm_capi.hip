@@ -1682,8 +1682,19 @@ int collide_overlapped(lbm_ctx* c)
         HIP_TRY(c, hipEventCreateWithFlags(&c->ev_cstart, hipEventDisableTiming));
         HIP_TRY(c, hipEventCreateWithFlags(&c->ev_cdone, hipEventDisableTiming));
     }
+    if (inplace(c) && c->pushed)
+        return fail(c, LBM_E_STATE,
+                    "the in-place mode's species are already collided into the next layout: lbm_stream must "
+                    "follow lbm_collide_species");
     HIP_TRY(c, hipEventRecord(c->ev_cstart, c->stream));
     HIP_TRY(c, hipStreamWaitEvent(c->cstream, c->ev_cstart, 0));
+    // the in-place mode's save pass (it reads the species populations and
+    // densities, which the chemistry leaves alone)
+    if (inplace(c))
+        if (int rc = aa_save(c)) {
+            (void)hipEventRecord(c->ev_cdone, c->cstream);
+            return rc;
+        }
     int rc = DISPATCH(c, [&] {
         const MultiView<P> v = make_mview<P>(c, 0, 1);
         if (!c->sur_on)
@@ -1704,8 +1715,12 @@ int collide_overlapped(lbm_ctx* c)
     c->omix_fresh = !rc && fp32(c);
     if (!rc) rc = run_mc(c, McKernel::Collide, "mc_collide_mixture", 0, 1);
     HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_cdone, 0));
-    if (!rc) rc = run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1);
-    if (!rc) c->pushed = true;
+    if (!rc && inplace(c)) {
+        rc = aa_collide_species(c);
+    } else if (!rc) {
+        rc = run_mc(c, McKernel::CollideSpeciesPush, "mc_collide_species_push", 0, 1);
+        if (!rc) c->pushed = true;
+    }
     ++c->step;
     return rc;
 }
@@ -1717,8 +1732,10 @@ int lbm_step(lbm_ctx* c, int nsteps)
     if (!c) return LBM_E_ARG;
     for (int s = 0; s < nsteps; ++s) {
         int rc;
-        // the overlapped schedule covers the fused mode's pushed steps with chemistry
-        const bool ov = c->overlap && multi(c) && fused(c) && c->push_ready && c->types_set && c->cfg.reaction;
+        // the overlapped schedule covers the fused mode's pushed steps and the
+        // in-place mode, with chemistry
+        const bool ov = c->overlap && multi(c) && c->types_set && c->cfg.reaction &&
+                        ((fused(c) && c->push_ready) || inplace(c));
         if (ov) {
             if ((rc = collide_overlapped(c))) return rc;
         } else if ((rc = lbm_collide_species(c)) || (rc = lbm_collide(c))) {

@@ -18,8 +18,8 @@ import numpy as np
 import pytest
 
 from lbm_amd import cases
-from lbm_amd.capi import (LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_STRICT, POP_F, POP_FPC, POP_G,
-                          POP_GPC, LbmError, MixtureLattice)
+from lbm_amd.capi import (LBM_FP32_SHIFTED, LBM_FP64, LBM_MODE_FUSED, LBM_MODE_INPLACE, LBM_MODE_STRICT, POP_F,
+                          POP_FPC, POP_G, POP_GPC, LbmError, MixtureLattice)
 from lbm_amd.cases import TYPE_A, TYPE_FS
 from mode_compare import assert_same, compare as _compare, make, snapshot
 
@@ -128,20 +128,22 @@ def test_fused_reads_between_collide_species_and_streaming():
         fused.close()
 
 
-@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
-def test_overlapped_step_with_surrogate_matches_serial_calls(precision, monkeypatch):
-    """lbm_step in the fused mode runs the chemistry (network pass, fallback
-    list) on a second stream beside the transport pass and the mixture
-    collide.  With the same network, envelope and audits, four steps of it
-    are bitwise those of the five calls issued one by one (a context created
-    with LBM_AMD_OVERLAP=0 steps serially), surrogate counters included."""
+@pytest.mark.parametrize("precision,mode", [(LBM_FP64, LBM_MODE_FUSED), (LBM_FP32_SHIFTED, LBM_MODE_FUSED),
+                                            (LBM_FP32_SHIFTED, LBM_MODE_INPLACE)])
+def test_overlapped_step_with_surrogate_matches_serial_calls(precision, mode, monkeypatch):
+    """lbm_step in the fused and in-place modes runs the chemistry (network
+    pass, fallback list) on a second stream beside the transport pass and the
+    mixture collide.  With the same network, envelope and audits, four steps
+    of it are bitwise those of the five calls issued one by one (a context
+    created with LBM_AMD_OVERLAP=0 steps serially), surrogate counters
+    included."""
     fl = cases.mixing_layer(6, 12, 5, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=3.0)
     F = fl["type"].ravel() == 0
     lats = []
     try:
         for env in ("1", "0"):
             monkeypatch.setenv("LBM_AMD_OVERLAP", env)
-            lat = make(fl, LBM_MODE_FUSED, precision)
+            lat = make(fl, mode, precision)
             lats.append(lat)
         ov, ser = lats
         print(f"overlapped vs serial surrogate steps precision {precision}", flush=True)
