@@ -397,7 +397,27 @@ def mixing_c5_bench(args):
     train_s = time.perf_counter() - t1
     lat.surrogate_enable(True, audit_stride=64, audit_tol=0.5)
     log(f"surrogate trained in {train_s:.1f} s; timing {steps} steps")
-    sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS)
+    # maintained surrogate as in mixing256 (fallback and audited states join a
+    # training set of up to 16 M rows; a window of bad updates above
+    # REFRESH_RATE refreshes the network beside the steps), run in chunks of
+    # 100 steps with a progress line after each
+    lat.surrogate_capture(cap_steps * cells // cap_stride + 16_000_000, 1 << 30)
+    maint = {"step_s": 0.0, "refresh_s": 0.0, "refreshes": 0}
+
+    def run(k):
+        done = 0
+        while done < k:
+            m = min(100, k - done)
+            st, rs, nr = lat.step_with_refresh(m, window=args.refresh_window, max_rate=args.refresh_rate,
+                                               iters=args.refresh_iters, batch=batch, overlap=True)
+            maint["step_s"] += st
+            maint["refresh_s"] += rs
+            maint["refreshes"] += nr
+            done += m
+            if k > 100:
+                log(f"step {done} of {k}, {maint['refreshes']} refreshes")
+
+    sec, kern = timed(lat, steps, args.warmup, MULTI_KERNELS, run=run)
     mlups = cells * steps / sec / 1e6
     log(f"{mlups:.1f} MLUPS")
     sstats = lat.surrogate_stats()
@@ -458,7 +478,11 @@ def mixing_c5_bench(args):
     rel = np.abs(T_f - T64) / T64
     extra = {"kernels": {n: {"avg_ms": round(v["avg_ms"], 4), "launches": v["launches"]} for n, v in kern.items()},
              "device_bytes_fp32": dev_bytes, "nan_count": diag["nan_count"],
-             "surrogate": {"train_s": round(train_s, 2), "capture_stride": cap_stride, **sstats},
+             "surrogate": {"train_s": round(train_s, 2), "capture_stride": cap_stride, **sstats,
+                           "maintenance": {"window": args.refresh_window, "max_rate": args.refresh_rate,
+                                           "iters": args.refresh_iters, "overlapped": True,
+                                           "refreshes": maint["refreshes"],
+                                           "refresh_s": round(maint["refresh_s"], 3)}},
              "fallback_rate_over_run": round((sstats["fallback"] + sstats["audit_failures"]) /
                                              max(sstats["predicted"] + sstats["fallback"], 1), 6),
              "twin": {"grid": [tx, ty, tz], "precision": "f64", "chemistry": "plain", "steps": twin_steps,

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -174,6 +176,7 @@ struct lbm_ctx {
     bool refreshing = false;
     // lbm_step's overlapped schedule: the chemistry on its own stream beside
     // the transport pass and the mixture collide (LBM_AMD_OVERLAP=0: serial)
+    int64_t low_max = INT64_MAX;  // LBM_AMD_LOW_MAX: most floor pairs the in-place list may hold at once
     hipStream_t cstream = nullptr;
     hipEvent_t ev_cstart = nullptr, ev_cdone = nullptr;
     bool overlap = true;
@@ -607,6 +610,7 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     if ((e = hipSetDevice(cfg->device)) != hipSuccess) return bail("hipSetDevice", e);
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("hipStreamCreate", e);
     if (const char* ov = getenv("LBM_AMD_OVERLAP")) c->overlap = ov[0] != '0';
+    if (const char* lm = getenv("LBM_AMD_LOW_MAX")) c->low_max = std::max<int64_t>(1, std::atoll(lm));
     const size_t pop_bytes = (size_t)n * Q * c->esize;
     for (int b = 0; b < 2; ++b) {
         if (cfg->n_species == 0) {  // the mixture f exists only without species (lbm.hpp:62-64)
@@ -1405,6 +1409,18 @@ int lbm_init(lbm_ctx* c)
 namespace {
 
 // (Re)allocates the low species list with room for `cap` entries.
+// Whether a list of `cap` pairs fits: within the LBM_AMD_LOW_MAX pairs the
+// environment allows (a test hook) and the device memory free once the
+// current list is released.
+bool low_list_fits(lbm_ctx* c, int64_t cap)
+{
+    if (cap > c->low_max) return false;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return true;
+    const size_t need = (sizeof(int) + 1 + c->esize * Q) * (size_t)cap;
+    return need + (64u << 20) <= free_b + c->low_bytes;
+}
+
 int aa_grow(lbm_ctx* c, int64_t cap)
 {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1415,6 +1431,10 @@ int aa_grow(lbm_ctx* c, int64_t cap)
     c->low_cap = 0;
     c->bytes -= c->low_bytes;
     c->low_bytes = 0;
+    if (!low_list_fits(c, cap))
+        return fail(c, LBM_E_HIP,
+                    "the in-place mode's list of species held at SPECIES_MIN needs " + std::to_string(cap) +
+                        " (cell, species) pairs, more than the device memory free");
     HIP_TRY(c, hipMalloc((void**)&c->low_cell, sizeof(int) * cap));
     HIP_TRY(c, hipMalloc((void**)&c->low_sp, cap));
     HIP_TRY(c, hipMalloc(&c->low_keep, c->esize * Q * cap));
@@ -1430,10 +1450,12 @@ int aa_grow(lbm_ctx* c, int64_t cap)
 int aa_save(lbm_ctx* c)
 {
     if (!c->low_cell) {  // first use: room for 1/16 of all (cell, species) pairs
-        const int64_t cap = std::max<int64_t>(4096, c->d.n * c->ns / 16);
+        const int64_t cap = std::min<int64_t>(c->low_max, std::max<int64_t>(4096, c->d.n * c->ns / 16));
         if (int rc = aa_grow(c, cap)) return rc;
     }
-    int rc = DISPATCH(c, launch(c, "mc_aa_save", [&] { launch_mc_aa_save<P>(make_mview<P>(c, 0, 0), c->aa_first, c->stream); }));
+    int rc = DISPATCH(c, launch(c, "mc_aa_save", [&] {
+        launch_mc_aa_save<P>(make_mview<P>(c, 0, 0), c->aa_first, 0, (int)c->ns, c->stream);
+    }));
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(c->low_count_host, c->low_count, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev_save, c->stream));
@@ -1447,14 +1469,42 @@ int aa_collide_species(lbm_ctx* c)
 {
     HIP_TRY(c, hipEventSynchronize(c->ev_save));
     const int64_t found = *c->low_count_host;
-    if (found > c->low_cap) {
-        if (int rc = aa_grow(c, found + found / 4 + 4096)) return rc;
-        int rc = DISPATCH(c, launch(c, "mc_aa_save", [&] { launch_mc_aa_save<P>(make_mview<P>(c, 0, 0), c->aa_first, c->stream); }));
-        if (rc) return rc;
+    const int ns = (int)c->ns;
+    auto save = [&](int a0, int na) {
+        return DISPATCH(c, launch(c, "mc_aa_save", [&] {
+            launch_mc_aa_save<P>(make_mview<P>(c, 0, 0), c->aa_first, a0, na, c->stream);
+        }));
+    };
+    auto collide = [&](int a0, int na, bool side) {
+        return DISPATCH(c, launch(c, "mc_collide_species_inplace", [&] {
+            launch_mc_aa_collide<P>(make_mview<P>(c, 0, 0), a0, na, side, c->stream);
+        }));
+    };
+    if (found <= c->low_cap || low_list_fits(c, found + found / 4 + 4096)) {
+        if (found > c->low_cap) {
+            if (int rc = aa_grow(c, found + found / 4 + 4096)) return rc;
+            if (int rc = save(0, ns)) return rc;
+        }
+        if (int rc = collide(0, ns, true)) return rc;
+    } else {
+        // every species' floor pairs at once would not fit: save and collide
+        // one species at a time (a species' kept values sit only in its own
+        // slots, which only its own collision overwrites), then stream the
+        // side cells of all
+        for (int a = 0; a < ns; ++a) {
+            if (int rc = save(a, 1)) return rc;
+            HIP_TRY(c, hipMemcpyAsync(c->low_count_host, c->low_count, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                      c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            const int64_t fa = *c->low_count_host;
+            if (fa > c->low_cap) {
+                if (int rc = aa_grow(c, fa + fa / 4 + 4096)) return rc;
+                if (int rc = save(a, 1)) return rc;
+            }
+            if (int rc = collide(a, 1, false)) return rc;
+        }
+        if (int rc = collide(0, 0, true)) return rc;
     }
-    int rc = DISPATCH(c, launch(c, "mc_collide_species_inplace",
-                                [&] { launch_mc_aa_collide<P>(make_mview<P>(c, 0, 0), c->stream); }));
-    if (rc) return rc;
     c->aa_swapped = !c->aa_swapped;
     c->aa_first = false;
     c->pushed = true;

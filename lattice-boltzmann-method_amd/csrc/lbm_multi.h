@@ -96,10 +96,13 @@ void launch_mc(McKernel k, const MultiView<P>& v, hipStream_t s);
 // 0).  *v.low_count is zeroed first on the stream.  launch_mc_aa_collide then
 // collides every pair (the listed ones from their saved values, side cells
 // into the side store) and writes each result where the other layout keeps it.
+// Both cover the species a0 .. a0 + na - 1 (all NS, or one at a time when
+// the list of every species' floor pairs would not fit in device memory);
+// with `side`, launch_mc_aa_collide then streams the side cells.
 template <typename P>
-void launch_mc_aa_save(const MultiView<P>& v, bool first, hipStream_t s);
+void launch_mc_aa_save(const MultiView<P>& v, bool first, int a0, int na, hipStream_t s);
 template <typename P>
-void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s);
+void launch_mc_aa_collide(const MultiView<P>& v, int a0, int na, bool side, hipStream_t s);
 // The moment pass and TMS_BC read the species populations in the layout
 // v.swapped; the energy populations stream as in the fused mode.
 

@@ -900,10 +900,10 @@ __device__ inline int64_t aa_plain_dst(const Dims& d, const Wrap3& w, int c, int
 }
 
 template <typename P, int SW>
-__global__ __launch_bounds__(BLOCK) void k_mc_aa_save(MultiView<P> v, int first)
+__global__ __launch_bounds__(BLOCK) void k_mc_aa_save(MultiView<P> v, int first, int a0)
 {
     const int c = xcd_block() * BLOCK + threadIdx.x;
-    const int a = blockIdx.y;
+    const int a = a0 + (int)blockIdx.y;
     const int64_t n = v.d.n;
     const bool listed = c < n && v.type[c] == TYPE_F && (v.nbflag[c] & NB_PUSH) && v.rhoa[a * n + c] <= SPECIES_MIN;
     // wavefront-aggregated append
@@ -931,10 +931,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc_aa_save(MultiView<P> v, int first)
 
 // grid: xcd_species_grid(n, BLOCK, NS).  Plain cells (not listed) and side cells.
 template <typename P, int SW>
-__global__ __launch_bounds__(BLOCK) void k_mc_collide_species_aa(MultiView<P> v)
+__global__ __launch_bounds__(BLOCK) void k_mc_collide_species_aa(MultiView<P> v, int a0, int na)
 {
     int cb, a;
-    xcd_species_block(NS, cb, a);
+    xcd_species_block(na, cb, a);
+    a += a0;
     const int64_t c64 = (int64_t)cb * BLOCK + threadIdx.x;
     if (c64 >= v.d.n) return;
     const int c = (int)c64;
@@ -2694,15 +2695,15 @@ static int cu_count()
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
 template <typename P>
-void launch_mc_aa_save(const MultiView<P>& v, bool first, hipStream_t s)
+void launch_mc_aa_save(const MultiView<P>& v, bool first, int a0, int na, hipStream_t s)
 {
     if constexpr (!kernels_built<P>()) {
         return;  // refused before launch (lbm_create, surrogate entry points)
     } else {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, v.low_count);
-        const dim3 g(xcd_grid(v.d.n, BLOCK), NS);
-        if (v.swapped) hipLaunchKernelGGL((k_mc_aa_save<P, 1>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
-        else hipLaunchKernelGGL((k_mc_aa_save<P, 0>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0);
+        const dim3 g(xcd_grid(v.d.n, BLOCK), na);
+        if (v.swapped) hipLaunchKernelGGL((k_mc_aa_save<P, 1>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0, a0);
+        else hipLaunchKernelGGL((k_mc_aa_save<P, 0>), g, dim3(BLOCK), 0, s, v, first ? 1 : 0, a0);
     }
 }
 
@@ -2730,21 +2731,25 @@ void launch_mc_smoothing(const MultiView<P>& v, void* tmp, double alpha, hipStre
 }
 
 template <typename P>
-void launch_mc_aa_collide(const MultiView<P>& v, hipStream_t s)
+void launch_mc_aa_collide(const MultiView<P>& v, int a0, int na, bool side, hipStream_t s)
 {
     if constexpr (!kernels_built<P>()) {
         return;  // refused before launch (lbm_create, surrogate entry points)
     } else {
-        const dim3 g(xcd_species_grid(v.d.n, BLOCK, NS)), gl(4 * cu_count()),
+        const dim3 g(xcd_species_grid(v.d.n, BLOCK, na)), gl(4 * cu_count()),
             gs((unsigned)((v.nside + BLOCK - 1) / BLOCK), NS);
         if (v.swapped) {
-            hipLaunchKernelGGL((k_mc_collide_species_aa<P, 1>), g, dim3(BLOCK), 0, s, v);
-            hipLaunchKernelGGL((k_mc_collide_species_low<P, 1>), gl, dim3(BLOCK), 0, s, v);
-            if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 1>), gs, dim3(BLOCK), 0, s, v);
+            if (na > 0) {
+                hipLaunchKernelGGL((k_mc_collide_species_aa<P, 1>), g, dim3(BLOCK), 0, s, v, a0, na);
+                hipLaunchKernelGGL((k_mc_collide_species_low<P, 1>), gl, dim3(BLOCK), 0, s, v);
+            }
+            if (side && v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 1>), gs, dim3(BLOCK), 0, s, v);
         } else {
-            hipLaunchKernelGGL((k_mc_collide_species_aa<P, 0>), g, dim3(BLOCK), 0, s, v);
-            hipLaunchKernelGGL((k_mc_collide_species_low<P, 0>), gl, dim3(BLOCK), 0, s, v);
-            if (v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 0>), gs, dim3(BLOCK), 0, s, v);
+            if (na > 0) {
+                hipLaunchKernelGGL((k_mc_collide_species_aa<P, 0>), g, dim3(BLOCK), 0, s, v, a0, na);
+                hipLaunchKernelGGL((k_mc_collide_species_low<P, 0>), gl, dim3(BLOCK), 0, s, v);
+            }
+            if (side && v.nside > 0) hipLaunchKernelGGL((k_mc_side_push_aa<P, 0>), gs, dim3(BLOCK), 0, s, v);
         }
     }
 }
@@ -2780,10 +2785,10 @@ template void launch_mc_diag_mach<double>(const MultiView<double>&, double*, hip
 template void launch_mc_diag_mach<float>(const MultiView<float>&, double*, hipStream_t);
 template void launch_mc_smoothing<double>(const MultiView<double>&, void*, double, hipStream_t);
 template void launch_mc_smoothing<float>(const MultiView<float>&, void*, double, hipStream_t);
-template void launch_mc_aa_save<double>(const MultiView<double>&, bool, hipStream_t);
-template void launch_mc_aa_save<float>(const MultiView<float>&, bool, hipStream_t);
-template void launch_mc_aa_collide<double>(const MultiView<double>&, hipStream_t);
-template void launch_mc_aa_collide<float>(const MultiView<float>&, hipStream_t);
+template void launch_mc_aa_save<double>(const MultiView<double>&, bool, int, int, hipStream_t);
+template void launch_mc_aa_save<float>(const MultiView<float>&, bool, int, int, hipStream_t);
+template void launch_mc_aa_collide<double>(const MultiView<double>&, int, int, bool, hipStream_t);
+template void launch_mc_aa_collide<float>(const MultiView<float>&, int, int, bool, hipStream_t);
 template void launch_mc_tms<float>(const MultiView<float>&, const int*, const int*, int, double*, hipStream_t);
 template void launch_mc_sur_chemistry<float>(const MultiView<float>&, const sur::Mlp&, uint16_t*, float*,
                                              uint8_t*, int64_t, unsigned*, unsigned*, hipStream_t);

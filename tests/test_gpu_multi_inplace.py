@@ -62,6 +62,19 @@ def test_inplace_low_species_list_grows():
     compare(fl, 4, reaction=False, tag="floor species everywhere")
 
 
+@pytest.mark.parametrize("precision", [LBM_FP64, LBM_FP32_SHIFTED])
+def test_inplace_species_rounds_when_the_floor_list_does_not_fit(precision, monkeypatch):
+    """When the list of every species' floor pairs would not fit in device
+    memory, the in-place collide saves and collides one species at a time
+    (LBM_AMD_LOW_MAX caps the list below the box's 6 x 120 floor pairs, above
+    one species' 120): still bitwise the strict calls, with and without
+    reaction."""
+    monkeypatch.setenv("LBM_AMD_LOW_MAX", "300")
+    fl = cases.reacting_box(6, 5, 4, T_K=1300.0, amp_T=0.1, amp_u=0.01, X={"H2": 0.3, "O2": 0.15, "N2": 0.55})
+    compare(fl, 4, precision=precision, reaction=False, tag="floor list in species rounds")
+    compare(fl, 4, precision=precision, reaction=True, tag="floor list in species rounds, reacting")
+
+
 def test_inplace_fp32_mixing_layer_matches_strict():
     compare(cases.mixing_layer(6, 12, 5, T_fuel=300.0, T_ox=1200.0, U_s=0.02, delta=3.0), 5,
             precision=LBM_FP32_SHIFTED, tag="mixing layer fp32")
